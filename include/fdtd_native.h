@@ -1,0 +1,111 @@
+/*
+ * Device-resident simulation handle (extension; no single reference
+ * counterpart).
+ *
+ * The reference drives a run through the low-level calls in a host loop
+ * (engine/fdtd/simulate.c:48-54; validate_probe_dft.c:66-82): H update,
+ * source_apply at t = n dt, E update, then probes at t = (n+1) dt.  This
+ * handle runs that same step order with every field, CPML memory, ADE
+ * state, probe and DFT accumulator resident on the GPU, in FP64 (bitwise
+ * the reference arithmetic) or FP32 fields with FP64 DFT accumulators.
+ *
+ * Lifetime: fdtd_sim_create snapshots the grid's current host fields, the
+ * CPML profiles and the materials; later changes to those host objects are
+ * not seen (use fdtd_sim_upload for fields).  All functions return 0 (or a
+ * non-negative id) on success and -1 on invalid arguments or device errors.
+ */
+#ifndef RFSYN_FDTD_NATIVE_H
+#define RFSYN_FDTD_NATIVE_H
+
+#include <stddef.h>
+#include "grid.h"
+#include "cpml.h"
+#include "materials.h"
+#include "source.h"
+#include "field_component.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdtd_sim fdtd_sim_t;
+
+typedef enum {
+    FDTD_FP64 = 0,   /* fields, CPML and ADE in double: the parity path */
+    FDTD_FP32 = 1    /* fields, CPML and ADE in float; DFT and energy sums in double */
+} fdtd_precision_t;
+
+typedef struct {
+    fdtd_precision_t precision;
+    int dft_compensated;   /* 1: Neumaier-compensated double DFT sums;
+                              0: plain sums in the reference order      */
+    size_t dft_block;      /* steps buffered between DFT flushes (0: 32) */
+} fdtd_sim_options_t;
+
+fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml,
+                            const materials_t *mat,
+                            const fdtd_sim_options_t *opt);
+void fdtd_sim_destroy(fdtd_sim_t *s);
+
+/* Point source, applied as source_apply(src, g, n*dt) between the H and E
+ * updates of step n.  The waveform is evaluated on the host. */
+int fdtd_sim_add_point_source(fdtd_sim_t *s, const source_t *src);
+
+/* Plane source on the E component `component` over the plane
+ * {axis coordinate == index}, on every cell of that plane the E update
+ * covers.  after_e = 0: applied like source_apply at t = n dt before the
+ * E update; after_e = 1: added after the E update at t = (n+1) dt (the
+ * injection of validation_common.h:16-18, 173). */
+int fdtd_sim_add_plane_source(fdtd_sim_t *s, field_component_t component,
+                              int axis, size_t index, source_mode_t mode,
+                              int after_e, double amplitude,
+                              source_waveform_fn waveform,
+                              const void *params);
+
+/* Point probe on any component, read after the E update of each step.
+ * td_capacity: number of time-domain samples kept (0: none).
+ * freqs_hz/n_freqs: running DFT at t = (n+1) dt, as probe_dft_record with
+ * the caller of validate_probe_dft.c:77-80.  Returns the probe id. */
+int fdtd_sim_add_probe(fdtd_sim_t *s, field_component_t component,
+                       size_t i, size_t j, size_t k, size_t td_capacity,
+                       const double *freqs_hz, size_t n_freqs);
+
+/* Full-plane DFT monitor of the two E components tangential to the plane
+ * {axis coordinate == index}, over [0,n_a) x [0,n_b) of the other axes in
+ * increasing axis order, at t = (n+1) dt.  Returns the monitor id. */
+int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index,
+                               const double *freqs_hz, size_t n_freqs);
+
+/* Advance n_steps full steps. */
+int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps);
+size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
+
+/* Copies min(n, recorded) time-domain samples; returns the count or -1. */
+long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n);
+/* Accumulated DFT (acc_re, acc_im exactly as probe_dft_t holds them). */
+int fdtd_sim_probe_dft(fdtd_sim_t *s, int probe, double *re, double *im);
+/* Plane DFT of tangential component `which` (0: lower axis, 1: upper);
+ * layout [f][a][b]. */
+int fdtd_sim_plane_dft(fdtd_sim_t *s, int monitor, int which,
+                       double *re, double *im);
+
+/* Device fields <-> the grid's host arrays (converted to/from double). */
+int fdtd_sim_download(fdtd_sim_t *s, fdtd_grid_t *g);
+int fdtd_sim_upload(fdtd_sim_t *s, const fdtd_grid_t *g);
+
+/* Electric and magnetic energy, 0.5 sum eps E^2 dV and 0.5 sum mu0 H^2 dV
+ * over the cells each component updates, in double with a fixed
+ * reduction order (eps from the scalar coefficient of each edge). */
+int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic);
+
+/* Bytes of device memory held by the handle. */
+size_t fdtd_sim_device_bytes(const fdtd_sim_t *s);
+
+/* Last error message of this thread ("" when none). */
+const char *fdtd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,211 @@
+/*
+ * Thin C ABI between the C host code of librfsyn_fdtd and its HIP kernels.
+ *
+ * Everything here is plain C: device buffers are opaque pointers, kernel
+ * arguments are plain structs passed by value.  The HIP side (*.hip) owns
+ * the runtime calls; the host side (*.c) never includes HIP headers.
+ */
+#ifndef RFSYN_RF_DEVICE_H
+#define RFSYN_RF_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ runtime */
+
+typedef struct rf_stream rf_stream_t;      /* wraps one hipStream_t */
+typedef struct rf_graph rf_graph_t;        /* wraps one hipGraphExec_t */
+
+int rf_init(void);                          /* selects device 0; 0 on success */
+int rf_malloc(void **p, size_t bytes);      /* zero-filled device memory */
+void rf_free(void *p);
+int rf_host_alloc(void **p, size_t bytes);  /* pinned host memory, zero-filled */
+void rf_host_free(void *p);
+int rf_h2d(void *dst, const void *src, size_t bytes, rf_stream_t *s);
+int rf_d2h(void *dst, const void *src, size_t bytes, rf_stream_t *s);
+int rf_d2d(void *dst, const void *src, size_t bytes, rf_stream_t *s);
+int rf_memset(void *dst, int v, size_t bytes, rf_stream_t *s);
+rf_stream_t *rf_stream_create(void);
+void rf_stream_destroy(rf_stream_t *s);
+int rf_sync(rf_stream_t *s);
+int rf_graph_begin(rf_stream_t *s);
+int rf_graph_end(rf_stream_t *s, rf_graph_t **out);
+int rf_graph_launch(rf_graph_t *g, rf_stream_t *s);
+void rf_graph_destroy(rf_graph_t *g);
+const char *rf_error_string(void);          /* last runtime error text */
+
+/* ----------------------------------------------------------- geometry */
+
+/* Padded reference layout: idx = i*sx + j*sy + k, sy = nz+1,
+ * sx = (ny+1)*(nz+1). */
+typedef struct {
+    int n[3];           /* nx, ny, nz */
+    int per;            /* periodic mask, x=1 y=2 z=4 */
+    long long sx, sy;   /* plane and row strides in elements */
+} rf_geom_t;
+
+/* CPML on the device.  psi memories exist only in slabs: along axis a,
+ * coordinate q is in the slab when q < lo[a] or q >= hi[a], and is stored
+ * at compressed coordinate q (q < lo) or lo + q - hi (q >= hi).  A slab
+ * array for derivative axis a has the padded extents of the grid with
+ * axis a replaced by len[a] = lo[a] + (n[a] + 1 - hi[a]). */
+typedef struct {
+    const void *kinv_m[3], *b_m[3], *a_m[3];    /* n+1 entries per axis */
+    const void *kinv_h[3], *b_h[3], *a_h[3];    /* n entries per axis */
+    int lo[3], hi[3], len[3];
+    /* psi per component c and derivative s (axis (c+1+s)%3) */
+    void *psi_h[3][2];
+    void *psi_e[3][2];
+} rf_cpml_t;
+
+/* Material entry flags per component. */
+#define RF_EDGE_TENSOR 1u       /* updated by the tensor kernel: skip */
+#define RF_EDGE_ADE    2u       /* dispersive: ADE update */
+
+/* Per-tag material entry (one per distinct combination of the three
+ * component coefficients and flags), in the field precision. */
+typedef struct {
+    double Ca[3], Cb[3];
+    uint32_t flags[3];
+    int ade[3];                 /* ADE coefficient set per component, -1 none */
+} rf_entry64_t;
+
+typedef struct {
+    float Ca[3], Cb[3];
+    uint32_t flags[3];
+    int ade[3];
+} rf_entry32_t;
+
+#define RF_MAX_SLOTS 8
+/* Trapezoidal-ADE coefficients of one dispersive material. */
+typedef struct {
+    int n_poles;
+    int has_p[RF_MAX_SLOTS];
+    double kj[RF_MAX_SLOTS], kp[RF_MAX_SLOTS], v[RF_MAX_SLOTS];
+    double Cu, hdt;
+} rf_ade64_t;
+
+typedef struct {
+    int n_poles;
+    int has_p[RF_MAX_SLOTS];
+    float kj[RF_MAX_SLOTS], kp[RF_MAX_SLOTS], v[RF_MAX_SLOTS];
+    float Cu, hdt;
+} rf_ade32_t;
+
+typedef struct {
+    const void *tag;            /* per padded node; NULL = vacuum */
+    int tag_bytes;              /* 1, 2 or 4 */
+    const void *entries;        /* rf_entry64_t or rf_entry32_t */
+    const void *ade;            /* rf_ade64_t or rf_ade32_t */
+    /* ADE state box [b0, b0+bn) per axis; state arrays [slot][c][cell] */
+    int b0[3], bn[3];
+    int n_slots;
+    void *J, *P;                /* n_slots * 3 * cells each */
+    double vac_ca;              /* vacuum Cb = dt/(eps0 dx) when tag == NULL */
+} rf_mat_t;
+
+/* One tensor edge: position, component, kind and neighbour weights.
+ * Neighbours follow the order of the reference pair rule: for each other
+ * component d in increasing order, the 4 d-edges at x + s_c e_c - s_d e_d
+ * with the offset along the lower-numbered axis varying fastest; a zero
+ * mask bit means "no tensor there". */
+typedef struct {
+    int i, j, k;
+    int comp;
+    int kind;                   /* 2 or 3 */
+    uint32_t mask;              /* bit 4*slot + q: neighbour present */
+    double diag, scale;
+    double w[8];                /* [slot*4 + q], slot 0/1 = the two d's */
+} rf_tedge64_t;
+
+typedef struct {
+    int i, j, k;
+    int comp;
+    int kind;
+    uint32_t mask;
+    float diag, scale;
+    float w[8];
+} rf_tedge32_t;
+
+/* Sources and probes of the device-resident loop. */
+typedef struct {
+    long long idx;              /* padded index */
+    int comp;                   /* 0..2 */
+    int hard;
+    const double *wave;         /* value per absolute step */
+} rf_point_src_t;
+
+typedef struct {
+    int comp, axis, index, hard;
+    const double *wave;         /* value per absolute step */
+} rf_plane_src_t;
+
+/* ----------------------------------------------------------- launches
+ * `prec` selects the field precision: 0 = double, 1 = float.  All
+ * launches go to stream s and return 0 or -1. */
+
+int rf_launch_update_h(int prec, const rf_geom_t *g, void *const f[6], double cb, double dx,
+                       const rf_cpml_t *cpml, rf_stream_t *s);
+int rf_launch_update_e(int prec, const rf_geom_t *g, void *const f[6], double dx,
+                       const rf_cpml_t *cpml, const rf_mat_t *mat, rf_stream_t *s);
+int rf_launch_tensor(int prec, const rf_geom_t *g, void *const f[6], const void *edges,
+                     long long n_edges, rf_stream_t *s);
+
+/* Point sources at step *d_step (before E). */
+int rf_launch_point_sources(int prec, void *const f[6], const rf_point_src_t *srcs_dev, int n,
+                            const long long *d_step, int step_offset, rf_stream_t *s);
+/* Plane source over [0,n_a) x [0,n_b) of the plane; step index
+ * *d_step + step_offset selects the waveform value. */
+int rf_launch_plane_source(int prec, const rf_geom_t *g, void *const f[6], const rf_plane_src_t *p,
+                           const long long *d_step, int step_offset, rf_stream_t *s);
+
+/* Probe gather: ring[(step % ring_len) * n + p] = field value (as double);
+ * td[p * td_cap + step] when step < td_cap.  Then increments *d_step. */
+int rf_launch_probe_record(int prec, void *const f[6], const long long *idx_dev, const int *comp_dev,
+                           int n, double *ring, int ring_len, double *td, long long td_cap,
+                           long long *d_step, rf_stream_t *s);
+
+/* Point-probe DFT over the last `count` ring entries, ending at step
+ * *d_step - 1.  One accumulator per flattened (probe, frequency) pair t:
+ * probe freq_probe[t], angular frequency freq_w[t] (rad/s):
+ *   acc_re += v cos(w t_n) dt,  acc_im -= v sin(w t_n) dt,  t_n = (n+1) dt
+ * (probe.c:87-91).  comp_sum != 0: Neumaier-compensated sums. */
+int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const int *freq_probe,
+                        int n_freq, const double *freq_w, double dt, const long long *d_step,
+                        int count, double *acc_re, double *acc_im, double *cmp_re, double *cmp_im,
+                        int comp_sum, rf_stream_t *s);
+
+/* Plane monitor capture: copies component comp on the plane into
+ * buf[(*d_step % ring_len) * na * nb + a * nb + b] (buf_prec == prec). */
+int rf_launch_plane_capture(int prec, const rf_geom_t *g, void *const f[6], int axis, int index,
+                            int comp, void *buf, int buf_prec, int ring_len,
+                            const long long *d_step, rf_stream_t *s);
+
+/* Plane monitor DFT flush over `count` (<= 64) buffered steps ending at
+ * step *d_step - 1; accumulators [f][cell]; tw_scratch holds
+ * 2 * nf * count doubles. */
+int rf_launch_plane_dft(const void *buf, int buf_prec, long long cells, int ring_len, int nf,
+                        const double *freq_w, double dt, const long long *d_step, int count,
+                        double *tw_scratch, double *acc_re, double *acc_im, double *cmp_re,
+                        double *cmp_im, int comp_sum, rf_stream_t *s);
+
+/* Energy partial sums (double): out[0] = sum 0.5 eps E^2, out[1] = sum
+ * 0.5 mu0 H^2 over i<nx, j<ny, k<nz, unscaled by dx^3.  eps of each edge
+ * is dt/(dx Cb) from cb64 (per tag entry, per component) or vac_eps. */
+int rf_launch_energy(int prec, const rf_geom_t *g, void *const f[6], const void *tag, int tag_bytes,
+                     const double *eps_tab, double vac_eps, double mu0, double *partials,
+                     int n_partials, double *out, rf_stream_t *s);
+
+/* Precision conversion between the double host layout and device fields. */
+int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,
+                      rf_stream_t *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

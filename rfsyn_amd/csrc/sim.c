@@ -1,0 +1,690 @@
+/*
+ * Device-resident simulation handle (fdtd_native.h) and the one-call
+ * entry point fdtd_simulate_echo (engine/fdtd/simulate.c:10-69).
+ *
+ * One step on the GPU stream, in the reference order (simulate.c:48-54,
+ * validate_probe_dft.c:66-82):
+ *   H kernel -> point/plane sources at t = n dt -> E kernel (+ tensor
+ *   edges) -> post-E plane sources at t = (n+1) dt -> plane-monitor
+ *   capture -> probe gather, which also advances the device step counter.
+ * Every dft_block steps the buffered probe and plane values are folded
+ * into the FP64 DFT accumulators.  A block of dft_block steps plus its
+ * flush is captured once as a HIP graph and replayed, so the host issues
+ * one launch per block.  Kernels read the step index from device memory,
+ * which is what makes the captured block replayable.
+ */
+#include "host_internal.h"
+#include "fdtd_native.h"
+#include "simulate.h"
+#include "source.h"
+#include "constants.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct {
+    source_waveform_fn wf;
+    const void *params;
+    double amp;
+    int after_e;          /* table holds amp*wf((n+1)dt) instead of amp*wf(n dt) */
+    double *d_tab;        /* device, wave_cap entries */
+} wave_t;
+
+typedef struct {
+    int comp;
+    long long idx;
+    size_t td_cap;
+    size_t nf, f_off;
+} sprobe_t;
+
+typedef struct {
+    int axis, index, comp[2], na, nb, nf;
+    double *freqs;
+    double *d_w;
+    void *ring[2];
+    double *acc_re[2], *acc_im[2], *cmp_re[2], *cmp_im[2];
+} smon_t;
+
+typedef struct {
+    rf_plane_src_t d;
+    int wave;             /* index into waves */
+    int after_e;
+} splane_t;
+
+struct fdtd_sim {
+    int prec;
+    size_t es;
+    size_t nx, ny, nz;
+    double dx, dt;
+    unsigned per;
+    rf_geom_t geom;
+    size_t cells;
+    rf_stream_t *st;
+    void *f[6];
+    int has_cpml;
+    rf_cpml_dev_t cp;
+    int has_mat;
+    rf_mat_dev_t mat;
+    rf_mat_t vac;
+    long long *d_step;
+    size_t steps;
+    int comp_sum;
+    int K;
+    int pending;
+    int started;
+
+    wave_t *waves; size_t n_waves;
+    size_t wave_cap;
+    int *pt_wave; source_t *pts; size_t n_pts;
+    rf_point_src_t *d_pts;
+    splane_t *planes; size_t n_planes;
+
+    sprobe_t *probes; size_t n_probes;
+    double *freqs_all; size_t n_freq_tot;
+    long long *d_pidx; int *d_pcomp;
+    double *d_ring, *d_td;
+    size_t td_stride;
+    int *d_fprobe; double *d_fw;
+    double *d_acc_re, *d_acc_im, *d_cmp_re, *d_cmp_im;
+
+    smon_t *mons; size_t n_mons;
+    double *d_tw;
+
+    double *d_energy;     /* partials + 2 results */
+    int n_partials;
+
+    rf_graph_t *graph;
+    size_t bytes;
+};
+
+static int dalloc(fdtd_sim_t *s, void **p, size_t bytes) {
+    if (rf_malloc(p, bytes) != 0) {
+        rf_set_error("device allocation of %zu bytes failed: %s", bytes, rf_error_string());
+        return -1;
+    }
+    s->bytes += bytes;
+    return 0;
+}
+
+static int upload_fields(fdtd_sim_t *s, const fdtd_grid_t *g) {
+    const double *src[6] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz};
+    size_t bytes = s->cells * sizeof(double);
+    void *tmp = NULL;
+    if (s->prec && rf_malloc(&tmp, bytes) != 0) return -1;
+    for (int q = 0; q < 6; q++) {
+        int rc;
+        if (s->prec == 0) {
+            rc = rf_h2d(s->f[q], src[q], bytes, s->st);
+        } else {
+            rc = rf_h2d(tmp, src[q], bytes, s->st);
+            if (!rc) rc = rf_launch_convert(s->f[q], 1, tmp, 0, (long long)s->cells, s->st);
+        }
+        if (!rc) rc = rf_sync(s->st);
+        if (rc) { rf_free(tmp); return -1; }
+    }
+    rf_free(tmp);
+    return 0;
+}
+
+fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml, const materials_t *mat,
+                            const fdtd_sim_options_t *opt) {
+    if (!g) { rf_set_error("NULL grid"); return NULL; }
+    if (pml && !rf_cpml_matches(pml, g)) { rf_set_error("CPML was built for another grid"); return NULL; }
+    if (mat && !rf_materials_matches(mat, g)) { rf_set_error("materials were built for another grid"); return NULL; }
+    fdtd_sim_options_t o = {FDTD_FP64, 0, 0};
+    if (opt) o = *opt;
+    if (o.precision != FDTD_FP64 && o.precision != FDTD_FP32) { rf_set_error("bad precision"); return NULL; }
+    if (rf_init() != 0) { rf_set_error("no GPU: %s", rf_error_string()); return NULL; }
+    fdtd_sim_t *s = calloc(1, sizeof *s);
+    if (!s) return NULL;
+    s->prec = o.precision == FDTD_FP32 ? 1 : 0;
+    s->es = s->prec ? sizeof(float) : sizeof(double);
+    s->nx = g->nx; s->ny = g->ny; s->nz = g->nz;
+    s->dx = g->dx; s->dt = g->dt;
+    s->per = g->periodic_axes;
+    s->geom = rf_geom_of(g);
+    s->cells = rf_cells(g);
+    s->comp_sum = o.dft_compensated ? 1 : 0;
+    s->K = o.dft_block ? (int)o.dft_block : 32;
+    if (s->K > 64) s->K = 64;
+    s->st = rf_stream_create();
+    if (!s->st) goto fail;
+    for (int q = 0; q < 6; q++)
+        if (dalloc(s, &s->f[q], s->cells * s->es)) goto fail;
+    if (upload_fields(s, g)) goto fail;
+    if (pml) {
+        if (rf_cpml_build(pml, s->prec, &s->cp) != 0) goto fail;
+        s->has_cpml = 1;
+        s->bytes += s->cp.bytes;
+    }
+    if (mat) {
+        if (rf_materials_build(mat, g, s->prec, &s->mat) != 0) goto fail;
+        s->has_mat = 1;
+        s->bytes += s->mat.bytes;
+    }
+    memset(&s->vac, 0, sizeof s->vac);
+    s->vac.vac_ca = g->dt / (EPS0 * g->dx);
+    if (dalloc(s, (void **)&s->d_step, sizeof(long long))) goto fail;
+    s->n_partials = 1024;
+    if (dalloc(s, (void **)&s->d_energy, (2 * (size_t)s->n_partials + 2) * sizeof(double))) goto fail;
+    return s;
+fail:
+    if (!fdtd_last_error()[0]) rf_set_error("fdtd_sim_create: %s", rf_error_string());
+    fdtd_sim_destroy(s);
+    return NULL;
+}
+
+void fdtd_sim_destroy(fdtd_sim_t *s) {
+    if (!s) return;
+    if (s->st) rf_sync(s->st);
+    rf_graph_destroy(s->graph);
+    for (int q = 0; q < 6; q++) rf_free(s->f[q]);
+    if (s->has_cpml) rf_cpml_release(&s->cp);
+    if (s->has_mat) rf_materials_release(&s->mat);
+    for (size_t w = 0; w < s->n_waves; w++) rf_free(s->waves[w].d_tab);
+    free(s->waves);
+    free(s->pt_wave);
+    free(s->pts);
+    rf_free(s->d_pts);
+    free(s->planes);
+    free(s->probes);
+    free(s->freqs_all);
+    rf_free(s->d_pidx); rf_free(s->d_pcomp); rf_free(s->d_ring); rf_free(s->d_td);
+    rf_free(s->d_fprobe); rf_free(s->d_fw);
+    rf_free(s->d_acc_re); rf_free(s->d_acc_im); rf_free(s->d_cmp_re); rf_free(s->d_cmp_im);
+    for (size_t m = 0; m < s->n_mons; m++) {
+        smon_t *mo = &s->mons[m];
+        free(mo->freqs);
+        rf_free(mo->d_w);
+        for (int w = 0; w < 2; w++) {
+            rf_free(mo->ring[w]);
+            rf_free(mo->acc_re[w]); rf_free(mo->acc_im[w]);
+            rf_free(mo->cmp_re[w]); rf_free(mo->cmp_im[w]);
+        }
+    }
+    free(s->mons);
+    rf_free(s->d_tw);
+    rf_free(s->d_step);
+    rf_free(s->d_energy);
+    rf_stream_destroy(s->st);
+    free(s);
+}
+
+static int add_wave(fdtd_sim_t *s, source_waveform_fn wf, const void *params, double amp, int after_e) {
+    wave_t *w = realloc(s->waves, (s->n_waves + 1) * sizeof *w);
+    if (!w) return -1;
+    s->waves = w;
+    wave_t nw = {wf, params, amp, after_e, NULL};
+    s->waves[s->n_waves] = nw;
+    return (int)s->n_waves++;
+}
+
+int fdtd_sim_add_point_source(fdtd_sim_t *s, const source_t *src) {
+    if (!s || !src || !src->waveform) { rf_set_error("bad source"); return -1; }
+    if (s->started) { rf_set_error("sources must be added before the first run"); return -1; }
+    if (src->component > FIELD_EZ) { rf_set_error("sources drive E components only"); return -1; }
+    if (src->i > s->nx || src->j > s->ny || src->k > s->nz) { rf_set_error("source outside the grid"); return -1; }
+    int w = add_wave(s, src->waveform, src->params, src->amplitude, 0);
+    if (w < 0) return -1;
+    source_t *p = realloc(s->pts, (s->n_pts + 1) * sizeof *p);
+    int *pw = realloc(s->pt_wave, (s->n_pts + 1) * sizeof *pw);
+    if (p) s->pts = p;
+    if (pw) s->pt_wave = pw;
+    if (!p || !pw) return -1;
+    s->pts[s->n_pts] = *src;
+    s->pt_wave[s->n_pts] = w;
+    s->n_pts++;
+    return 0;
+}
+
+int fdtd_sim_add_plane_source(fdtd_sim_t *s, field_component_t component, int axis, size_t index,
+                              source_mode_t mode, int after_e, double amplitude,
+                              source_waveform_fn waveform, const void *params) {
+    if (!s || component > FIELD_EZ || axis < 0 || axis > 2 || !waveform) { rf_set_error("bad plane source"); return -1; }
+    if (s->started) { rf_set_error("sources must be added before the first run"); return -1; }
+    size_t n[3] = {s->nx, s->ny, s->nz};
+    if (index > n[axis]) { rf_set_error("plane outside the grid"); return -1; }
+    int w = add_wave(s, waveform, params, amplitude, after_e ? 1 : 0);
+    if (w < 0) return -1;
+    splane_t *p = realloc(s->planes, (s->n_planes + 1) * sizeof *p);
+    if (!p) return -1;
+    s->planes = p;
+    splane_t pl;
+    memset(&pl, 0, sizeof pl);
+    pl.d.comp = (int)component;
+    pl.d.axis = axis;
+    pl.d.index = (int)index;
+    pl.d.hard = mode == SRC_HARD;
+    pl.wave = w;
+    pl.after_e = after_e ? 1 : 0;
+    s->planes[s->n_planes++] = pl;
+    return 0;
+}
+
+int fdtd_sim_add_probe(fdtd_sim_t *s, field_component_t component, size_t i, size_t j, size_t k,
+                       size_t td_capacity, const double *freqs_hz, size_t n_freqs) {
+    if (!s || component > FIELD_HZ || i > s->nx || j > s->ny || k > s->nz || (n_freqs && !freqs_hz)) {
+        rf_set_error("bad probe");
+        return -1;
+    }
+    if (s->started) { rf_set_error("probes must be added before the first run"); return -1; }
+    sprobe_t *p = realloc(s->probes, (s->n_probes + 1) * sizeof *p);
+    if (!p) return -1;
+    s->probes = p;
+    double *fa = realloc(s->freqs_all, (s->n_freq_tot + n_freqs + 1) * sizeof *fa);
+    if (!fa) return -1;
+    s->freqs_all = fa;
+    memcpy(s->freqs_all + s->n_freq_tot, freqs_hz, n_freqs * sizeof(double));
+    sprobe_t np;
+    np.comp = (int)component;
+    fdtd_grid_t tmp = {0};
+    tmp.nx = s->nx; tmp.ny = s->ny; tmp.nz = s->nz; tmp.periodic_axes = s->per;
+    np.idx = (long long)fdtd_index(&tmp, i, j, k);
+    np.td_cap = td_capacity;
+    np.nf = n_freqs;
+    np.f_off = s->n_freq_tot;
+    s->n_freq_tot += n_freqs;
+    s->probes[s->n_probes] = np;
+    return (int)s->n_probes++;
+}
+
+int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index, const double *freqs_hz,
+                               size_t n_freqs) {
+    if (!s || axis < 0 || axis > 2 || !freqs_hz || !n_freqs) { rf_set_error("bad monitor"); return -1; }
+    if (s->started) { rf_set_error("monitors must be added before the first run"); return -1; }
+    size_t n[3] = {s->nx, s->ny, s->nz};
+    if (index > n[axis]) { rf_set_error("monitor plane outside the grid"); return -1; }
+    if ((size_t)s->K * n_freqs * 16 > 64 * 1024) {
+        rf_set_error("dft_block * n_freqs too large for the plane DFT twiddle block (<= 4096)");
+        return -1;
+    }
+    smon_t *nm = realloc(s->mons, (s->n_mons + 1) * sizeof *nm);
+    if (!nm) return -1;
+    s->mons = nm;
+    smon_t *m = &s->mons[s->n_mons];
+    memset(m, 0, sizeof *m);
+    m->axis = axis;
+    m->index = (int)index;
+    int a = (axis + 1) % 3 < (axis + 2) % 3 ? (axis + 1) % 3 : (axis + 2) % 3;
+    m->comp[0] = a;
+    m->comp[1] = 3 - axis - a;
+    m->na = (int)n[m->comp[0]];
+    m->nb = (int)n[m->comp[1]];
+    m->nf = (int)n_freqs;
+    m->freqs = malloc(n_freqs * sizeof(double));
+    if (!m->freqs) return -1;
+    memcpy(m->freqs, freqs_hz, n_freqs * sizeof(double));
+    s->n_mons++;
+    return (int)s->n_mons - 1;
+}
+
+/* Device arrays for probes and monitors, allocated once at the first run. */
+static int finalize(fdtd_sim_t *s) {
+    int P = (int)s->n_probes;
+    size_t cap = 0;
+    for (size_t p = 0; p < s->n_probes; p++) if (s->probes[p].td_cap > cap) cap = s->probes[p].td_cap;
+    s->td_stride = cap;
+    if (P) {
+        long long *idx = malloc(P * sizeof *idx);
+        int *comp = malloc(P * sizeof *comp);
+        if (!idx || !comp) { free(idx); free(comp); return -1; }
+        for (int p = 0; p < P; p++) { idx[p] = s->probes[p].idx; comp[p] = s->probes[p].comp; }
+        int rc = dalloc(s, (void **)&s->d_pidx, P * sizeof *idx) ||
+                 dalloc(s, (void **)&s->d_pcomp, P * sizeof *comp) ||
+                 rf_h2d(s->d_pidx, idx, P * sizeof *idx, s->st) ||
+                 rf_h2d(s->d_pcomp, comp, P * sizeof *comp, s->st) || rf_sync(s->st);
+        free(idx);
+        free(comp);
+        if (rc) return -1;
+        if (dalloc(s, (void **)&s->d_ring, (size_t)s->K * P * sizeof(double))) return -1;
+        if (cap && dalloc(s, (void **)&s->d_td, cap * P * sizeof(double))) return -1;
+    }
+    if (s->n_freq_tot) {
+        size_t F = s->n_freq_tot;
+        int *fp = malloc(F * sizeof *fp);
+        double *fw = malloc(F * sizeof *fw);
+        if (!fp || !fw) { free(fp); free(fw); return -1; }
+        for (size_t p = 0; p < s->n_probes; p++)
+            for (size_t q = 0; q < s->probes[p].nf; q++) {
+                fp[s->probes[p].f_off + q] = (int)p;
+                fw[s->probes[p].f_off + q] = 2.0 * M_PI * s->freqs_all[s->probes[p].f_off + q];
+            }
+        int rc = dalloc(s, (void **)&s->d_fprobe, F * sizeof *fp) ||
+                 dalloc(s, (void **)&s->d_fw, F * sizeof *fw) ||
+                 dalloc(s, (void **)&s->d_acc_re, F * sizeof(double)) ||
+                 dalloc(s, (void **)&s->d_acc_im, F * sizeof(double)) ||
+                 dalloc(s, (void **)&s->d_cmp_re, F * sizeof(double)) ||
+                 dalloc(s, (void **)&s->d_cmp_im, F * sizeof(double)) ||
+                 rf_h2d(s->d_fprobe, fp, F * sizeof *fp, s->st) ||
+                 rf_h2d(s->d_fw, fw, F * sizeof *fw, s->st) || rf_sync(s->st);
+        free(fp);
+        free(fw);
+        if (rc) return -1;
+    }
+    int max_nf = 0;
+    for (size_t q = 0; q < s->n_mons; q++) {
+        smon_t *m = &s->mons[q];
+        if (m->nf > max_nf) max_nf = m->nf;
+        size_t cells = (size_t)m->na * m->nb;
+        double *fw = malloc(m->nf * sizeof *fw);
+        if (!fw) return -1;
+        for (int f = 0; f < m->nf; f++) fw[f] = 2.0 * M_PI * m->freqs[f];
+        int rc = dalloc(s, (void **)&m->d_w, m->nf * sizeof *fw) ||
+                 rf_h2d(m->d_w, fw, m->nf * sizeof *fw, s->st) || rf_sync(s->st);
+        free(fw);
+        if (rc) return -1;
+        for (int w = 0; w < 2; w++) {
+            size_t acc = (size_t)m->nf * cells * sizeof(double);
+            if (dalloc(s, &m->ring[w], (size_t)s->K * cells * s->es) ||
+                dalloc(s, (void **)&m->acc_re[w], acc) || dalloc(s, (void **)&m->acc_im[w], acc))
+                return -1;
+            if (s->comp_sum &&
+                (dalloc(s, (void **)&m->cmp_re[w], acc) || dalloc(s, (void **)&m->cmp_im[w], acc)))
+                return -1;
+        }
+    }
+    if (max_nf && dalloc(s, (void **)&s->d_tw, (size_t)2 * max_nf * s->K * sizeof(double))) return -1;
+    s->started = 1;
+    return 0;
+}
+
+/* Waveform tables cover steps [0, need); grown geometrically. */
+static int ensure_waves(fdtd_sim_t *s, size_t need) {
+    if (need <= s->wave_cap) return 0;
+    size_t cap = s->wave_cap ? s->wave_cap : 1024;
+    while (cap < need) cap *= 2;
+    double *host = malloc(cap * sizeof *host);
+    if (!host) return -1;
+    for (size_t w = 0; w < s->n_waves; w++) {
+        wave_t *wv = &s->waves[w];
+        for (size_t n = 0; n < cap; n++) {
+            double t = wv->after_e ? (double)(n + 1) * s->dt : (double)n * s->dt;
+            host[n] = wv->amp * wv->wf(t, wv->params);
+        }
+        double *d = NULL;
+        if (rf_malloc((void **)&d, cap * sizeof(double)) != 0 ||
+            rf_h2d(d, host, cap * sizeof(double), s->st) != 0 || rf_sync(s->st) != 0) {
+            rf_free(d);
+            free(host);
+            return -1;
+        }
+        rf_free(wv->d_tab);
+        wv->d_tab = d;
+    }
+    free(host);
+    s->wave_cap = cap;
+    if (s->n_pts) {
+        rf_point_src_t *hp = calloc(s->n_pts, sizeof *hp);
+        if (!hp) return -1;
+        fdtd_grid_t tmp = {0};
+        tmp.nx = s->nx; tmp.ny = s->ny; tmp.nz = s->nz; tmp.periodic_axes = s->per;
+        for (size_t p = 0; p < s->n_pts; p++) {
+            hp[p].idx = (long long)fdtd_index(&tmp, s->pts[p].i, s->pts[p].j, s->pts[p].k);
+            hp[p].comp = (int)s->pts[p].component;
+            hp[p].hard = s->pts[p].mode == SRC_HARD;
+            hp[p].wave = s->waves[s->pt_wave[p]].d_tab;
+        }
+        int rc = 0;
+        if (!s->d_pts) rc = dalloc(s, (void **)&s->d_pts, s->n_pts * sizeof *hp);
+        if (!rc) rc = rf_h2d(s->d_pts, hp, s->n_pts * sizeof *hp, s->st) || rf_sync(s->st);
+        free(hp);
+        if (rc) return -1;
+    }
+    for (size_t p = 0; p < s->n_planes; p++) s->planes[p].d.wave = s->waves[s->planes[p].wave].d_tab;
+    /* plane sources pass their table by value: a captured block is stale */
+    rf_graph_destroy(s->graph);
+    s->graph = NULL;
+    return 0;
+}
+
+static int enqueue_step(fdtd_sim_t *s) {
+    const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
+    const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
+    double cb = s->dt / (MU0 * s->dx);
+    if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
+    if (s->n_pts && rf_launch_point_sources(s->prec, s->f, s->d_pts, (int)s->n_pts, s->d_step, 0, s->st))
+        return -1;
+    for (size_t p = 0; p < s->n_planes; p++)
+        if (!s->planes[p].after_e &&
+            rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+            return -1;
+    if (rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) return -1;
+    if (s->has_mat && s->mat.n_tensor &&
+        rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
+        return -1;
+    for (size_t p = 0; p < s->n_planes; p++)
+        if (s->planes[p].after_e &&
+            rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+            return -1;
+    for (size_t q = 0; q < s->n_mons; q++) {
+        smon_t *m = &s->mons[q];
+        for (int w = 0; w < 2; w++)
+            if (rf_launch_plane_capture(s->prec, &s->geom, s->f, m->axis, m->index, m->comp[w], m->ring[w],
+                                        s->prec, s->K, s->d_step, s->st))
+                return -1;
+    }
+    return rf_launch_probe_record(s->prec, s->f, s->d_pidx, s->d_pcomp, (int)s->n_probes, s->d_ring, s->K,
+                                  s->d_td, (long long)s->td_stride, s->d_step, s->st);
+}
+
+static int enqueue_flush(fdtd_sim_t *s, int count) {
+    if (count <= 0) return 0;
+    if (s->n_freq_tot &&
+        rf_launch_probe_dft(s->d_ring, s->K, (int)s->n_probes, s->d_fprobe, (int)s->n_freq_tot, s->d_fw,
+                            s->dt, s->d_step, count, s->d_acc_re, s->d_acc_im, s->d_cmp_re, s->d_cmp_im,
+                            s->comp_sum, s->st))
+        return -1;
+    for (size_t q = 0; q < s->n_mons; q++) {
+        smon_t *m = &s->mons[q];
+        long long cells = (long long)m->na * m->nb;
+        for (int w = 0; w < 2; w++)
+            if (rf_launch_plane_dft(m->ring[w], s->prec, cells, s->K, m->nf, m->d_w, s->dt, s->d_step, count,
+                                    s->d_tw, m->acc_re[w], m->acc_im[w], m->cmp_re[w], m->cmp_im[w],
+                                    s->comp_sum, s->st))
+                return -1;
+    }
+    return 0;
+}
+
+static int capture_block(fdtd_sim_t *s) {
+    if (rf_graph_begin(s->st)) return -1;
+    int rc = 0;
+    for (int q = 0; q < s->K && !rc; q++) rc = enqueue_step(s);
+    if (!rc) rc = enqueue_flush(s, s->K);
+    rf_graph_t *g = NULL;
+    int rc2 = rf_graph_end(s->st, &g);
+    if (rc || rc2) { rf_graph_destroy(g); return -1; }
+    s->graph = g;
+    return 0;
+}
+
+int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
+    if (!s) return -1;
+    if (!s->started && finalize(s) != 0) {
+        rf_set_error("fdtd_sim_run set-up: %s", rf_error_string());
+        return -1;
+    }
+    if (ensure_waves(s, s->steps + n_steps) != 0) {
+        rf_set_error("waveform tables: %s", rf_error_string());
+        return -1;
+    }
+    size_t left = n_steps;
+    while (left) {
+        if (s->pending == 0 && left >= (size_t)s->K) {
+            if (!s->graph && capture_block(s) != 0) {
+                rf_set_error("graph capture: %s", rf_error_string());
+                return -1;
+            }
+            if (rf_graph_launch(s->graph, s->st)) return -1;
+            s->steps += (size_t)s->K;
+            left -= (size_t)s->K;
+            continue;
+        }
+        if (enqueue_step(s)) return -1;
+        s->steps++;
+        left--;
+        if (++s->pending == s->K) {
+            if (enqueue_flush(s, s->K)) return -1;
+            s->pending = 0;
+        }
+    }
+    if (s->pending) {
+        if (enqueue_flush(s, s->pending)) return -1;
+        s->pending = 0;
+    }
+    if (rf_sync(s->st)) {
+        rf_set_error("fdtd_sim_run: %s", rf_error_string());
+        return -1;
+    }
+    return 0;
+}
+
+size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }
+
+long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n) {
+    if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out)) return -1;
+    size_t have = s->steps < s->probes[probe].td_cap ? s->steps : s->probes[probe].td_cap;
+    size_t c = n < have ? n : have;
+    if (c && rf_d2h(out, s->d_td + (size_t)probe * s->td_stride, c * sizeof(double), s->st)) return -1;
+    return (long)c;
+}
+
+int fdtd_sim_probe_dft(fdtd_sim_t *s, int probe, double *re, double *im) {
+    if (!s || probe < 0 || (size_t)probe >= s->n_probes || !re || !im) return -1;
+    const sprobe_t *p = &s->probes[probe];
+    if (!p->nf) return 0;
+    if (!s->started) { memset(re, 0, p->nf * sizeof *re); memset(im, 0, p->nf * sizeof *im); return 0; }
+    if (rf_d2h(re, s->d_acc_re + p->f_off, p->nf * sizeof(double), s->st) ||
+        rf_d2h(im, s->d_acc_im + p->f_off, p->nf * sizeof(double), s->st))
+        return -1;
+    if (s->comp_sum) {
+        double *c = malloc(p->nf * sizeof *c);
+        if (!c) return -1;
+        if (rf_d2h(c, s->d_cmp_re + p->f_off, p->nf * sizeof(double), s->st)) { free(c); return -1; }
+        for (size_t q = 0; q < p->nf; q++) re[q] += c[q];
+        if (rf_d2h(c, s->d_cmp_im + p->f_off, p->nf * sizeof(double), s->st)) { free(c); return -1; }
+        for (size_t q = 0; q < p->nf; q++) im[q] += c[q];
+        free(c);
+    }
+    return 0;
+}
+
+int fdtd_sim_plane_dft(fdtd_sim_t *s, int monitor, int which, double *re, double *im) {
+    if (!s || monitor < 0 || (size_t)monitor >= s->n_mons || which < 0 || which > 1 || !re || !im) return -1;
+    smon_t *m = &s->mons[monitor];
+    size_t tot = (size_t)m->nf * m->na * m->nb;
+    if (!s->started) { memset(re, 0, tot * sizeof *re); memset(im, 0, tot * sizeof *im); return 0; }
+    if (rf_d2h(re, m->acc_re[which], tot * sizeof(double), s->st) ||
+        rf_d2h(im, m->acc_im[which], tot * sizeof(double), s->st))
+        return -1;
+    if (s->comp_sum) {
+        double *c = malloc(tot * sizeof *c);
+        if (!c) return -1;
+        if (rf_d2h(c, m->cmp_re[which], tot * sizeof(double), s->st)) { free(c); return -1; }
+        for (size_t q = 0; q < tot; q++) re[q] += c[q];
+        if (rf_d2h(c, m->cmp_im[which], tot * sizeof(double), s->st)) { free(c); return -1; }
+        for (size_t q = 0; q < tot; q++) im[q] += c[q];
+        free(c);
+    }
+    return 0;
+}
+
+static int same_shape(const fdtd_sim_t *s, const fdtd_grid_t *g) {
+    return s && g && s->nx == g->nx && s->ny == g->ny && s->nz == g->nz;
+}
+
+int fdtd_sim_download(fdtd_sim_t *s, fdtd_grid_t *g) {
+    if (!same_shape(s, g)) return -1;
+    double *dst[6] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz};
+    size_t bytes = s->cells * sizeof(double);
+    void *tmp = NULL;
+    if (s->prec && rf_malloc(&tmp, bytes) != 0) return -1;
+    for (int q = 0; q < 6; q++) {
+        int rc;
+        if (s->prec == 0) {
+            rc = rf_d2h(dst[q], s->f[q], bytes, s->st);
+        } else {
+            rc = rf_launch_convert(tmp, 0, s->f[q], 1, (long long)s->cells, s->st);
+            if (!rc) rc = rf_d2h(dst[q], tmp, bytes, s->st);
+        }
+        if (rc) { rf_free(tmp); return -1; }
+    }
+    rf_free(tmp);
+    return 0;
+}
+
+int fdtd_sim_upload(fdtd_sim_t *s, const fdtd_grid_t *g) {
+    if (!same_shape(s, g)) return -1;
+    return upload_fields(s, g);
+}
+
+int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic) {
+    if (!s || !electric || !magnetic) return -1;
+    double vac_eps = s->dt / (s->dx * s->vac.vac_ca);
+    const void *tag = s->has_mat ? s->mat.d.tag : NULL;
+    int tb = s->has_mat ? s->mat.d.tag_bytes : 0;
+    const double *eps = s->has_mat ? s->mat.eps_tab : NULL;
+    double out[2];
+    double *d_out = s->d_energy + 2 * (size_t)s->n_partials;
+    if (rf_launch_energy(s->prec, &s->geom, s->f, tag, tb, eps, vac_eps, MU0, s->d_energy, s->n_partials,
+                         d_out, s->st) ||
+        rf_d2h(out, d_out, sizeof out, s->st))
+        return -1;
+    double vol = s->dx * s->dx * s->dx;
+    *electric = out[0] * vol;
+    *magnetic = out[1] * vol;
+    return 0;
+}
+
+size_t fdtd_sim_device_bytes(const fdtd_sim_t *s) { return s ? s->bytes : 0; }
+
+/* ------------------------------------------------- fdtd_simulate_echo */
+
+/* engine/fdtd/simulate.c:10-66 on the device-resident path in FP64. */
+int fdtd_simulate_echo_with_setup(const fdtd_sim_config_t *cfg, double *out_td, double *sample_rate_out,
+                                  fdtd_material_setup_fn setup, void *context) {
+    if (!cfg || !out_td || cfg->n_steps == 0) return -1;
+    fdtd_grid_t *g = fdtd_grid_create(cfg->nx, cfg->ny, cfg->nz, cfg->dx, cfg->courant_safety);
+    if (!g) return -1;
+    int rc = -1;
+    cpml_t *pml = NULL;
+    materials_t *mat = NULL;
+    fdtd_sim_t *s = NULL;
+    if (cfg->src_i > g->nx || cfg->src_j > g->ny || cfg->src_k > g->nz ||
+        cfg->probe_i > g->nx || cfg->probe_j > g->ny || cfg->probe_k > g->nz) goto out;
+    pml = cpml_create(g, cpml_default_params(cfg->npml));
+    if (!pml) goto out;
+    mat = materials_create(g);
+    if (!mat) goto out;
+    if (setup && setup(g, mat, context) != 0) goto out;
+    source_gaussian_deriv_params_t sp = {cfg->t0_over_dt * g->dt, cfg->tau_over_dt * g->dt};
+    source_t src = source_make_point(cfg->src_i, cfg->src_j, cfg->src_k, cfg->src_component, SRC_SOFT, 1.0,
+                                     source_waveform_gaussian_derivative, &sp);
+    fdtd_sim_options_t opt = {FDTD_FP64, 0, 0};
+    s = fdtd_sim_create(g, pml, mat, &opt);
+    if (!s) goto out;
+    /* source_apply ignores H components (source.c:45-49) */
+    if (src.component <= FIELD_EZ && fdtd_sim_add_point_source(s, &src) != 0) goto out;
+    if (fdtd_sim_add_probe(s, cfg->probe_component, cfg->probe_i, cfg->probe_j, cfg->probe_k, cfg->n_steps,
+                           NULL, 0) != 0)
+        goto out;
+    if (fdtd_sim_run(s, cfg->n_steps) != 0) goto out;
+    if (fdtd_sim_probe_td(s, 0, out_td, cfg->n_steps) != (long)cfg->n_steps) goto out;
+    if (sample_rate_out) *sample_rate_out = 1.0 / g->dt;
+    rc = 0;
+out:
+    fdtd_sim_destroy(s);
+    materials_destroy(mat);
+    cpml_destroy(pml);
+    fdtd_grid_destroy(g);
+    return rc;
+}
+
+int fdtd_simulate_echo(const fdtd_sim_config_t *cfg, double *out_td, double *sample_rate_out) {
+    return fdtd_simulate_echo_with_setup(cfg, out_td, sample_rate_out, NULL, NULL);
+}

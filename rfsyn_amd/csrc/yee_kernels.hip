@@ -1,0 +1,814 @@
+// Yee H/E update kernels for gfx950, FP64 (bitwise the reference arithmetic)
+// and FP32 (fields, CPML and ADE in float).
+//
+// Layout: the reference padded layout idx = i*sx + j*sy + k, z contiguous
+// (engine/fdtd/grid.h:25-30).  One thread owns one node (i,j,k) and updates
+// the three components that live there, so every field value a wave loads
+// along z is a coalesced 64-lane row.  The step is memory-bound: each node
+// reads 6 field values plus its neighbours (served by L1/L2 from adjacent
+// rows and planes) and writes 3.
+//
+// Compiled with -ffp-contract=off: every multiply and add rounds separately,
+// in the order of engine/fdtd/grid.c:70-89, cpml.h:49-50, cpml.c:185-260 and
+// materials.c:975-1055, so the FP64 kernels reproduce the reference bits.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "rf_device.h"
+#include "rf_hip.h"
+
+namespace {
+
+struct Geo {
+    int nx, ny, nz;
+    bool px, py, pz;
+    long long sx, sy;
+};
+
+__device__ __forceinline__ Geo geo_of(const rf_geom_t &g) {
+    Geo o;
+    o.nx = g.n[0]; o.ny = g.n[1]; o.nz = g.n[2];
+    o.px = g.per & 1; o.py = g.per & 2; o.pz = g.per & 4;
+    o.sx = g.sx; o.sy = g.sy;
+    return o;
+}
+
+// Compressed slab coordinate of q along axis a, or -1 outside the slab.
+__device__ __forceinline__ int slab_coord(const rf_cpml_t &c, int a, int q) {
+    if (q < c.lo[a]) return q;
+    if (q >= c.hi[a]) return c.lo[a] + q - c.hi[a];
+    return -1;
+}
+
+__device__ __forceinline__ long long slab_index(const rf_cpml_t &c, const Geo &g, int a, int cq,
+                                                int i, int j, int k) {
+    long long ny1 = (a == 1) ? c.len[1] : g.ny + 1;
+    long long nz1 = (a == 2) ? c.len[2] : g.nz + 1;
+    long long ii = (a == 0) ? cq : i, jj = (a == 1) ? cq : j, kk = (a == 2) ? cq : k;
+    return (ii * ny1 + jj) * nz1 + kk;
+}
+
+// cpml_term of cpml.h:46-51 with psi held only in slabs; outside the slab
+// psi stays exactly zero, so the term is diff*kinv + dx*0 as in the
+// reference's full-size arrays.
+template <typename T>
+__device__ __forceinline__ T stretch(T diff, const rf_cpml_t &c, const Geo &g, bool h, int axis,
+                                     int q, T *psi_arr, int i, int j, int k, T dx) {
+    const T *kinv = static_cast<const T *>(h ? c.kinv_h[axis] : c.kinv_m[axis]);
+    const T *bb = static_cast<const T *>(h ? c.b_h[axis] : c.b_m[axis]);
+    const T *aa = static_cast<const T *>(h ? c.a_h[axis] : c.a_m[axis]);
+    int cq = slab_coord(c, axis, q);
+    T psi = T(0);
+    if (cq >= 0) {
+        long long s = slab_index(c, g, axis, cq, i, j, k);
+        psi = bb[q] * psi_arr[s] + aa[q] * diff;
+        psi_arr[s] = psi;
+    }
+    return diff * kinv[q] + dx * psi;
+}
+
+// ----------------------------------------------------------------- H
+
+template <typename T, bool CPML>
+__global__ void __launch_bounds__(256)
+k_update_h(rf_geom_t gg, const T *__restrict__ ex, const T *__restrict__ ey,
+           const T *__restrict__ ez, T *__restrict__ hx, T *__restrict__ hy, T *__restrict__ hz,
+           T cb, T dx, rf_cpml_t cp) {
+    const Geo g = geo_of(gg);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k > g.nz || j > g.ny) return;
+    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+
+    const bool do_x = (i < g.nx + !g.px) && j < g.ny && k < g.nz;
+    const bool do_y = i < g.nx && (j < g.ny + !g.py) && k < g.nz;
+    const bool do_z = i < g.nx && j < g.ny && (k < g.nz + !g.pz);
+    if (!(do_x | do_y | do_z)) return;
+
+    // +1 neighbours with periodic wrap (grid.h:26-28).
+    const long long di = (g.px && i + 1 == g.nx) ? -(long long)i * g.sx : g.sx;
+    const long long dj = (g.py && j + 1 == g.ny) ? -(long long)j * g.sy : g.sy;
+    const long long dk = (g.pz && k + 1 == g.nz) ? -(long long)k : 1;
+
+    const T exc = ex[idx], eyc = ey[idx], ezc = ez[idx];
+    if (do_x) {   // grid.c:70, cpml.c:185-189
+        T dA = ez[idx + dj] - ezc;
+        T dB = ey[idx + dk] - eyc;
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[0][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[0][1]), i, j, k, dx);
+        }
+        hx[idx] -= cb * (dA - dB);
+    }
+    if (do_y) {   // grid.c:78, cpml.c:198-202
+        T dA = ex[idx + dk] - exc;
+        T dB = ez[idx + di] - ezc;
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[1][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[1][1]), i, j, k, dx);
+        }
+        hy[idx] -= cb * (dA - dB);
+    }
+    if (do_z) {   // grid.c:86, cpml.c:211-215
+        T dA = ey[idx + di] - eyc;
+        T dB = ex[idx + dj] - exc;
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[2][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[2][1]), i, j, k, dx);
+        }
+        hz[idx] -= cb * (dA - dB);
+    }
+}
+
+// ----------------------------------------------------------------- E
+
+template <typename T> struct EntryOf;
+template <> struct EntryOf<double> { using type = rf_entry64_t; using ade = rf_ade64_t; };
+template <> struct EntryOf<float> { using type = rf_entry32_t; using ade = rf_ade32_t; };
+
+struct NoTag { };
+
+template <typename TagT>
+__device__ __forceinline__ unsigned load_tag(const void *p, long long idx) {
+    return static_cast<unsigned>(static_cast<const TagT *>(p)[idx]);
+}
+
+// Trapezoidal ADE edge (materials.h): returns E^{n+1} and advances J, P.
+template <typename T>
+__device__ __forceinline__ T ade_edge(const typename EntryOf<T>::ade &a, T ca, T cb, T e_old,
+                                      T curl_a, T curl_b, T *J, T *P, long long slot_stride,
+                                      long long off) {
+    T u[RF_MAX_SLOTS];
+    T sum_u = T(0);
+    for (int s = 0; s < a.n_poles; s++) {
+        const long long o = s * slot_stride + off;
+        u[s] = a.has_p[s] ? a.kj[s] * J[o] - a.kp[s] * P[o] : a.kj[s] * J[o];
+        sum_u += u[s];
+    }
+    const T e_new = ca * e_old + cb * (curl_a - curl_b) - a.Cu * sum_u;
+    for (int s = 0; s < a.n_poles; s++) {
+        const long long o = s * slot_stride + off;
+        const T S = u[s] + a.v[s] * (e_new + e_old);
+        J[o] = S - J[o];
+        if (a.has_p[s]) P[o] = P[o] + a.hdt * S;
+    }
+    return e_new;
+}
+
+template <typename T, typename TagT, bool CPML>
+__device__ __forceinline__ void e_component(int c, T *__restrict__ E, long long idx, T dA, T dB,
+                                            const rf_mat_t &m, unsigned tag, const Geo &g,
+                                            int i, int j, int k) {
+    using Entry = typename EntryOf<T>::type;
+    using Ade = typename EntryOf<T>::ade;
+    if constexpr (std::is_same<TagT, NoTag>::value) {
+        E[idx] = T(1) * E[idx] + static_cast<T>(m.vac_ca) * (dA - dB);
+    } else {
+        const Entry &en = static_cast<const Entry *>(m.entries)[tag];
+        const uint32_t fl = en.flags[c];
+        if (fl & RF_EDGE_TENSOR) return;
+        if (fl & RF_EDGE_ADE) {
+            const Ade &a = static_cast<const Ade *>(m.ade)[en.ade[c]];
+            const long long cells = (long long)m.bn[0] * m.bn[1] * m.bn[2];
+            const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
+            const long long off = (long long)c * cells + cell;
+            E[idx] = ade_edge<T>(a, en.Ca[c], en.Cb[c], E[idx], dA, dB, static_cast<T *>(m.J),
+                                 static_cast<T *>(m.P), 3 * cells, off);
+            return;
+        }
+        E[idx] = en.Ca[c] * E[idx] + en.Cb[c] * (dA - dB);
+    }
+}
+
+template <typename T, typename TagT, bool CPML>
+__global__ void __launch_bounds__(256)
+k_update_e(rf_geom_t gg, T *__restrict__ ex, T *__restrict__ ey, T *__restrict__ ez,
+           const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
+           T dx, rf_cpml_t cp, rf_mat_t m) {
+    const Geo g = geo_of(gg);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+
+    // E ranges (grid.c:95-117): own axis from 0, the other axes from 1
+    // unless periodic (PEC planes at 0 and n are never updated).
+    const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
+    const bool do_x = j_ok && k_ok;
+    const bool do_y = i_ok && k_ok;
+    const bool do_z = i_ok && j_ok;
+    if (!(do_x | do_y | do_z)) return;
+
+    // -1 neighbours with wrap at 0 (materials.c:808-810).
+    const long long di = (i == 0) ? (long long)(g.nx - 1) * g.sx : -g.sx;
+    const long long dj = (j == 0) ? (long long)(g.ny - 1) * g.sy : -g.sy;
+    const long long dk = (k == 0) ? (long long)(g.nz - 1) : -1;
+
+    unsigned tag = 0;
+    if constexpr (!std::is_same<TagT, NoTag>::value) tag = load_tag<TagT>(m.tag, idx);
+
+    const T hxc = hx[idx], hyc = hy[idx], hzc = hz[idx];
+    if (do_x) {   // materials.c:975-1000
+        T dA = hzc - hz[idx + dj];
+        T dB = hyc - hy[idx + dk];
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[0][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
+        }
+        e_component<T, TagT, CPML>(0, ex, idx, dA, dB, m, tag, g, i, j, k);
+    }
+    if (do_y) {   // materials.c:1015-1040
+        T dA = hxc - hx[idx + dk];
+        T dB = hzc - hz[idx + di];
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[1][1]), i, j, k, dx);
+        }
+        e_component<T, TagT, CPML>(1, ey, idx, dA, dB, m, tag, g, i, j, k);
+    }
+    if (do_z) {   // materials.c:1050-1058
+        T dA = hyc - hy[idx + di];
+        T dB = hxc - hx[idx + dj];
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[2][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[2][1]), i, j, k, dx);
+        }
+        e_component<T, TagT, CPML>(2, ez, idx, dA, dB, m, tag, g, i, j, k);
+    }
+}
+
+// ----------------------------------------------------------- tensors
+
+template <typename T> struct TEdgeOf;
+template <> struct TEdgeOf<double> { using type = rf_tedge64_t; };
+template <> struct TEdgeOf<float> { using type = rf_tedge32_t; };
+
+struct Pos { int x[3]; };
+
+__device__ __forceinline__ int nxt(const Geo &g, int ax, int q) {
+    const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+    return q + 1 == n ? 0 : q + 1;
+}
+__device__ __forceinline__ int prv(const Geo &g, int ax, int q) {
+    const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+    return q == 0 ? n - 1 : q - 1;
+}
+__device__ __forceinline__ long long lin(const Geo &g, const Pos &p) {
+    return (long long)p.x[0] * g.sx + (long long)p.x[1] * g.sy + p.x[2];
+}
+
+template <typename T>
+__device__ __forceinline__ T curl_full(const Geo &g, const T *const H[3], int c, const Pos &x) {
+    const int p = (c + 1) % 3, q = (c + 2) % 3;
+    Pos xp = x, xq = x;
+    xp.x[p] = prv(g, p, x.x[p]);
+    xq.x[q] = prv(g, q, x.x[q]);
+    const long long idx = lin(g, x);
+    const T dA = H[q][idx] - H[q][lin(g, xp)];
+    const T dB = H[p][idx] - H[p][lin(g, xq)];
+    return dA - dB;
+}
+
+// Hz-only curls of the reference's z-invariant stencil (materials.c:856-901).
+template <typename T>
+__device__ __forceinline__ T curl_hz(const Geo &g, const T *Hz, int c, const Pos &x) {
+    Pos y = x;
+    if (c == 0) {
+        y.x[1] = prv(g, 1, x.x[1]);
+        return Hz[lin(g, x)] - Hz[lin(g, y)];
+    }
+    y.x[0] = prv(g, 0, x.x[0]);
+    return -(Hz[lin(g, x)] - Hz[lin(g, y)]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_tensor(rf_geom_t gg, T *ex, T *ey, T *ez, const T *hx, const T *hy, const T *hz,
+         const typename TEdgeOf<T>::type *__restrict__ edges, long long n_edges) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_edges) return;
+    const Geo g = geo_of(gg);
+    const typename TEdgeOf<T>::type e = edges[t];
+    const T *const H[3] = {hx, hy, hz};
+    T *const E[3] = {ex, ey, ez};
+    const int c = e.comp;
+    Pos x;
+    x.x[0] = e.i; x.x[1] = e.j; x.x[2] = e.k;
+    const long long idx = lin(g, x);
+    T value;
+    if (e.kind == 2) {   // materials.c:858, 881
+        Pos y = x;
+        if (c == 0) {
+            y.x[1] = prv(g, 1, x.x[1]);
+            value = e.diag * (hz[idx] - hz[lin(g, y)]);
+        } else {
+            y.x[0] = prv(g, 0, x.x[0]);
+            value = -e.diag * (hz[idx] - hz[lin(g, y)]);
+        }
+    } else {
+        value = e.diag * curl_full<T>(g, H, c, x);
+    }
+    int slot = 0;
+    for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        const int lo = c < d ? c : d;
+        for (int q = 0; q < 4; q++) {
+            if (!((e.mask >> (slot * 4 + q)) & 1u)) continue;
+            const int s_lo = q & 1, s_hi = (q >> 1) & 1;
+            const int s_c = (c == lo) ? s_lo : s_hi;
+            const int s_d = (d == lo) ? s_lo : s_hi;
+            Pos y = x;
+            if (s_c) y.x[c] = nxt(g, c, x.x[c]);
+            if (s_d) y.x[d] = prv(g, d, x.x[d]);
+            const T cd = e.kind == 2 ? curl_hz<T>(g, hz, d, y) : curl_full<T>(g, H, d, y);
+            value += e.w[slot * 4 + q] * cd;
+        }
+        slot++;
+    }
+    E[c][idx] += e.scale * value;
+}
+
+// ----------------------------------------------------------- sources
+
+template <typename T>
+__global__ void k_point_sources(T *ex, T *ey, T *ez, const rf_point_src_t *srcs, int n,
+                                const long long *d_step, int step_offset) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long long step = *d_step + step_offset;
+    T *const E[3] = {ex, ey, ez};
+    // Sequential in insertion order, like repeated source_apply calls.
+    for (int s = 0; s < n; s++) {
+        const rf_point_src_t p = srcs[s];
+        const double v = p.wave[step];
+        T *f = E[p.comp];
+        if (p.hard) f[p.idx] = static_cast<T>(v);
+        else f[p.idx] = static_cast<T>(static_cast<double>(f[p.idx]) + v);
+    }
+}
+
+template <typename T>
+__global__ void k_plane_source(rf_geom_t gg, T *ex, T *ey, T *ez, rf_plane_src_t p,
+                               const long long *d_step, int step_offset) {
+    const Geo g = geo_of(gg);
+    const int a = (p.axis + 1) % 3 < (p.axis + 2) % 3 ? (p.axis + 1) % 3 : (p.axis + 2) % 3;
+    const int b = 3 - p.axis - a;
+    const int na = a == 0 ? g.nx : a == 1 ? g.ny : g.nz;
+    const int nb = b == 0 ? g.nx : b == 1 ? g.ny : g.nz;
+    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ia = blockIdx.y;
+    if (ib >= nb || ia >= na) return;
+    Pos x;
+    x.x[p.axis] = p.index;
+    x.x[a] = ia;
+    x.x[b] = ib;
+    // fdtd_index wraps the plane coordinate on a periodic axis.
+    const int nplane = p.axis == 0 ? g.nx : p.axis == 1 ? g.ny : g.nz;
+    const bool wrap = (gg.per >> p.axis) & 1;
+    if (wrap) x.x[p.axis] %= nplane;
+    const long long idx = lin(g, x);
+    const double v = p.wave[*d_step + step_offset];
+    T *f = p.comp == 0 ? ex : p.comp == 1 ? ey : ez;
+    if (p.hard) f[idx] = static_cast<T>(v);
+    else f[idx] = static_cast<T>(static_cast<double>(f[idx]) + v);
+}
+
+// ------------------------------------------------------------ probes
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_probe_record(const T *ex, const T *ey, const T *ez, const T *hx, const T *hy, const T *hz,
+               const long long *idx, const int *comp, int n, double *ring, int ring_len,
+               double *td, long long td_cap, long long *d_step) {
+    const long long step = *d_step;
+    const T *const F[6] = {ex, ey, ez, hx, hy, hz};
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const double v = static_cast<double>(F[comp[p]][idx[p]]);
+        ring[(step % ring_len) * n + p] = v;
+        if (step < td_cap) td[(long long)p * td_cap + step] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *d_step = step + 1;
+}
+
+__device__ __forceinline__ void neumaier(double &s, double &c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+// One thread per (probe, frequency); sums over the buffered steps in time
+// order, as probe_dft_record does step by step (probe.c:87-91).
+__global__ void __launch_bounds__(256)
+k_probe_dft(const double *ring, int ring_len, int n_probes, const int *freq_probe, int n_freq,
+            const double *freq_w, double dt, const long long *d_step, int count, double *acc_re,
+            double *acc_im, double *cmp_re, double *cmp_im, int comp_sum) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_freq) return;
+    const int p = freq_probe[t];
+    const double w = freq_w[t];
+    const long long end = *d_step;
+    double re = acc_re[t], im = acc_im[t];
+    double cr = comp_sum ? cmp_re[t] : 0.0, ci = comp_sum ? cmp_im[t] : 0.0;
+    for (long long n = end - count; n < end; n++) {
+        const double v = ring[(n % ring_len) * n_probes + p];
+        const double tt = static_cast<double>(n + 1) * dt;
+        double sn, cs;
+        sincos(w * tt, &sn, &cs);
+        const double tr = v * cs * dt;
+        const double ti = v * sn * dt;
+        if (comp_sum) {
+            neumaier(re, cr, tr);
+            neumaier(im, ci, -ti);
+        } else {
+            re += tr;
+            im -= ti;
+        }
+    }
+    acc_re[t] = re;
+    acc_im[t] = im;
+    if (comp_sum) {
+        cmp_re[t] = cr;
+        cmp_im[t] = ci;
+    }
+}
+
+// ----------------------------------------------------- plane monitors
+
+template <typename T, typename B>
+__global__ void __launch_bounds__(256)
+k_plane_capture(rf_geom_t gg, const T *f, int axis, int index, B *buf, int ring_len,
+                const long long *d_step) {
+    const Geo g = geo_of(gg);
+    const int a = (axis + 1) % 3 < (axis + 2) % 3 ? (axis + 1) % 3 : (axis + 2) % 3;
+    const int b = 3 - axis - a;
+    const int na = a == 0 ? g.nx : a == 1 ? g.ny : g.nz;
+    const int nb = b == 0 ? g.nx : b == 1 ? g.ny : g.nz;
+    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ia = blockIdx.y;
+    if (ib >= nb || ia >= na) return;
+    Pos x;
+    x.x[axis] = index;
+    if ((gg.per >> axis) & 1) {
+        const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+        x.x[axis] %= n;
+    }
+    x.x[a] = ia;
+    x.x[b] = ib;
+    const long long slot = *d_step % ring_len;
+    buf[slot * (long long)na * nb + (long long)ia * nb + ib] = static_cast<B>(f[lin(g, x)]);
+}
+
+// Twiddles for the buffered block: tw[s][f] = (cos, sin)(w_f (n_s+1) dt).
+__global__ void k_twiddles(int nf, const double *freq_w, double dt, const long long *d_step,
+                           int count, double2 *tw) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf * count) return;
+    const int s = t / nf, f = t % nf;
+    const long long n = *d_step - count + s;
+    const double tt = static_cast<double>(n + 1) * dt;
+    double sn, cs;
+    sincos(freq_w[f] * tt, &sn, &cs);
+    tw[t] = make_double2(cs, sn);
+}
+
+// One thread per plane cell; for every frequency, sum the buffered steps
+// in time order into the double accumulators [f][cell].
+template <typename B>
+__global__ void __launch_bounds__(256)
+k_plane_dft(const B *buf, long long cells, int ring_len, int nf, const double2 *tw, double dt,
+            const long long *d_step, int count, double *acc_re, double *acc_im, double *cmp_re,
+            double *cmp_im, int comp_sum) {
+    extern __shared__ double2 s_tw[];
+    for (int t = threadIdx.x; t < nf * count; t += blockDim.x) s_tw[t] = tw[t];
+    __syncthreads();
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= cells) return;
+    const long long end = *d_step;
+    double v[64];
+    const int cnt = count > 64 ? 64 : count;
+    for (int s = 0; s < cnt; s++) {
+        const long long n = end - count + s;
+        v[s] = static_cast<double>(buf[(n % ring_len) * cells + cell]);
+    }
+    for (int f = 0; f < nf; f++) {
+        const long long o = (long long)f * cells + cell;
+        double re = acc_re[o], im = acc_im[o];
+        double cr = comp_sum ? cmp_re[o] : 0.0, ci = comp_sum ? cmp_im[o] : 0.0;
+        for (int s = 0; s < cnt; s++) {
+            const double2 w = s_tw[s * nf + f];
+            const double tr = v[s] * w.x * dt;
+            const double ti = v[s] * w.y * dt;
+            if (comp_sum) {
+                neumaier(re, cr, tr);
+                neumaier(im, ci, -ti);
+            } else {
+                re += tr;
+                im -= ti;
+            }
+        }
+        acc_re[o] = re;
+        acc_im[o] = im;
+        if (comp_sum) {
+            cmp_re[o] = cr;
+            cmp_im[o] = ci;
+        }
+    }
+}
+
+// ------------------------------------------------------------ energy
+
+template <typename T, typename TagT>
+__global__ void __launch_bounds__(256)
+k_energy(rf_geom_t gg, const T *ex, const T *ey, const T *ez, const T *hx, const T *hy,
+         const T *hz, const void *tag, const double *eps_tab, double vac_eps, double mu0,
+         double *partials) {
+    const Geo g = geo_of(gg);
+    const long long cells = (long long)g.nx * g.ny * g.nz;
+    double ue = 0.0, uh = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long long r = t / g.nz;
+        const int j = (int)(r % g.ny), i = (int)(r / g.ny);
+        const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+        double eps[3] = {vac_eps, vac_eps, vac_eps};
+        if constexpr (!std::is_same<TagT, NoTag>::value) {
+            const unsigned tg = load_tag<TagT>(tag, idx);
+            eps[0] = eps_tab[3 * tg]; eps[1] = eps_tab[3 * tg + 1]; eps[2] = eps_tab[3 * tg + 2];
+        }
+        const double e0 = ex[idx], e1 = ey[idx], e2 = ez[idx];
+        const double h0 = hx[idx], h1 = hy[idx], h2 = hz[idx];
+        ue += 0.5 * eps[0] * e0 * e0 + 0.5 * eps[1] * e1 * e1 + 0.5 * eps[2] * e2 * e2;
+        uh += 0.5 * mu0 * h0 * h0 + 0.5 * mu0 * h1 * h1 + 0.5 * mu0 * h2 * h2;
+    }
+    __shared__ double se[256], sh[256];
+    se[threadIdx.x] = ue;
+    sh[threadIdx.x] = uh;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            se[threadIdx.x] += se[threadIdx.x + w];
+            sh[threadIdx.x] += sh[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = se[0];
+        partials[2 * blockIdx.x + 1] = sh[0];
+    }
+}
+
+__global__ void k_energy_final(const double *partials, int n, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double e = 0.0, h = 0.0;
+    for (int b = 0; b < n; b++) {
+        e += partials[2 * b];
+        h += partials[2 * b + 1];
+    }
+    out[0] = e;
+    out[1] = h;
+}
+
+template <typename D, typename S>
+__global__ void k_convert(D *dst, const S *src, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        dst[t] = static_cast<D>(src[t]);
+}
+
+int launched() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+int launch_h(const rf_geom_t *g, void *const f[6], double cb, double dx, const rf_cpml_t *cp,
+             hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    dim3 grd((g->n[2] + 1 + 63) / 64, (g->n[1] + 1 + 3) / 4, g->n[0] + 1);
+    rf_cpml_t none{};
+    T *F[6];
+    for (int q = 0; q < 6; q++) F[q] = static_cast<T *>(f[q]);
+    if (cp)
+        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, 0, st, g[0], F[0], F[1], F[2], F[3], F[4],
+                           F[5], static_cast<T>(cb), static_cast<T>(dx), *cp);
+    else
+        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, st, g[0], F[0], F[1], F[2], F[3],
+                           F[4], F[5], static_cast<T>(cb), static_cast<T>(dx), none);
+    return launched();
+}
+
+template <typename T, typename TagT>
+int launch_e_tag(const rf_geom_t *g, T *const F[6], double dx, const rf_cpml_t *cp,
+                 const rf_mat_t *m, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    dim3 grd((g->n[2] + 63) / 64, (g->n[1] + 3) / 4, g->n[0]);
+    rf_cpml_t none{};
+    if (cp)
+        hipLaunchKernelGGL((k_update_e<T, TagT, true>), grd, blk, 0, st, g[0], F[0], F[1], F[2], F[3],
+                           F[4], F[5], static_cast<T>(dx), *cp, *m);
+    else
+        hipLaunchKernelGGL((k_update_e<T, TagT, false>), grd, blk, 0, st, g[0], F[0], F[1], F[2],
+                           F[3], F[4], F[5], static_cast<T>(dx), none, *m);
+    return launched();
+}
+
+template <typename T>
+int launch_e(const rf_geom_t *g, void *const f[6], double dx, const rf_cpml_t *cp,
+             const rf_mat_t *m, hipStream_t st) {
+    T *F[6];
+    for (int q = 0; q < 6; q++) F[q] = static_cast<T *>(f[q]);
+    if (!m->tag) return launch_e_tag<T, NoTag>(g, F, dx, cp, m, st);
+    switch (m->tag_bytes) {
+    case 1: return launch_e_tag<T, uint8_t>(g, F, dx, cp, m, st);
+    case 2: return launch_e_tag<T, uint16_t>(g, F, dx, cp, m, st);
+    case 4: return launch_e_tag<T, uint32_t>(g, F, dx, cp, m, st);
+    }
+    return -1;
+}
+
+template <typename T, typename TagT>
+int launch_energy_t(const rf_geom_t *g, T *const F[6], const void *tag, const double *eps_tab,
+                    double vac_eps, double mu0, double *partials, int n_partials, double *out,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((k_energy<T, TagT>), dim3(n_partials), dim3(256), 0, st, g[0], F[0], F[1],
+                       F[2], F[3], F[4], F[5], tag, eps_tab, vac_eps, mu0, partials);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, st, partials, n_partials, out);
+    return launched();
+}
+
+}  // namespace
+
+static hipStream_t hs(rf_stream_t *s) { return rf_raw_stream(s); }
+
+extern "C" {
+
+int rf_launch_update_h(int prec, const rf_geom_t *g, void *const f[6], double cb, double dx,
+                       const rf_cpml_t *cpml, rf_stream_t *s) {
+    return prec == 0 ? launch_h<double>(g, f, cb, dx, cpml, hs(s))
+                     : launch_h<float>(g, f, cb, dx, cpml, hs(s));
+}
+
+int rf_launch_update_e(int prec, const rf_geom_t *g, void *const f[6], double dx,
+                       const rf_cpml_t *cpml, const rf_mat_t *mat, rf_stream_t *s) {
+    return prec == 0 ? launch_e<double>(g, f, dx, cpml, mat, hs(s))
+                     : launch_e<float>(g, f, dx, cpml, mat, hs(s));
+}
+
+int rf_launch_tensor(int prec, const rf_geom_t *g, void *const f[6], const void *edges,
+                     long long n_edges, rf_stream_t *s) {
+    if (n_edges <= 0) return 0;
+    dim3 grd((unsigned)((n_edges + 255) / 256));
+    if (prec == 0) {
+        double *F[6];
+        for (int q = 0; q < 6; q++) F[q] = static_cast<double *>(f[q]);
+        hipLaunchKernelGGL(k_tensor<double>, grd, dim3(256), 0, hs(s), g[0], F[0], F[1], F[2], F[3],
+                           F[4], F[5], static_cast<const rf_tedge64_t *>(edges), n_edges);
+    } else {
+        float *F[6];
+        for (int q = 0; q < 6; q++) F[q] = static_cast<float *>(f[q]);
+        hipLaunchKernelGGL(k_tensor<float>, grd, dim3(256), 0, hs(s), g[0], F[0], F[1], F[2], F[3],
+                           F[4], F[5], static_cast<const rf_tedge32_t *>(edges), n_edges);
+    }
+    return launched();
+}
+
+int rf_launch_point_sources(int prec, void *const f[6], const rf_point_src_t *srcs_dev, int n,
+                            const long long *d_step, int step_offset, rf_stream_t *s) {
+    if (n <= 0) return 0;
+    if (prec == 0)
+        hipLaunchKernelGGL(k_point_sources<double>, dim3(1), dim3(64), 0, hs(s),
+                           static_cast<double *>(f[0]), static_cast<double *>(f[1]),
+                           static_cast<double *>(f[2]), srcs_dev, n, d_step, step_offset);
+    else
+        hipLaunchKernelGGL(k_point_sources<float>, dim3(1), dim3(64), 0, hs(s),
+                           static_cast<float *>(f[0]), static_cast<float *>(f[1]),
+                           static_cast<float *>(f[2]), srcs_dev, n, d_step, step_offset);
+    return launched();
+}
+
+int rf_launch_plane_source(int prec, const rf_geom_t *g, void *const f[6], const rf_plane_src_t *p,
+                           const long long *d_step, int step_offset, rf_stream_t *s) {
+    const int a = (p->axis + 1) % 3 < (p->axis + 2) % 3 ? (p->axis + 1) % 3 : (p->axis + 2) % 3;
+    const int b = 3 - p->axis - a;
+    dim3 grd((g->n[b] + 255) / 256, g->n[a]);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_plane_source<double>, grd, dim3(256), 0, hs(s), g[0],
+                           static_cast<double *>(f[0]), static_cast<double *>(f[1]),
+                           static_cast<double *>(f[2]), *p, d_step, step_offset);
+    else
+        hipLaunchKernelGGL(k_plane_source<float>, grd, dim3(256), 0, hs(s), g[0],
+                           static_cast<float *>(f[0]), static_cast<float *>(f[1]),
+                           static_cast<float *>(f[2]), *p, d_step, step_offset);
+    return launched();
+}
+
+int rf_launch_probe_record(int prec, void *const f[6], const long long *idx_dev, const int *comp_dev,
+                           int n, double *ring, int ring_len, double *td, long long td_cap,
+                           long long *d_step, rf_stream_t *s) {
+    if (prec == 0) {
+        double *F[6];
+        for (int q = 0; q < 6; q++) F[q] = static_cast<double *>(f[q]);
+        hipLaunchKernelGGL(k_probe_record<double>, dim3(1), dim3(256), 0, hs(s), F[0], F[1], F[2],
+                           F[3], F[4], F[5], idx_dev, comp_dev, n, ring, ring_len, td, td_cap, d_step);
+    } else {
+        float *F[6];
+        for (int q = 0; q < 6; q++) F[q] = static_cast<float *>(f[q]);
+        hipLaunchKernelGGL(k_probe_record<float>, dim3(1), dim3(256), 0, hs(s), F[0], F[1], F[2],
+                           F[3], F[4], F[5], idx_dev, comp_dev, n, ring, ring_len, td, td_cap, d_step);
+    }
+    return launched();
+}
+
+int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const int *freq_probe,
+                        int n_freq, const double *freq_w, double dt, const long long *d_step,
+                        int count, double *acc_re, double *acc_im, double *cmp_re, double *cmp_im,
+                        int comp_sum, rf_stream_t *s) {
+    if (n_freq <= 0 || count <= 0) return 0;
+    hipLaunchKernelGGL(k_probe_dft, dim3((n_freq + 255) / 256), dim3(256), 0, hs(s), ring, ring_len,
+                       n_probes, freq_probe, n_freq, freq_w, dt, d_step, count, acc_re, acc_im,
+                       cmp_re, cmp_im, comp_sum);
+    return launched();
+}
+
+int rf_launch_plane_capture(int prec, const rf_geom_t *g, void *const f[6], int axis, int index,
+                            int comp, void *buf, int buf_prec, int ring_len,
+                            const long long *d_step, rf_stream_t *s) {
+    const int a = (axis + 1) % 3 < (axis + 2) % 3 ? (axis + 1) % 3 : (axis + 2) % 3;
+    const int b = 3 - axis - a;
+    dim3 grd((g->n[b] + 255) / 256, g->n[a]);
+    if (prec == 0 && buf_prec == 0)
+        hipLaunchKernelGGL((k_plane_capture<double, double>), grd, dim3(256), 0, hs(s), g[0],
+                           static_cast<const double *>(f[comp]), axis, index,
+                           static_cast<double *>(buf), ring_len, d_step);
+    else if (prec == 1 && buf_prec == 1)
+        hipLaunchKernelGGL((k_plane_capture<float, float>), grd, dim3(256), 0, hs(s), g[0],
+                           static_cast<const float *>(f[comp]), axis, index,
+                           static_cast<float *>(buf), ring_len, d_step);
+    else
+        return -1;
+    return launched();
+}
+
+int rf_launch_plane_dft(const void *buf, int buf_prec, long long cells, int ring_len, int nf,
+                        const double *freq_w, double dt, const long long *d_step, int count,
+                        double *tw_scratch, double *acc_re, double *acc_im, double *cmp_re,
+                        double *cmp_im, int comp_sum, rf_stream_t *s) {
+    if (count <= 0) return 0;
+    if (count > 64 || (size_t)nf * count * sizeof(double2) > 64 * 1024) return -1;
+    double2 *tw = reinterpret_cast<double2 *>(tw_scratch);
+    hipLaunchKernelGGL(k_twiddles, dim3((nf * count + 255) / 256), dim3(256), 0, hs(s), nf, freq_w,
+                       dt, d_step, count, tw);
+    const size_t lds = (size_t)nf * count * sizeof(double2);
+    dim3 grd((unsigned)((cells + 255) / 256));
+    if (buf_prec == 0)
+        hipLaunchKernelGGL((k_plane_dft<double>), grd, dim3(256), lds, hs(s),
+                           static_cast<const double *>(buf), cells, ring_len, nf, tw, dt, d_step,
+                           count, acc_re, acc_im, cmp_re, cmp_im, comp_sum);
+    else
+        hipLaunchKernelGGL((k_plane_dft<float>), grd, dim3(256), lds, hs(s),
+                           static_cast<const float *>(buf), cells, ring_len, nf, tw, dt, d_step,
+                           count, acc_re, acc_im, cmp_re, cmp_im, comp_sum);
+    return launched();
+}
+
+int rf_launch_energy(int prec, const rf_geom_t *g, void *const f[6], const void *tag, int tag_bytes,
+                     const double *eps_tab, double vac_eps, double mu0, double *partials,
+                     int n_partials, double *out, rf_stream_t *s) {
+    if (prec == 0) {
+        double *F[6];
+        for (int q = 0; q < 6; q++) F[q] = static_cast<double *>(f[q]);
+        if (!tag) return launch_energy_t<double, NoTag>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+        if (tag_bytes == 1) return launch_energy_t<double, uint8_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+        if (tag_bytes == 2) return launch_energy_t<double, uint16_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+        return launch_energy_t<double, uint32_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+    }
+    float *F[6];
+    for (int q = 0; q < 6; q++) F[q] = static_cast<float *>(f[q]);
+    if (!tag) return launch_energy_t<float, NoTag>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+    if (tag_bytes == 1) return launch_energy_t<float, uint8_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+    if (tag_bytes == 2) return launch_energy_t<float, uint16_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+    return launch_energy_t<float, uint32_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+}
+
+int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,
+                      rf_stream_t *s) {
+    if (n <= 0) return 0;
+    const unsigned blocks = (unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256);
+    if (dst_prec == 1 && src_prec == 0)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks), dim3(256), 0, hs(s),
+                           static_cast<float *>(dst), static_cast<const double *>(src), n);
+    else if (dst_prec == 0 && src_prec == 1)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks), dim3(256), 0, hs(s),
+                           static_cast<double *>(dst), static_cast<const float *>(src), n);
+    else
+        return -1;
+    return launched();
+}
+
+}  // extern "C"
