@@ -98,6 +98,13 @@ int fdtd_sim_upload(fdtd_sim_t *s, const fdtd_grid_t *g);
  * reduction order (eps from the scalar coefficient of each edge). */
 int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic);
 
+/* Profiling mode: on != 0 launches every kernel directly (no graph
+ * replay) and times the H and E kernels with HIP events on the handle's
+ * stream.  fdtd_sim_kernel_times returns the accumulated milliseconds and
+ * the number of timed steps since the previous call, then resets them. */
+int fdtd_sim_set_profiling(fdtd_sim_t *s, int on);
+int fdtd_sim_kernel_times(fdtd_sim_t *s, double *h_ms, double *e_ms, long *steps);
+
 /* Bytes of device memory held by the handle. */
 size_t fdtd_sim_device_bytes(const fdtd_sim_t *s);
 

@@ -191,6 +191,8 @@ EXTENSION_API = {
     "fdtd_sim_upload": (C.c_int, [_vp, _gp]),
     "fdtd_sim_energy": (C.c_int, [_vp, _dp, _dp]),
     "fdtd_sim_device_bytes": (_sz, [_vp]),
+    "fdtd_sim_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "fdtd_sim_kernel_times": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_long)]),
     "fdtd_last_error": (C.c_char_p, []),
 }
 

@@ -37,6 +37,11 @@ int rf_graph_end(rf_stream_t *s, rf_graph_t **out);
 int rf_graph_launch(rf_graph_t *g, rf_stream_t *s);
 void rf_graph_destroy(rf_graph_t *g);
 const char *rf_error_string(void);          /* last runtime error text */
+typedef struct rf_event rf_event_t;        /* wraps one hipEvent_t */
+rf_event_t *rf_event_create(void);
+void rf_event_destroy(rf_event_t *e);
+int rf_event_record(rf_event_t *e, rf_stream_t *s);
+int rf_event_elapsed_ms(rf_event_t *a, rf_event_t *b, float *ms);
 
 /* ----------------------------------------------------------- geometry */
 

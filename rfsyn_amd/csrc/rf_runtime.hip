@@ -155,3 +155,36 @@ void rf_graph_destroy(rf_graph_t *g) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ events
+struct rf_event { hipEvent_t e; };
+
+extern "C" {
+
+rf_event_t *rf_event_create(void) {
+    if (rf_init() != 0) return nullptr;
+    rf_event_t *e = new rf_event;
+    if (hipEventCreate(&e->e) != hipSuccess) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+void rf_event_destroy(rf_event_t *e) {
+    if (!e) return;
+    (void)hipEventDestroy(e->e);
+    delete e;
+}
+
+int rf_event_record(rf_event_t *e, rf_stream_t *s) {
+    RF_CHECK(hipEventRecord(e->e, raw(s)));
+    return 0;
+}
+
+int rf_event_elapsed_ms(rf_event_t *a, rf_event_t *b, float *ms) {
+    RF_CHECK(hipEventElapsedTime(ms, a->e, b->e));
+    return 0;
+}
+
+}  // extern "C"

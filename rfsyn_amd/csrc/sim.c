@@ -100,6 +100,13 @@ struct fdtd_sim {
 
     rf_graph_t *graph;
     size_t bytes;
+
+    /* kernel timing with HIP events (profiling mode launches directly) */
+    int profiling;
+    rf_event_t **ev;
+    size_t n_ev, ev_used;
+    double prof_h_ms, prof_e_ms;
+    long prof_launches;
 };
 
 static int dalloc(fdtd_sim_t *s, void **p, size_t bytes) {
@@ -211,6 +218,8 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     rf_free(s->d_tw);
     rf_free(s->d_step);
     rf_free(s->d_energy);
+    for (size_t q = 0; q < s->n_ev; q++) rf_event_destroy(s->ev[q]);
+    free(s->ev);
     rf_stream_destroy(s->st);
     free(s);
 }
@@ -442,18 +451,28 @@ static int ensure_waves(fdtd_sim_t *s, size_t need) {
     return 0;
 }
 
+static int mark(fdtd_sim_t *s, int q) {
+    if (!s->profiling) return 0;
+    return rf_event_record(s->ev[s->ev_used + (size_t)q], s->st);
+}
+
 static int enqueue_step(fdtd_sim_t *s) {
     const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
     const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
     double cb = s->dt / (MU0 * s->dx);
+    if (mark(s, 0)) return -1;
     if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
+    if (mark(s, 1)) return -1;
     if (s->n_pts && rf_launch_point_sources(s->prec, s->f, s->d_pts, (int)s->n_pts, s->d_step, 0, s->st))
         return -1;
     for (size_t p = 0; p < s->n_planes; p++)
         if (!s->planes[p].after_e &&
             rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
             return -1;
+    if (mark(s, 2)) return -1;
     if (rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) return -1;
+    if (mark(s, 3)) return -1;
+    if (s->profiling) s->ev_used += 4;
     if (s->has_mat && s->mat.n_tensor &&
         rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
         return -1;
@@ -503,6 +522,45 @@ static int capture_block(fdtd_sim_t *s) {
     return 0;
 }
 
+/* Folds the recorded event pairs into the totals (synchronises). */
+static int prof_collect(fdtd_sim_t *s) {
+    if (rf_sync(s->st)) return -1;
+    for (size_t q = 0; q + 4 <= s->ev_used; q += 4) {
+        float h = 0.f, e = 0.f;
+        if (rf_event_elapsed_ms(s->ev[q], s->ev[q + 1], &h) || rf_event_elapsed_ms(s->ev[q + 2], s->ev[q + 3], &e))
+            return -1;
+        s->prof_h_ms += h;
+        s->prof_e_ms += e;
+        s->prof_launches++;
+    }
+    s->ev_used = 0;
+    return 0;
+}
+
+int fdtd_sim_set_profiling(fdtd_sim_t *s, int on) {
+    if (!s) return -1;
+    if (on && !s->ev) {
+        s->n_ev = 4 * 256;
+        s->ev = calloc(s->n_ev, sizeof *s->ev);
+        if (!s->ev) return -1;
+        for (size_t q = 0; q < s->n_ev; q++)
+            if (!(s->ev[q] = rf_event_create())) return -1;
+    }
+    s->profiling = on ? 1 : 0;
+    s->ev_used = 0;
+    return 0;
+}
+
+int fdtd_sim_kernel_times(fdtd_sim_t *s, double *h_ms, double *e_ms, long *launches) {
+    if (!s || !h_ms || !e_ms || !launches) return -1;
+    *h_ms = s->prof_h_ms;
+    *e_ms = s->prof_e_ms;
+    *launches = s->prof_launches;
+    s->prof_h_ms = s->prof_e_ms = 0.0;
+    s->prof_launches = 0;
+    return 0;
+}
+
 int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
     if (!s) return -1;
     if (!s->started && finalize(s) != 0) {
@@ -515,7 +573,8 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
     }
     size_t left = n_steps;
     while (left) {
-        if (s->pending == 0 && left >= (size_t)s->K) {
+        if (s->profiling && s->ev_used + 4 > s->n_ev && prof_collect(s) != 0) return -1;
+        if (!s->profiling && s->pending == 0 && left >= (size_t)s->K) {
             if (!s->graph && capture_block(s) != 0) {
                 rf_set_error("graph capture: %s", rf_error_string());
                 return -1;
@@ -541,7 +600,7 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
         rf_set_error("fdtd_sim_run: %s", rf_error_string());
         return -1;
     }
-    return 0;
+    return s->profiling ? prof_collect(s) : 0;
 }
 
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }

@@ -1,223 +1,5 @@
-"""Scene builders shared by the tests.
-
-Every builder takes a bound library (product, oracle or reference build)
-and drives it only through the C API of include/*.h, so the same scene can
-be run on each and compared.
-"""
-import ctypes as C
-import math
-
-import numpy as np
-
-from rfsyn_amd import capi
-
-EPS0 = 8.8541878128e-12
-MU0 = 1.25663706212e-6
-C0 = 299792458.0
-
-
-# --------------------------------------------------------------- PCG32
-class Pcg32:
-    """engine/core/rng.c:3-23 (PCG-XSH-RR 64/32)."""
-
-    MASK = (1 << 64) - 1
-
-    def __init__(self, seed, stream=0):
-        self.state = 0
-        self.inc = ((stream << 1) | 1) & self.MASK
-        self.next_u32()
-        self.state = (self.state + seed) & self.MASK
-        self.next_u32()
-
-    def next_u32(self):
-        old = self.state
-        self.state = (old * 6364136223846793005 + self.inc) & self.MASK
-        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
-        rot = old >> 59
-        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
-
-    def uniform(self):
-        """rng_uniform: (r >> 8) / 2^24 as float32."""
-        return float(np.float32((self.next_u32() >> 8) / float(1 << 24)))
-
-
-# --------------------------------------------------------------- helpers
-def field_arrays(g):
-    """numpy views of the six host arrays of an fdtd_grid_t pointer."""
-    gg = g.contents
-    n = (gg.nx + 1) * (gg.ny + 1) * (gg.nz + 1)
-    return {name: np.ctypeslib.as_array(getattr(gg, name), shape=(n,))
-            for name in ("ex", "ey", "ez", "hx", "hy", "hz")}
-
-
-def pattern_fill(g, scale=1.0e-4):
-    """tests/periodic_wrapping_unit.c:23-43 fill pattern on every padded node."""
-    gg = g.contents
-    nx, ny, nz = gg.nx, gg.ny, gg.nz
-    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
-    f = field_arrays(g)
-    for c, name in enumerate(("ex", "ey", "ez", "hx", "hy", "hz")):
-        v = (31 * i + 17 * j + 7 * k + 5 * c) % 29
-        f[name][:] = ((v.astype(np.float64) - 14.0) * scale).ravel()
-
-
-def random_fill(g, seed, amp=1.0e-3):
-    rng = np.random.default_rng(seed)
-    for name, a in field_arrays(g).items():
-        a[:] = rng.uniform(-amp, amp, a.shape)
-
-
-def same_values(a, b):
-    """Bitwise equality treating +0 and -0 as equal."""
-    a = np.asarray(a)
-    b = np.asarray(b)
-    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
-
-
-def c1_cpml_params(npml=10):
-    """CPML of the 64^3 FP64 scene: m=3, ma=1, kappa_max=1, alpha_max=0.05, R0=1e-8."""
-    return capi.CpmlParams(npml, 3.0, 1.0, 1.0, 0.05, 1e-8)
-
-
-def modgauss_params(f0, bw):
-    """Modulated Gaussian with -20 dB full bandwidth bw: tau = 2 sqrt(ln 10)/(pi bw),
-    sigma = tau/sqrt 2, delay t0 = 4 sigma."""
-    tau = 2.0 * math.sqrt(math.log(10.0)) / (math.pi * bw)
-    sigma = tau / math.sqrt(2.0)
-    return capi.ModGaussParams(f0, 4.0 * sigma, tau)
-
-
-def c1_probes(n=64, src=(32, 32, 32), count=8, seed=1, lo=10, hi=54, min_dist=15.0):
-    """Seeded Ez probe nodes in [lo,hi)^3 at least min_dist cells from the source."""
-    rng = Pcg32(seed, 0)
-    out = []
-    while len(out) < count:
-        p = tuple(lo + rng.next_u32() % (hi - lo) for _ in range(3))
-        if math.dist(p, src) >= min_dist and p not in out:
-            out.append(p)
-    return out
-
-
-def c1_freqs(nf=64, f_lo=20e9, f_hi=40e9):
-    return np.array([f_lo + (f_hi - f_lo) * q / (nf - 1) for q in range(nf)], dtype=np.float64)
-
-
-class NativeRun:
-    """Holds a native fdtd_sim built on one library and the objects it needs."""
-
-    def __init__(self, lib, n, dx=1e-3, courant=0.99, periodic=0, cpml=None,
-                 precision=capi.FDTD_FP64, compensated=0, block=0, material_setup=None,
-                 init_fields=None):
-        self.lib = lib
-        nx, ny, nz = n
-        self.g = lib.fdtd_grid_create(nx, ny, nz, dx, courant)
-        assert self.g
-        if periodic:
-            assert lib.fdtd_grid_set_periodic(self.g, periodic) == 0
-        self.pml = lib.cpml_create(self.g, cpml) if cpml is not None else None
-        if cpml is not None:
-            assert self.pml
-        self.mat = lib.materials_create(self.g)
-        assert self.mat
-        if material_setup:
-            material_setup(lib, self.g, self.mat)
-        if init_fields:
-            init_fields(self.g)
-        opt = capi.SimOptions(precision, compensated, block)
-        self.sim = lib.fdtd_sim_create(self.g, self.pml, self.mat, C.byref(opt))
-        assert self.sim, lib.fdtd_last_error()
-        self._keep = []
-        self.probe_nf = {}
-
-    def add_point_source(self, comp, pos, wf_name, params, amp=1.0, hard=False):
-        self._keep.append(params)
-        wf = capi.waveform_ptr(self.lib, wf_name)
-        src = self.lib.source_make_point(pos[0], pos[1], pos[2], comp,
-                                         capi.SRC_HARD if hard else capi.SRC_SOFT, amp, wf,
-                                         C.cast(C.pointer(params), C.c_void_p))
-        assert self.lib.fdtd_sim_add_point_source(self.sim, C.byref(src)) == 0
-
-    def add_plane_source(self, comp, axis, index, wf_name, params, amp=1.0, after_e=0):
-        self._keep.append(params)
-        wf = capi.waveform_ptr(self.lib, wf_name)
-        assert self.lib.fdtd_sim_add_plane_source(self.sim, comp, axis, index, capi.SRC_SOFT, after_e, amp,
-                                                  wf, C.cast(C.pointer(params), C.c_void_p)) == 0
-
-    def add_probe(self, comp, pos, td_cap, freqs=None):
-        nf = 0 if freqs is None else len(freqs)
-        fp = None
-        if nf:
-            arr = np.ascontiguousarray(freqs, dtype=np.float64)
-            self._keep.append(arr)
-            fp = arr.ctypes.data_as(C.POINTER(C.c_double))
-        pid = self.lib.fdtd_sim_add_probe(self.sim, comp, pos[0], pos[1], pos[2], td_cap, fp, nf)
-        assert pid >= 0
-        self.probe_nf[pid] = nf
-        return pid
-
-    def add_plane_monitor(self, axis, index, freqs):
-        arr = np.ascontiguousarray(freqs, dtype=np.float64)
-        self._keep.append(arr)
-        mid = self.lib.fdtd_sim_add_plane_monitor(self.sim, axis, index,
-                                                  arr.ctypes.data_as(C.POINTER(C.c_double)), len(arr))
-        assert mid >= 0
-        return mid
-
-    def run(self, steps):
-        assert self.lib.fdtd_sim_run(self.sim, steps) == 0, self.lib.fdtd_last_error()
-
-    def td(self, pid, n):
-        out = np.zeros(n)
-        got = self.lib.fdtd_sim_probe_td(self.sim, pid, out.ctypes.data_as(C.POINTER(C.c_double)), n)
-        assert got == n
-        return out
-
-    def dft(self, pid):
-        nf = self.probe_nf[pid]
-        re = np.zeros(nf)
-        im = np.zeros(nf)
-        assert self.lib.fdtd_sim_probe_dft(self.sim, pid, re.ctypes.data_as(C.POINTER(C.c_double)),
-                                           im.ctypes.data_as(C.POINTER(C.c_double))) == 0
-        return re + 1j * im
-
-    def plane_dft(self, mid, which, shape):
-        re = np.zeros(shape)
-        im = np.zeros(shape)
-        assert self.lib.fdtd_sim_plane_dft(self.sim, mid, which, re.ctypes.data_as(C.POINTER(C.c_double)),
-                                           im.ctypes.data_as(C.POINTER(C.c_double))) == 0
-        return re + 1j * im
-
-    def fields(self):
-        assert self.lib.fdtd_sim_download(self.sim, self.g) == 0
-        return {k: v.copy() for k, v in field_arrays(self.g).items()}
-
-    def energy(self):
-        e = C.c_double()
-        h = C.c_double()
-        assert self.lib.fdtd_sim_energy(self.sim, C.byref(e), C.byref(h)) == 0
-        return e.value, h.value
-
-    def close(self):
-        self.lib.fdtd_sim_destroy(self.sim)
-        self.lib.materials_destroy(self.mat)
-        if self.pml:
-            self.lib.cpml_destroy(self.pml)
-        self.lib.fdtd_grid_destroy(self.g)
-
-
-def run_c1(lib, n=64, steps=2000, precision=capi.FDTD_FP64, compensated=0, n_probes=8, block=0):
-    """64^3 vacuum, 10-cell CPML, Ez modulated-Gaussian dipole at the centre,
-    8 seeded Ez probes with 64-frequency DFT (20-40 GHz)."""
-    c = n // 2
-    run = NativeRun(lib, (n, n, n), 1e-3, 0.99, cpml=c1_cpml_params(10), precision=precision,
-                    compensated=compensated, block=block)
-    run.add_point_source(capi.FIELD_EZ, (c, c, c), "source_waveform_modulated_gaussian",
-                         modgauss_params(30e9, 20e9))
-    freqs = c1_freqs()
-    probes = c1_probes(n, (c, c, c), n_probes, lo=10, hi=n - 10)
-    pids = [run.add_probe(capi.FIELD_EZ, p, steps, freqs) for p in probes]
-    run.run(steps)
-    td = np.stack([run.td(p, steps) for p in pids])
-    dft = np.stack([run.dft(p) for p in pids])
-    run.close()
-    return td, dft
+"""Test-side access to the seeded scene builders of rfsyn_amd.driver."""
+from rfsyn_amd.driver import *  # noqa: F401,F403
+from rfsyn_amd.driver import (Pcg32, NativeRun, c1_cpml_params, c1_freqs, c1_probes,  # noqa: F401
+                              field_arrays, modgauss_params, pattern_fill, random_fill, run_c1,
+                              same_values)
