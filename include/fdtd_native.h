@@ -37,8 +37,10 @@ typedef enum {
 
 typedef struct {
     fdtd_precision_t precision;
-    int dft_compensated;   /* 1: Neumaier-compensated double DFT sums;
-                              0: plain sums in the reference order      */
+    int dft_compensated;   /* point probes: 1 = Neumaier-compensated double
+                              DFT sums, 0 = plain sums in the reference
+                              order; plane monitors always use plain
+                              double sums                               */
     size_t dft_block;      /* steps buffered between DFT flushes (0: 32) */
 } fdtd_sim_options_t;
 

@@ -310,12 +310,16 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
             x[m->axis] = m->index;
             for (int w = 0; w < 2; w++)
                 for (size_t f = 0; f < m->nf; f++) {
+                    /* the twiddle depends on (f, t) only: same terms as
+                     * dft_add, evaluated once per frequency */
                     double om = 2.0 * M_PI * m->freqs[f];
+                    double cs = cos(om * tp), sn = sin(om * tp);
                     for (x[m->comp[0]] = 0; x[m->comp[0]] < m->na; x[m->comp[0]]++)
                         for (x[m->comp[1]] = 0; x[m->comp[1]] < m->nb; x[m->comp[1]]++) {
                             double v = field_value(g, m->comp[w], fdtd_index(g, x[0], x[1], x[2]));
                             size_t o = (f * m->na + x[m->comp[0]]) * m->nb + x[m->comp[1]];
-                            dft_add(&m->re[w][o], &m->im[w][o], v, om, tp, g->dt, s->opt.dft_compensated);
+                            m->re[w][o].s += v * cs * g->dt;
+                            m->im[w][o].s -= v * sn * g->dt;
                         }
                 }
         }

@@ -392,9 +392,6 @@ static int finalize(fdtd_sim_t *s) {
             if (dalloc(s, &m->ring[w], (size_t)s->K * cells * s->es) ||
                 dalloc(s, (void **)&m->acc_re[w], acc) || dalloc(s, (void **)&m->acc_im[w], acc))
                 return -1;
-            if (s->comp_sum &&
-                (dalloc(s, (void **)&m->cmp_re[w], acc) || dalloc(s, (void **)&m->cmp_im[w], acc)))
-                return -1;
         }
     }
     if (max_nf && dalloc(s, (void **)&s->d_tw, (size_t)2 * max_nf * s->K * sizeof(double))) return -1;
@@ -503,8 +500,7 @@ static int enqueue_flush(fdtd_sim_t *s, int count) {
         long long cells = (long long)m->na * m->nb;
         for (int w = 0; w < 2; w++)
             if (rf_launch_plane_dft(m->ring[w], s->prec, cells, s->K, m->nf, m->d_w, s->dt, s->d_step, count,
-                                    s->d_tw, m->acc_re[w], m->acc_im[w], m->cmp_re[w], m->cmp_im[w],
-                                    s->comp_sum, s->st))
+                                    s->d_tw, m->acc_re[w], m->acc_im[w], NULL, NULL, 0, s->st))
                 return -1;
     }
     return 0;
@@ -641,15 +637,6 @@ int fdtd_sim_plane_dft(fdtd_sim_t *s, int monitor, int which, double *re, double
     if (rf_d2h(re, m->acc_re[which], tot * sizeof(double), s->st) ||
         rf_d2h(im, m->acc_im[which], tot * sizeof(double), s->st))
         return -1;
-    if (s->comp_sum) {
-        double *c = malloc(tot * sizeof *c);
-        if (!c) return -1;
-        if (rf_d2h(c, m->cmp_re[which], tot * sizeof(double), s->st)) { free(c); return -1; }
-        for (size_t q = 0; q < tot; q++) re[q] += c[q];
-        if (rf_d2h(c, m->cmp_im[which], tot * sizeof(double), s->st)) { free(c); return -1; }
-        for (size_t q = 0; q < tot; q++) im[q] += c[q];
-        free(c);
-    }
     return 0;
 }
 

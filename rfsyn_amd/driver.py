@@ -233,3 +233,82 @@ def run_c1(lib, n=64, steps=2000, precision=capi.FDTD_FP64, compensated=0, n_pro
     dft = np.stack([run.dft(p) for p in pids])
     run.close()
     return td, dft
+
+
+# ------------------------------------------------------------ C3 scene
+def c3_layers(seed=3, nz=256, npml=16):
+    """Seeded layer stack of the periodic dispersive scene along z (cell
+    indices): Drude grating, gap, 2-pole Lorentz layer, gap, eps=4 substrate."""
+    rng = Pcg32(seed, 0)
+    z = 64
+    t_g = 8 + rng.next_u32() % 17
+    gap1 = 4 + rng.next_u32() % 9
+    t_l = 16 + rng.next_u32() % 33
+    gap2 = 4 + rng.next_u32() % 9
+    t_s = 24 + rng.next_u32() % 41
+    duty = 0.3 + 0.4 * rng.uniform()
+    grating = (z, z + t_g)
+    lorentz = (grating[1] + gap1, grating[1] + gap1 + t_l)
+    sub = (lorentz[1] + gap2, lorentz[1] + gap2 + t_s)
+    assert sub[1] <= nz - npml - 8
+    return {"grating": grating, "lorentz": lorentz, "substrate": sub, "duty": duty, "period": 32}
+
+
+def c3_materials(lib, g, mat, layers):
+    d = capi.MaterialDispersive()
+    d.eps_inf = 1.0
+    d.n_drude = 1
+    d.drude_wp[0] = 1.37e16
+    d.drude_gamma[0] = 1e14
+    did = C.c_uint8()
+    assert lib.materials_register_dispersive(mat, C.byref(d), C.byref(did)) == 0
+    lz = capi.MaterialDispersive()
+    lz.eps_inf = 2.25
+    lz.n_lorentz = 2
+    for q, (de, f0) in enumerate(((1.0, 60e9), (0.5, 90e9))):
+        lz.lorentz_deps[q] = de
+        lz.lorentz_w0[q] = 2 * math.pi * f0
+        lz.lorentz_delta[q] = 0.05 * 2 * math.pi * f0
+    lid = C.c_uint8()
+    assert lib.materials_register_dispersive(mat, C.byref(lz), C.byref(lid)) == 0
+    gg = g.contents
+    nx, ny = gg.nx, gg.ny
+    per = layers["period"]
+    width = max(1, int(round(layers["duty"] * per)))
+    k0, k1 = layers["grating"]
+    for i0 in range(0, nx, per):
+        assert lib.materials_assign_box(mat, g, i0, min(i0 + width, nx), 0, ny, k0, k1, did.value) == 0
+    k0, k1 = layers["lorentz"]
+    assert lib.materials_assign_box(mat, g, 0, nx, 0, ny, k0, k1, lid.value) == 0
+    k0, k1 = layers["substrate"]
+    assert lib.materials_set_box(mat, g, 0, nx, 0, ny, k0, k1, 4.0, 0.0) == 0
+
+
+def c3_probes(n, layers, count=8, seed=5):
+    """Seeded Ex probes in the vacuum gaps and above/below the stack."""
+    rng = Pcg32(seed, 0)
+    zs = [layers["grating"][0] - 12, layers["grating"][1] + 1, layers["substrate"][1] + 6,
+          layers["lorentz"][0] - 2]
+    out = []
+    for q in range(count):
+        out.append((rng.next_u32() % n[0], rng.next_u32() % n[1], zs[q % len(zs)]))
+    return out
+
+
+def build_c3(lib, precision, td_cap, n=(512, 512, 256), npml=16, seed=3, monitors=True):
+    """512x512x256, dx = 0.5 mm, periodic x/y, 16-cell CPML on +-z, Ex soft
+    plane source at k = npml + 8 (40 GHz, 30 GHz -20 dB bandwidth), Drude
+    grating / 2-pole Lorentz / eps=4 stack, two full-plane monitors at 128
+    frequencies over 25-55 GHz, plus 8 Ex point probes with DFTs."""
+    layers = c3_layers(seed, n[2], npml)
+    run = NativeRun(lib, n, 0.5e-3, 0.99, periodic=3, cpml=capi.CpmlParams(npml, 3.0, 1.0, 1.0, 0.05, 1e-8),
+                    precision=precision, compensated=1,
+                    material_setup=lambda lb, g, m: c3_materials(lb, g, m, layers))
+    run.add_plane_source(capi.FIELD_EX, 2, npml + 8, "source_waveform_modulated_gaussian",
+                         modgauss_params(40e9, 30e9))
+    freqs = np.linspace(25e9, 55e9, 128)
+    mons = []
+    if monitors:
+        mons = [run.add_plane_monitor(2, npml + 4, freqs), run.add_plane_monitor(2, n[2] - npml - 4, freqs)]
+    pids = [run.add_probe(capi.FIELD_EX, p, td_cap, freqs[::2]) for p in c3_probes(n, layers)]
+    return run, pids, mons, layers
