@@ -1,6 +1,6 @@
 """Benchmark of the rfsyn FDTD core on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1]
 
 One step is one full Yee step (H update, sources, E update with CPML,
 materials and dispersive ADE, probes and plane monitors) over the whole
@@ -18,9 +18,9 @@ its average event-timed duration, against the 8 TB/s HBM3E peak.
 
 CPU baseline: c1 runs the rfsyn reference compiled from its own sources
 (oracle/_ref, kind "reference") on one pinned core over the same
-low-level calls its validators use; c3 needs Drude/Lorentz ADE and plane
-sources, which the reference does not have, so it times the CPU oracle
-(kind "port") on a bounded sample of steps of the same scene.
+low-level calls its validators use; c2 (3x3 tensors) and c3 (Drude/Lorentz
+ADE, plane sources) need physics the reference does not have, so they time
+the CPU oracle (kind "port") on a bounded sample of steps of the same scene.
 """
 import argparse
 import ctypes as C
@@ -167,7 +167,42 @@ class C3:
         return el, self.n[0] * self.n[1] * self.n[2], "port"
 
 
-WORKLOADS = {"c1": C1, "c3": C3}
+class C2:
+    """256^3, dx = 1 mm, 12-cell CPML (C1 grading), 40 seeded spheres
+    (radius 4-24 cells, eps_r 1-12) and an eps_r = 6 slab tilted 23 degrees
+    about x, with 3x3 subpixel inverse-permittivity tensors on interface
+    edges; Ez dipole at the centre, 16 Ez probes with 64-frequency DFTs,
+    3000 steps.  FP32 headline, FP64 alongside."""
+    steps_total = 3000
+    headline = capi.FDTD_FP32
+    n = (256, 256, 256)
+    npml = (12, 12, 12)
+
+    def __init__(self):
+        self.stats = {}
+        self.tag_bytes = 2
+        self.ade_values = 0
+        self.config = {"workload": "c2: 256^3 40 spheres + 23-degree tilted slab, 3x3 subpixel tensors, "
+                                   "12-cell CPML, 16 Ez probes x 64-freq DFT, 3000 steps",
+                       "grid": list(self.n), "dx_m": 1e-3, "courant": 0.99}
+
+    def build(self, lib, prec, td_cap):
+        from rfsyn_amd.driver import build_c2
+        run, pids = build_c2(lib, prec, td_cap, stats_out=self.stats)
+        self.config["scene"] = dict(self.stats)
+        return run, pids, []
+
+    def cpu_sample(self, lib, steps):
+        run, pids, _ = self.build(lib, capi.FDTD_FP64, 0)
+        run.run(1)
+        t0 = time.perf_counter()
+        run.run(steps)
+        el = time.perf_counter() - t0
+        run.close()
+        return el, self.n[0] * self.n[1] * self.n[2], "port"
+
+
+WORKLOADS = {"c1": C1, "c2": C2, "c3": C3}
 
 
 def cpu_baseline(wl, budget_s=15.0):
@@ -181,8 +216,8 @@ def cpu_baseline(wl, budget_s=15.0):
         if not os.path.exists(path):
             return None
         lib = capi.bind(C.CDLL(path))
-        if isinstance(wl, C3):
-            steps = 2     # about 6 s per step on one core; the set-up dominates
+        if isinstance(wl, (C2, C3)):
+            steps = 2 if isinstance(wl, C3) else 4   # seconds per step on one core
         else:
             t, cells, kind = wl.cpu_sample(lib, 5)
             steps = max(5, min(wl.steps_total, int(budget_s / max(t / 5, 1e-9))))
@@ -201,7 +236,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")

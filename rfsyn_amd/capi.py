@@ -117,6 +117,25 @@ class SimConfig(C.Structure):
                 ("probe_component", C.c_int)]
 
 
+class Shape(C.Structure):
+    """fdtd_shape_t (include/fdtd_scene.h)."""
+    _fields_ = [("kind", C.c_int), ("eps_r", C.c_double), ("p", C.c_double * 3), ("radius", C.c_double),
+                ("normal", C.c_double * 3), ("half_thickness", C.c_double), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3)]
+
+
+class SceneOptions(C.Structure):
+    _fields_ = [("supersample", C.c_int), ("keepout", C.c_size_t)]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("nodes_painted", C.c_size_t), ("override_edges", C.c_size_t),
+                ("tensor_edges", C.c_size_t), ("scalar_mixed_edges", C.c_size_t)]
+
+
+SHAPE_SPHERE, SHAPE_SLAB, SHAPE_BOX = 0, 1, 2
+
+
 class SimOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("dft_compensated", C.c_int),
                 ("dft_block", C.c_size_t)]
@@ -194,6 +213,8 @@ EXTENSION_API = {
     "fdtd_sim_set_profiling": (C.c_int, [_vp, C.c_int]),
     "fdtd_sim_kernel_times": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_long)]),
     "fdtd_last_error": (C.c_char_p, []),
+    "fdtd_scene_paint": (C.c_int, [_vp, _gp, C.POINTER(Shape), _sz, C.POINTER(SceneOptions),
+                                   C.POINTER(SceneStats)]),
 }
 
 

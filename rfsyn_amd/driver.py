@@ -312,3 +312,64 @@ def build_c3(lib, precision, td_cap, n=(512, 512, 256), npml=16, seed=3, monitor
         mons = [run.add_plane_monitor(2, npml + 4, freqs), run.add_plane_monitor(2, n[2] - npml - 4, freqs)]
     pids = [run.add_probe(capi.FIELD_EX, p, td_cap, freqs[::2]) for p in c3_probes(n, layers)]
     return run, pids, mons, layers
+
+
+# ------------------------------------------------------------ C2 scene
+def c2_shapes(n=256, npml=12, n_spheres=40, r_lo=4.0, r_hi=24.0, seed=2, slab_half=10.0):
+    """Seeded heterogeneous dielectric: a slab of eps_r = 6 through the grid
+    centre tilted 23 degrees about x (normal (0, -sin 23, cos 23)), then
+    n_spheres spheres (radius U[r_lo, r_hi] cells, eps_r U[1, 12], centre
+    uniform in the interior at least r + 2 cells from the CPML)."""
+    rng = Pcg32(seed, 0)
+    th = math.radians(23.0)
+    shapes = []
+    s = capi.Shape()
+    s.kind = capi.SHAPE_SLAB
+    s.eps_r = 6.0
+    s.p[:] = (n / 2.0, n / 2.0, n / 2.0)
+    s.normal[:] = (0.0, -math.sin(th), math.cos(th))
+    s.half_thickness = slab_half
+    s.lo[:] = (0.0, 0.0, 0.0)
+    s.hi[:] = (float(n), float(n), float(n))
+    shapes.append(s)
+    for _ in range(n_spheres):
+        r = r_lo + (r_hi - r_lo) * rng.uniform()
+        eps = 1.0 + 11.0 * rng.uniform()
+        lo, hi = npml + 2 + r, n - npml - 2 - r
+        c = [lo + (hi - lo) * rng.uniform() for _ in range(3)]
+        s = capi.Shape()
+        s.kind = capi.SHAPE_SPHERE
+        s.eps_r = eps
+        s.p[:] = c
+        s.radius = r
+        shapes.append(s)
+    return shapes
+
+
+def c2_paint(lib, g, mat, shapes, npml, stats_out=None):
+    arr = (capi.Shape * len(shapes))(*shapes)
+    opt = capi.SceneOptions(4, npml + 2)
+    st = capi.SceneStats()
+    assert lib.fdtd_scene_paint(mat, g, arr, len(shapes), C.byref(opt), C.byref(st)) == 0
+    if stats_out is not None:
+        stats_out.update(nodes=st.nodes_painted, overrides=st.override_edges, tensors=st.tensor_edges,
+                         scalar_mixed=st.scalar_mixed_edges)
+
+
+def build_c2(lib, precision, td_cap, n=256, npml=12, n_spheres=40, r_lo=4.0, r_hi=24.0, n_probes=16,
+             seed=2, stats_out=None):
+    """256^3, dx = 1 mm, 12-cell CPML on all faces (C1 grading), the seeded
+    sphere + tilted-slab scene with 3x3 subpixel tensors on interface
+    edges, Ez modulated-Gaussian dipole at the centre, 16 seeded Ez probes
+    with 64-frequency DFTs over 20-40 GHz."""
+    shapes = c2_shapes(n, npml, n_spheres, r_lo, r_hi, seed)
+    run = NativeRun(lib, (n, n, n), 1e-3, 0.99, cpml=c1_cpml_params(npml), precision=precision,
+                    compensated=1,
+                    material_setup=lambda lb, g, m: c2_paint(lb, g, m, shapes, npml, stats_out))
+    c = n // 2
+    run.add_point_source(capi.FIELD_EZ, (c, c, c), "source_waveform_modulated_gaussian",
+                         modgauss_params(30e9, 20e9))
+    freqs = c1_freqs()
+    probes = c1_probes(n, (c, c, c), n_probes, seed=seed, lo=npml + 3, hi=n - npml - 3)
+    pids = [run.add_probe(capi.FIELD_EZ, p, td_cap, freqs) for p in probes]
+    return run, pids

@@ -136,3 +136,30 @@ def test_dispersive_periodic_stack_fp64(gpu, oracle):
     assert _dft_close(a["T"], b["T"])
     for x, y in zip(a["energy"], b["energy"]):
         assert abs(x - y) <= 1e-12 * abs(y)
+
+
+def _c2_small(lib, precision, steps):
+    from rfsyn_amd.driver import build_c2
+    run, pids = build_c2(lib, precision, steps, n=48, npml=6, n_spheres=5, r_lo=3.0, r_hi=6.0, n_probes=4)
+    run.run(steps)
+    out = {"td": np.stack([run.td(p, steps) for p in pids]), "dft": np.stack([run.dft(p) for p in pids]),
+           "fields": run.fields()}
+    run.close()
+    return out
+
+
+def test_smoothed_dielectric_fp64(gpu, oracle):
+    """Spheres + tilted slab with 3x3 subpixel tensors (48^3, 6-cell CPML)."""
+    a = _c2_small(gpu, capi.FDTD_FP64, 150)
+    b = _c2_small(oracle, capi.FDTD_FP64, 150)
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
+    assert _dft_close(a["dft"], b["dft"])
+
+
+def test_smoothed_dielectric_fp32_vs_fp64(gpu):
+    a = _c2_small(gpu, capi.FDTD_FP64, 400)
+    b = _c2_small(gpu, capi.FDTD_FP32, 400)
+    peak = np.max(np.abs(a["dft"]), axis=1, keepdims=True)
+    assert np.max(np.abs(b["dft"] - a["dft"]) / peak) <= 1e-4
