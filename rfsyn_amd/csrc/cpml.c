@@ -152,7 +152,7 @@ static int upload_profile(rf_cpml_dev_t *c, const double *src, size_t n, int pre
     return 0;
 }
 
-int rf_cpml_build(const cpml_t *p, int prec, rf_cpml_dev_t *c) {
+int rf_cpml_build(const cpml_t *p, int prec, int with_h2, rf_cpml_dev_t *c) {
     memset(c, 0, sizeof *c);
     size_t n[3] = {p->nx, p->ny, p->nz};
     cpml_t *pp = (cpml_t *)p;
@@ -181,13 +181,21 @@ int rf_cpml_build(const cpml_t *p, int prec, rf_cpml_dev_t *c) {
             ext[ax] = (size_t)c->d.len[ax];
             size_t cnt = ext[0] * ext[1] * ext[2];
             if (!cnt) continue;
-            void *h = NULL, *e = NULL;
+            void *h = NULL, *h2 = NULL, *e = NULL;
             if (rf_malloc(&h, cnt * es) || push_owned(c, h) || rf_malloc(&e, cnt * es) || push_owned(c, e)) {
                 rf_cpml_release(c);
                 return -1;
             }
             c->bytes += 2 * cnt * es;
+            if (with_h2) {   /* second H-psi set for the double-buffered fused step */
+                if (rf_malloc(&h2, cnt * es) || push_owned(c, h2)) {
+                    rf_cpml_release(c);
+                    return -1;
+                }
+                c->bytes += cnt * es;
+            }
             c->d.psi_h[comp][s] = h;
+            c->psi_h2[comp][s] = h2;
             c->d.psi_e[comp][s] = e;
         }
     return 0;
@@ -202,7 +210,7 @@ rf_cpml_dev_t *rf_cpml_dev(cpml_t *p) {
     if (p->dev) return (rf_cpml_dev_t *)p->dev;
     rf_cpml_dev_t *c = calloc(1, sizeof *c);
     if (!c) return NULL;
-    if (rf_cpml_build(p, 0, c) != 0) { free(c); return NULL; }
+    if (rf_cpml_build(p, 0, 0, c) != 0) { free(c); return NULL; }
     p->dev = (struct cpml_device *)c;
     return c;
 }

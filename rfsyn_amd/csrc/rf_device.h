@@ -149,6 +149,37 @@ typedef struct {
     const double *wave;         /* value per absolute step */
 } rf_plane_src_t;
 
+/* Fused leapfrog step: one kernel computes H^{n+1/2} and then E^{n+1}
+ * from the "A" buffers (E^n, H^{n-1/2}) into the "B" buffers, marching
+ * along x over (j,k) tiles and recomputing the one-cell H halo of each
+ * tile.  psi_h is double-buffered (read cp.psi_h, write psi_hb) because
+ * halo recomputation reads a neighbour's psi; psi_e and the ADE state are
+ * updated in place.  Every node of B is written (copies where a component
+ * is not updated), so B is a complete state.  Pre-E point and plane
+ * sources and post-E plane sources are applied inside the kernel in the
+ * order of the step loop. */
+#define RF_LEAP_MAX_PTS 4
+#define RF_LEAP_MAX_PLANES 4
+typedef struct {
+    rf_geom_t g;
+    const void *ea[3], *ha[3];
+    void *eb[3], *hb[3];
+    double cb, dx;
+    rf_cpml_t cp;
+    void *psi_hb[3][2];
+    rf_mat_t mat;
+    int n_pts;
+    rf_point_src_t pts[RF_LEAP_MAX_PTS];
+    int n_planes;
+    rf_plane_src_t planes[RF_LEAP_MAX_PLANES];
+    int plane_after[RF_LEAP_MAX_PLANES];
+    const long long *d_step;
+    int chunk;                  /* x planes per block (0: all) */
+} rf_leap_t;
+
+/* has_cpml selects the CPML variant; tag_bytes 0 = vacuum (mat.vac_ca). */
+int rf_launch_leapfrog(int prec, int has_cpml, const rf_leap_t *a, rf_stream_t *s);
+
 /* ----------------------------------------------------------- launches
  * `prec` selects the field precision: 0 = double, 1 = float.  All
  * launches go to stream s and return 0 or -1. */

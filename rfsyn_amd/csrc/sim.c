@@ -98,8 +98,14 @@ struct fdtd_sim {
     double *d_energy;     /* partials + 2 results */
     int n_partials;
 
-    rf_graph_t *graph;
+    rf_graph_t *graphs[2];   /* captured block per buffer parity */
     size_t bytes;
+
+    /* fused leapfrog path: state in f, next state written to f2, then swapped */
+    int fused;
+    int parity;
+    void *f2[6];
+    rf_point_src_t *h_pts;   /* host copy of the point sources (device wave tables) */
 
     /* kernel timing with HIP events (profiling mode launches directly) */
     int profiling;
@@ -165,7 +171,7 @@ fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml, const mater
         if (dalloc(s, &s->f[q], s->cells * s->es)) goto fail;
     if (upload_fields(s, g)) goto fail;
     if (pml) {
-        if (rf_cpml_build(pml, s->prec, &s->cp) != 0) goto fail;
+        if (rf_cpml_build(pml, s->prec, 1, &s->cp) != 0) goto fail;
         s->has_cpml = 1;
         s->bytes += s->cp.bytes;
     }
@@ -189,8 +195,10 @@ fail:
 void fdtd_sim_destroy(fdtd_sim_t *s) {
     if (!s) return;
     if (s->st) rf_sync(s->st);
-    rf_graph_destroy(s->graph);
-    for (int q = 0; q < 6; q++) rf_free(s->f[q]);
+    rf_graph_destroy(s->graphs[0]);
+    rf_graph_destroy(s->graphs[1]);
+    for (int q = 0; q < 6; q++) { rf_free(s->f[q]); rf_free(s->f2[q]); }
+    free(s->h_pts);
     if (s->has_cpml) rf_cpml_release(&s->cp);
     if (s->has_mat) rf_materials_release(&s->mat);
     for (size_t w = 0; w < s->n_waves; w++) rf_free(s->waves[w].d_tab);
@@ -332,8 +340,27 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index, const doub
     return (int)s->n_mons - 1;
 }
 
+/* The fused leapfrog kernel carries up to RF_LEAP_MAX_PTS point and
+ * RF_LEAP_MAX_PLANES plane sources in its arguments; RFSYN_FUSED=0 in the
+ * environment selects the separate H and E kernels instead. */
+static int want_fused(const fdtd_sim_t *s) {
+    const char *env = getenv("RFSYN_FUSED");
+    if (env && env[0] == '0') return 0;
+    return s->n_pts <= RF_LEAP_MAX_PTS && s->n_planes <= RF_LEAP_MAX_PLANES;
+}
+
 /* Device arrays for probes and monitors, allocated once at the first run. */
 static int finalize(fdtd_sim_t *s) {
+    s->fused = want_fused(s);
+    if (s->fused) {
+        /* B starts as a copy of A so nodes no kernel writes agree */
+        for (int q = 0; q < 6; q++)
+            if (dalloc(s, &s->f2[q], s->cells * s->es) ||
+                rf_d2d(s->f2[q], s->f[q], s->cells * s->es, s->st))
+                return -1;
+        if (rf_sync(s->st)) return -1;
+        if (s->K & 1) s->K++;   /* a captured block must return to its buffer parity */
+    }
     int P = (int)s->n_probes;
     size_t cap = 0;
     for (size_t p = 0; p < s->n_probes; p++) if (s->probes[p].td_cap > cap) cap = s->probes[p].td_cap;
@@ -438,13 +465,75 @@ static int ensure_waves(fdtd_sim_t *s, size_t need) {
         int rc = 0;
         if (!s->d_pts) rc = dalloc(s, (void **)&s->d_pts, s->n_pts * sizeof *hp);
         if (!rc) rc = rf_h2d(s->d_pts, hp, s->n_pts * sizeof *hp, s->st) || rf_sync(s->st);
-        free(hp);
+        free(s->h_pts);
+        s->h_pts = hp;
         if (rc) return -1;
     }
     for (size_t p = 0; p < s->n_planes; p++) s->planes[p].d.wave = s->waves[s->planes[p].wave].d_tab;
-    /* plane sources pass their table by value: a captured block is stale */
-    rf_graph_destroy(s->graph);
-    s->graph = NULL;
+    /* sources pass their tables by value: captured blocks are stale */
+    for (int q = 0; q < 2; q++) {
+        rf_graph_destroy(s->graphs[q]);
+        s->graphs[q] = NULL;
+    }
+    return 0;
+}
+
+/* x planes per block of the fused kernel: enough blocks to fill the
+ * chip several times over, each marching at least 16 planes. */
+static int leap_chunk(const fdtd_sim_t *s) {
+    const int TJ = 8;
+    long long kn = (s->per & 4) ? (long long)s->nz : (long long)s->nz + 1;
+    long long jn = (s->per & 2) ? (long long)s->ny : (long long)s->ny + 1;
+    long long in = (s->per & 1) ? (long long)s->nx : (long long)s->nx + 1;
+    long long tiles = ((kn + 63) / 64) * ((jn + TJ - 1) / TJ);
+    long long chunks = (2048 + tiles - 1) / tiles;
+    long long chunk = (in + chunks - 1) / chunks;
+    if (chunk < 16) chunk = 16;
+    return (int)chunk;
+}
+
+static int enqueue_fused(fdtd_sim_t *s) {
+    rf_leap_t a;
+    memset(&a, 0, sizeof a);
+    a.g = s->geom;
+    for (int q = 0; q < 3; q++) {
+        a.ea[q] = s->f[q];
+        a.ha[q] = s->f[3 + q];
+        a.eb[q] = s->f2[q];
+        a.hb[q] = s->f2[3 + q];
+    }
+    a.cb = s->dt / (MU0 * s->dx);
+    a.dx = s->dx;
+    if (s->has_cpml) {
+        a.cp = s->cp.d;
+        for (int c = 0; c < 3; c++)
+            for (int q = 0; q < 2; q++) a.psi_hb[c][q] = s->cp.psi_h2[c][q];
+    }
+    a.mat = s->has_mat ? s->mat.d : s->vac;
+    a.n_pts = (int)s->n_pts;
+    for (size_t p = 0; p < s->n_pts; p++) a.pts[p] = s->h_pts[p];
+    a.n_planes = (int)s->n_planes;
+    for (size_t p = 0; p < s->n_planes; p++) {
+        a.planes[p] = s->planes[p].d;
+        a.plane_after[p] = s->planes[p].after_e;
+    }
+    a.d_step = s->d_step;
+    a.chunk = leap_chunk(s);
+    if (rf_launch_leapfrog(s->prec, s->has_cpml, &a, s->st)) return -1;
+    /* B now holds state n+1 */
+    for (int q = 0; q < 6; q++) {
+        void *t = s->f[q];
+        s->f[q] = s->f2[q];
+        s->f2[q] = t;
+    }
+    if (s->has_cpml)
+        for (int c = 0; c < 3; c++)
+            for (int q = 0; q < 2; q++) {
+                void *t = s->cp.d.psi_h[c][q];
+                s->cp.d.psi_h[c][q] = s->cp.psi_h2[c][q];
+                s->cp.psi_h2[c][q] = t;
+            }
+    s->parity ^= 1;
     return 0;
 }
 
@@ -457,26 +546,35 @@ static int enqueue_step(fdtd_sim_t *s) {
     const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
     const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
     double cb = s->dt / (MU0 * s->dx);
-    if (mark(s, 0)) return -1;
-    if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
-    if (mark(s, 1)) return -1;
-    if (s->n_pts && rf_launch_point_sources(s->prec, s->f, s->d_pts, (int)s->n_pts, s->d_step, 0, s->st))
-        return -1;
-    for (size_t p = 0; p < s->n_planes; p++)
-        if (!s->planes[p].after_e &&
-            rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+    if (s->fused) {
+        /* one kernel for H, sources and E; timed as the "H" slot */
+        if (mark(s, 0) || enqueue_fused(s) || mark(s, 1) || mark(s, 2) || mark(s, 3)) return -1;
+        if (s->profiling) s->ev_used += 4;
+        if (s->has_mat && s->mat.n_tensor &&
+            rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
             return -1;
-    if (mark(s, 2)) return -1;
-    if (rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) return -1;
-    if (mark(s, 3)) return -1;
-    if (s->profiling) s->ev_used += 4;
-    if (s->has_mat && s->mat.n_tensor &&
-        rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
-        return -1;
-    for (size_t p = 0; p < s->n_planes; p++)
-        if (s->planes[p].after_e &&
-            rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+    } else {
+        if (mark(s, 0)) return -1;
+        if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
+        if (mark(s, 1)) return -1;
+        if (s->n_pts && rf_launch_point_sources(s->prec, s->f, s->d_pts, (int)s->n_pts, s->d_step, 0, s->st))
             return -1;
+        for (size_t p = 0; p < s->n_planes; p++)
+            if (!s->planes[p].after_e &&
+                rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+                return -1;
+        if (mark(s, 2)) return -1;
+        if (rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) return -1;
+        if (mark(s, 3)) return -1;
+        if (s->profiling) s->ev_used += 4;
+        if (s->has_mat && s->mat.n_tensor &&
+            rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
+            return -1;
+        for (size_t p = 0; p < s->n_planes; p++)
+            if (s->planes[p].after_e &&
+                rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+                return -1;
+    }
     for (size_t q = 0; q < s->n_mons; q++) {
         smon_t *m = &s->mons[q];
         for (int w = 0; w < 2; w++)
@@ -514,7 +612,9 @@ static int capture_block(fdtd_sim_t *s) {
     rf_graph_t *g = NULL;
     int rc2 = rf_graph_end(s->st, &g);
     if (rc || rc2) { rf_graph_destroy(g); return -1; }
-    s->graph = g;
+    /* capture advanced the host-side buffer bookkeeping by K steps; K is
+     * even on the fused path, so the parity is back where it started */
+    s->graphs[s->parity] = g;
     return 0;
 }
 
@@ -571,11 +671,11 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
     while (left) {
         if (s->profiling && s->ev_used + 4 > s->n_ev && prof_collect(s) != 0) return -1;
         if (!s->profiling && s->pending == 0 && left >= (size_t)s->K) {
-            if (!s->graph && capture_block(s) != 0) {
+            if (!s->graphs[s->parity] && capture_block(s) != 0) {
                 rf_set_error("graph capture: %s", rf_error_string());
                 return -1;
             }
-            if (rf_graph_launch(s->graph, s->st)) return -1;
+            if (rf_graph_launch(s->graphs[s->parity], s->st)) return -1;
             s->steps += (size_t)s->K;
             left -= (size_t)s->K;
             continue;
@@ -600,6 +700,8 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
 }
 
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }
+
+int fdtd_sim_fused(const fdtd_sim_t *s) { return s ? s->fused : -1; }
 
 long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n) {
     if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out)) return -1;

@@ -18,56 +18,11 @@
 
 #include "rf_device.h"
 #include "rf_hip.h"
+#include "yee_device.h"
 
 namespace {
 
-struct Geo {
-    int nx, ny, nz;
-    bool px, py, pz;
-    long long sx, sy;
-};
-
-__device__ __forceinline__ Geo geo_of(const rf_geom_t &g) {
-    Geo o;
-    o.nx = g.n[0]; o.ny = g.n[1]; o.nz = g.n[2];
-    o.px = g.per & 1; o.py = g.per & 2; o.pz = g.per & 4;
-    o.sx = g.sx; o.sy = g.sy;
-    return o;
-}
-
-// Compressed slab coordinate of q along axis a, or -1 outside the slab.
-__device__ __forceinline__ int slab_coord(const rf_cpml_t &c, int a, int q) {
-    if (q < c.lo[a]) return q;
-    if (q >= c.hi[a]) return c.lo[a] + q - c.hi[a];
-    return -1;
-}
-
-__device__ __forceinline__ long long slab_index(const rf_cpml_t &c, const Geo &g, int a, int cq,
-                                                int i, int j, int k) {
-    long long ny1 = (a == 1) ? c.len[1] : g.ny + 1;
-    long long nz1 = (a == 2) ? c.len[2] : g.nz + 1;
-    long long ii = (a == 0) ? cq : i, jj = (a == 1) ? cq : j, kk = (a == 2) ? cq : k;
-    return (ii * ny1 + jj) * nz1 + kk;
-}
-
-// cpml_term of cpml.h:46-51 with psi held only in slabs; outside the slab
-// psi stays exactly zero, so the term is diff*kinv + dx*0 as in the
-// reference's full-size arrays.
-template <typename T>
-__device__ __forceinline__ T stretch(T diff, const rf_cpml_t &c, const Geo &g, bool h, int axis,
-                                     int q, T *psi_arr, int i, int j, int k, T dx) {
-    const T *kinv = static_cast<const T *>(h ? c.kinv_h[axis] : c.kinv_m[axis]);
-    const T *bb = static_cast<const T *>(h ? c.b_h[axis] : c.b_m[axis]);
-    const T *aa = static_cast<const T *>(h ? c.a_h[axis] : c.a_m[axis]);
-    int cq = slab_coord(c, axis, q);
-    T psi = T(0);
-    if (cq >= 0) {
-        long long s = slab_index(c, g, axis, cq, i, j, k);
-        psi = bb[q] * psi_arr[s] + aa[q] * diff;
-        psi_arr[s] = psi;
-    }
-    return diff * kinv[q] + dx * psi;
-}
+using namespace rfk;
 
 // ----------------------------------------------------------------- H
 
@@ -124,39 +79,6 @@ k_update_h(rf_geom_t gg, const T *__restrict__ ex, const T *__restrict__ ey,
 }
 
 // ----------------------------------------------------------------- E
-
-template <typename T> struct EntryOf;
-template <> struct EntryOf<double> { using type = rf_entry64_t; using ade = rf_ade64_t; };
-template <> struct EntryOf<float> { using type = rf_entry32_t; using ade = rf_ade32_t; };
-
-struct NoTag { };
-
-template <typename TagT>
-__device__ __forceinline__ unsigned load_tag(const void *p, long long idx) {
-    return static_cast<unsigned>(static_cast<const TagT *>(p)[idx]);
-}
-
-// Trapezoidal ADE edge (materials.h): returns E^{n+1} and advances J, P.
-template <typename T>
-__device__ __forceinline__ T ade_edge(const typename EntryOf<T>::ade &a, T ca, T cb, T e_old,
-                                      T curl_a, T curl_b, T *J, T *P, long long slot_stride,
-                                      long long off) {
-    T u[RF_MAX_SLOTS];
-    T sum_u = T(0);
-    for (int s = 0; s < a.n_poles; s++) {
-        const long long o = s * slot_stride + off;
-        u[s] = a.has_p[s] ? a.kj[s] * J[o] - a.kp[s] * P[o] : a.kj[s] * J[o];
-        sum_u += u[s];
-    }
-    const T e_new = ca * e_old + cb * (curl_a - curl_b) - a.Cu * sum_u;
-    for (int s = 0; s < a.n_poles; s++) {
-        const long long o = s * slot_stride + off;
-        const T S = u[s] + a.v[s] * (e_new + e_old);
-        J[o] = S - J[o];
-        if (a.has_p[s]) P[o] = P[o] + a.hdt * S;
-    }
-    return e_new;
-}
 
 template <typename T, typename TagT, bool CPML>
 __device__ __forceinline__ void e_component(int c, T *__restrict__ E, long long idx, T dA, T dB,

@@ -1,0 +1,231 @@
+"""Pins for the oracle's extensions that have no reference implementation.
+
+* Dispersive ADE: on a fully periodic grid filled with one dispersive
+  medium and a uniform E field, the curl vanishes, H stays zero and each
+  edge evolves as the trapezoidal rule applied to the linear ODE
+  x' = A x, x = (E, J_1, P_1, ...):  eps0 eps_inf E' = -sigma E - sum J,
+  J' = -w0^2 P - g J + eps0 A_p E, P' = J.  The trapezoidal (bilinear)
+  propagator (I - dt/2 A)^-1 (I + dt/2 A) is built here from the ODE with
+  numpy and compared with the oracle's field history.
+* 3x3 tensors: the operator H -> (H update of (E update of H)) is
+  -cb C^T Xi C; the pair rule must make it symmetric with zero row sums
+  (uniform H is curl-free), the 3D form of the reference's operator test
+  (tests/materials_tensor_unit.c:1057-1165).  A diagonal tensor 1/eps I is
+  the scalar update (materials_tensor_unit.c:394-449).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from rfsyn_amd import capi
+from scenes import field_arrays
+
+EPS0 = 8.8541878128e-12
+
+
+def _uniform_dispersive(oracle, disp, steps, e0=1.0, n=4):
+    g = oracle.fdtd_grid_create(n, n, n, 1e-3, 0.99)
+    assert oracle.fdtd_grid_set_periodic(g, 7) == 0
+    m = oracle.materials_create(g)
+    mid = C.c_uint8()
+    assert oracle.materials_register_dispersive(m, C.byref(disp), C.byref(mid)) == 0
+    assert oracle.materials_assign_box(m, g, 0, n, 0, n, 0, n, mid.value) == 0
+    f = field_arrays(g)
+    f["ex"][:] = e0
+    hist = []
+    for _ in range(steps):
+        oracle.fdtd_update_h(g)
+        oracle.fdtd_update_e_materials(g, m)
+        hist.append(f["ex"][0])
+        assert np.all(f["hx"] == 0.0) and np.all(f["hy"] == 0.0) and np.all(f["hz"] == 0.0)
+    dt = g.contents.dt
+    oracle.materials_destroy(m)
+    oracle.fdtd_grid_destroy(g)
+    return np.array(hist), dt
+
+
+def _propagator(disp, dt):
+    """(E, J_p, P_p ...) state matrix of the continuous ODE, bilinear map."""
+    poles = []
+    for q in range(disp.n_drude):
+        poles.append((0.0, disp.drude_gamma[q], disp.drude_wp[q] ** 2))
+    for q in range(disp.n_lorentz):
+        w0 = disp.lorentz_w0[q]
+        poles.append((w0 * w0, 2 * disp.lorentz_delta[q], disp.lorentz_deps[q] * w0 * w0))
+    n = 1 + 2 * len(poles)
+    A = np.zeros((n, n))
+    ce = EPS0 * disp.eps_inf
+    A[0, 0] = -disp.sigma / ce
+    for p, (w0sq, gam, amp) in enumerate(poles):
+        jj, pp = 1 + 2 * p, 2 + 2 * p
+        A[0, jj] = -1.0 / ce
+        A[jj, 0] = EPS0 * amp
+        A[jj, jj] = -gam
+        A[jj, pp] = -w0sq
+        A[pp, jj] = 1.0
+    I = np.eye(n)
+    return np.linalg.solve(I - 0.5 * dt * A, I + 0.5 * dt * A)
+
+
+def _disp(eps_inf=1.0, sigma=0.0, drude=(), lorentz=()):
+    d = capi.MaterialDispersive()
+    d.eps_inf = eps_inf
+    d.sigma = sigma
+    d.n_drude = len(drude)
+    for q, (wp, gam) in enumerate(drude):
+        d.drude_wp[q] = wp
+        d.drude_gamma[q] = gam
+    d.n_lorentz = len(lorentz)
+    for q, (de, w0, dl) in enumerate(lorentz):
+        d.lorentz_deps[q] = de
+        d.lorentz_w0[q] = w0
+        d.lorentz_delta[q] = dl
+    return d
+
+
+CASES = {
+    "drude": _disp(1.0, 0.0, drude=[(2 * np.pi * 30e9, 2e10)]),
+    "drude_stiff": _disp(1.0, 0.0, drude=[(1.37e16, 1e14)]),
+    "lorentz2": _disp(2.25, 0.0, lorentz=[(1.0, 2 * np.pi * 60e9, 0.05 * 2 * np.pi * 60e9),
+                                          (0.5, 2 * np.pi * 90e9, 0.05 * 2 * np.pi * 90e9)]),
+    "mixed_lossy": _disp(3.0, 0.02, drude=[(2 * np.pi * 20e9, 5e9)],
+                         lorentz=[(2.0, 2 * np.pi * 45e9, 1e9)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_ade_trapezoidal_propagator(oracle, name):
+    disp = CASES[name]
+    steps = 400
+    hist, dt = _uniform_dispersive(oracle, disp, steps)
+    M = _propagator(disp, dt)
+    x = np.zeros(M.shape[0])
+    x[0] = 1.0
+    want = []
+    for _ in range(steps):
+        x = M @ x
+        want.append(x[0])
+    want = np.array(want)
+    assert np.max(np.abs(hist - want)) <= 1e-9 * max(1.0, np.max(np.abs(want)))
+    # unconditional stability: the propagator's spectral radius <= 1
+    assert np.max(np.abs(np.linalg.eigvals(M))) <= 1.0 + 1e-12
+
+
+def test_zero_pole_dispersive_is_plain_material(oracle):
+    """eps_inf and sigma alone give the plain lossy update (to rounding)."""
+    n = 6
+    out = []
+    for disp in (True, False):
+        g = oracle.fdtd_grid_create(n, n, n, 1e-3, 0.9)
+        oracle.fdtd_grid_set_periodic(g, 7)
+        rng = np.random.default_rng(3)
+        for a in field_arrays(g).values():
+            a[:] = rng.uniform(-1, 1, a.shape)
+        m = oracle.materials_create(g)
+        if disp:
+            mid = C.c_uint8()
+            assert oracle.materials_register_dispersive(m, C.byref(_disp(4.0, 0.03)), C.byref(mid)) == 0
+            assert oracle.materials_assign_box(m, g, 0, n, 0, n, 0, n, mid.value) == 0
+        else:
+            assert oracle.materials_set_box(m, g, 0, n, 0, n, 0, n, 4.0, 0.03) == 0
+        for _ in range(30):
+            oracle.fdtd_update_h(g)
+            oracle.fdtd_update_e_materials(g, m)
+        out.append({k: v.copy() for k, v in field_arrays(g).items()})
+        oracle.materials_destroy(m)
+        oracle.fdtd_grid_destroy(g)
+    for k in out[0]:
+        assert np.allclose(out[0][k], out[1][k], rtol=1e-12, atol=1e-12 * np.max(np.abs(out[1][k])))
+
+
+def _operator_matrix(oracle, n, tensors):
+    """Columns: H increment of one H update after one E update, for each
+    H basis vector, on a fully periodic n^3 grid (E and H zero otherwise)."""
+    g = oracle.fdtd_grid_create(n, n, n, 1e-3, 0.9)
+    oracle.fdtd_grid_set_periodic(g, 7)
+    m = oracle.materials_create(g)
+    for (c, i, j, k, t6) in tensors:
+        arr = (C.c_double * 6)(*t6)
+        assert oracle.materials_set_component_tensor3(m, g, c, i, j, k, arr) == 0
+    f = field_arrays(g)
+    names = ("hx", "hy", "hz")
+    dofs = [(q, i, j, k) for q in range(3) for i in range(n) for j in range(n) for k in range(n)]
+    gg = g.contents
+
+    def lin(i, j, k):
+        return (i * (gg.ny + 1) + j) * (gg.nz + 1) + k
+
+    A = np.zeros((len(dofs), len(dofs)))
+    for col, (q, i, j, k) in enumerate(dofs):
+        for a in f.values():
+            a[:] = 0.0
+        f[names[q]][lin(i, j, k)] = 1.0
+        oracle.fdtd_update_e_materials(g, m)
+        f[names[q]][lin(i, j, k)] = 0.0
+        oracle.fdtd_update_h(g)
+        A[:, col] = [f[names[qq]][lin(ii, jj, kk)] for (qq, ii, jj, kk) in dofs]
+    oracle.materials_destroy(m)
+    oracle.fdtd_grid_destroy(g)
+    return A, dofs
+
+
+def _random_tensors(n, seed, frac=0.6):
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(3):
+        for i in range(n):
+            for j in range(n):
+                for k in range(n):
+                    if rng.uniform() > frac:
+                        continue
+                    # random SPD inverse permittivity with eigenvalues in (0.1, 1)
+                    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+                    M = Q @ np.diag(rng.uniform(0.1, 1.0, 3)) @ Q.T
+                    out.append((c, i, j, k, [M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2], M[2, 2]]))
+    return out
+
+
+def test_tensor3_operator_symmetric(oracle):
+    n = 4
+    A, _ = _operator_matrix(oracle, n, _random_tensors(n, 11))
+    scale = np.max(np.abs(A))
+    assert scale > 0
+    assert np.max(np.abs(A - A.T)) <= 1e-11 * scale
+    # uniform H of each component is curl-free: zero row sums per component block
+    m = n ** 3
+    for q in range(3):
+        assert np.max(np.abs(A[:, q * m:(q + 1) * m].sum(axis=1))) <= 1e-11 * scale
+    # negative semi-definite (the H-H block of a lossless Yee operator)
+    w = np.linalg.eigvalsh(0.5 * (A + A.T))
+    assert w.max() <= 1e-9 * scale
+
+
+def test_diagonal_tensor3_is_scalar(oracle):
+    """A tensor (1/eps) I on every edge equals the scalar eps update."""
+    n, eps = 5, 3.5
+    res = []
+    for use_tensor in (True, False):
+        g = oracle.fdtd_grid_create(n, n, n, 1e-3, 0.9)
+        oracle.fdtd_grid_set_periodic(g, 7)
+        rng = np.random.default_rng(5)
+        for a in field_arrays(g).values():
+            a[:] = rng.uniform(-1, 1, a.shape)
+        m = oracle.materials_create(g)
+        if use_tensor:
+            t6 = (C.c_double * 6)(1 / eps, 0, 0, 1 / eps, 0, 1 / eps)
+            for c in range(3):
+                for i in range(n):
+                    for j in range(n):
+                        for k in range(n):
+                            assert oracle.materials_set_component_tensor3(m, g, c, i, j, k, t6) == 0
+        else:
+            assert oracle.materials_set_box(m, g, 0, n, 0, n, 0, n, eps, 0.0) == 0
+        for _ in range(20):
+            oracle.fdtd_update_h(g)
+            oracle.fdtd_update_e_materials(g, m)
+        res.append({k: v.copy() for k, v in field_arrays(g).items()})
+        oracle.materials_destroy(m)
+        oracle.fdtd_grid_destroy(g)
+    for k in res[0]:
+        assert np.allclose(res[0][k], res[1][k], rtol=1e-12, atol=1e-13 * np.max(np.abs(res[1][k])))
