@@ -15,7 +15,9 @@
  *     (s x s x s sub-samples) and n the shape's surface normal at the edge:
  *     the 3D form of subpixel_te_mix_epsilon_tensor (subpixel_te.c:106-132);
  *   - edges within `keepout` cells of a non-periodic boundary (the CPML and
- *     the tensor stencil margin) get the scalar 1/M_cc instead of a tensor.
+ *     the tensor stencil margin) get the scalar 1/M_cc instead of a tensor,
+ *     with f and n_c^2 quantised to 1/64 steps so that band holds few
+ *     distinct coefficients.
  * Materials are lossless here.  Coordinates are in cell units with node
  * (i,j,k) at (i,j,k).
  */

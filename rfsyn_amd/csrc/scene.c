@@ -251,9 +251,16 @@ int fdtd_scene_paint(materials_t *mat, const fdtd_grid_t *g, const fdtd_shape_t 
                         if (!((per >> a) & 1u) && (xe[a] < (double)keep || xe[a] > (double)n[a] - (double)keep))
                             near_face = 1;
                     if (near_face) {
-                        if (materials_set_component(mat, g, comp, i, j, k, 1.0 / M[c][c] < 1.0 ? 1.0 : 1.0 / M[c][c],
-                                                    0.0) != 0)
-                            goto out;
+                        /* scalar 1/M_cc from the fill fraction and normal
+                         * quantised to 1/64 steps, so the band holds a small
+                         * set of distinct coefficients (compact device tags) */
+                        double fq = floor(f * 64.0 + 0.5) / 64.0;
+                        double nc = floor(nv[c] * nv[c] * 64.0 + 0.5) / 64.0;
+                        double lnq = (1.0 - fq) / e_out + fq / e_in;
+                        double ltq = 1.0 / ((1.0 - fq) * e_out + fq * e_in);
+                        double mcc = (lnq - ltq) * nc + ltq;
+                        double eps_c = 1.0 / mcc < 1.0 ? 1.0 : 1.0 / mcc;
+                        if (materials_set_component(mat, g, comp, i, j, k, eps_c, 0.0) != 0) goto out;
                         st.scalar_mixed_edges++;
                         continue;
                     }

@@ -385,7 +385,8 @@ k_plane_capture(rf_geom_t gg, const T *f, int axis, int index, B *buf, int ring_
     buf[slot * (long long)na * nb + (long long)ia * nb + ib] = static_cast<B>(f[lin(g, x)]);
 }
 
-// Twiddles for the buffered block: tw[s][f] = (cos, sin)(w_f (n_s+1) dt).
+// Twiddles for the buffered block, scaled by dt:
+// tw[s][f] = dt (cos, sin)(w_f (n_s+1) dt).
 __global__ void k_twiddles(int nf, const double *freq_w, double dt, const long long *d_step,
                            int count, double2 *tw) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,50 +396,44 @@ __global__ void k_twiddles(int nf, const double *freq_w, double dt, const long l
     const double tt = static_cast<double>(n + 1) * dt;
     double sn, cs;
     sincos(freq_w[f] * tt, &sn, &cs);
-    tw[t] = make_double2(cs, sn);
+    tw[t] = make_double2(cs * dt, sn * dt);
 }
 
-// One thread per plane cell; for every frequency, sum the buffered steps
-// in time order into the double accumulators [f][cell].
-template <typename B>
+// Time-blocked plane DFT: one thread per plane cell keeps its KC buffered
+// samples in registers and, for every frequency, adds them to the double
+// accumulators [f][cell] in time order (one read and one write of each
+// accumulator per block of steps instead of per step).  Twiddles come from
+// LDS; the samples past `count` contribute nothing.
+template <typename B, int KC>
 __global__ void __launch_bounds__(256)
-k_plane_dft(const B *buf, long long cells, int ring_len, int nf, const double2 *tw, double dt,
-            const long long *d_step, int count, double *acc_re, double *acc_im, double *cmp_re,
-            double *cmp_im, int comp_sum) {
+k_plane_dft(const B *buf, long long cells, int ring_len, int nf, const double2 *tw,
+            const long long *d_step, int count, double *acc_re, double *acc_im) {
     extern __shared__ double2 s_tw[];
     for (int t = threadIdx.x; t < nf * count; t += blockDim.x) s_tw[t] = tw[t];
     __syncthreads();
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= cells) return;
     const long long end = *d_step;
-    double v[64];
-    const int cnt = count > 64 ? 64 : count;
-    for (int s = 0; s < cnt; s++) {
+    B v[KC];
+#pragma unroll
+    for (int s = 0; s < KC; s++) {
         const long long n = end - count + s;
-        v[s] = static_cast<double>(buf[(n % ring_len) * cells + cell]);
+        v[s] = s < count ? buf[(n % ring_len) * cells + cell] : B(0);
     }
     for (int f = 0; f < nf; f++) {
         const long long o = (long long)f * cells + cell;
         double re = acc_re[o], im = acc_im[o];
-        double cr = comp_sum ? cmp_re[o] : 0.0, ci = comp_sum ? cmp_im[o] : 0.0;
-        for (int s = 0; s < cnt; s++) {
-            const double2 w = s_tw[s * nf + f];
-            const double tr = v[s] * w.x * dt;
-            const double ti = v[s] * w.y * dt;
-            if (comp_sum) {
-                neumaier(re, cr, tr);
-                neumaier(im, ci, -ti);
-            } else {
-                re += tr;
-                im -= ti;
+#pragma unroll
+        for (int s = 0; s < KC; s++) {
+            if (s < count) {
+                const double2 w = s_tw[s * nf + f];
+                const double x = static_cast<double>(v[s]);
+                re += x * w.x;
+                im -= x * w.y;
             }
         }
         acc_re[o] = re;
         acc_im[o] = im;
-        if (comp_sum) {
-            cmp_re[o] = cr;
-            cmp_im[o] = ci;
-        }
     }
 }
 
@@ -686,16 +681,24 @@ int rf_launch_plane_dft(const void *buf, int buf_prec, long long cells, int ring
     double2 *tw = reinterpret_cast<double2 *>(tw_scratch);
     hipLaunchKernelGGL(k_twiddles, dim3((nf * count + 255) / 256), dim3(256), 0, hs(s), nf, freq_w,
                        dt, d_step, count, tw);
+    (void)cmp_re;
+    (void)cmp_im;
+    if (comp_sum) return -1;   /* plane monitors use plain double sums */
     const size_t lds = (size_t)nf * count * sizeof(double2);
     dim3 grd((unsigned)((cells + 255) / 256));
-    if (buf_prec == 0)
-        hipLaunchKernelGGL((k_plane_dft<double>), grd, dim3(256), lds, hs(s),
-                           static_cast<const double *>(buf), cells, ring_len, nf, tw, dt, d_step,
-                           count, acc_re, acc_im, cmp_re, cmp_im, comp_sum);
-    else
-        hipLaunchKernelGGL((k_plane_dft<float>), grd, dim3(256), lds, hs(s),
-                           static_cast<const float *>(buf), cells, ring_len, nf, tw, dt, d_step,
-                           count, acc_re, acc_im, cmp_re, cmp_im, comp_sum);
+#define RF_PLANE_DFT(BT, KC)                                                                        \
+    hipLaunchKernelGGL((k_plane_dft<BT, KC>), grd, dim3(256), lds, hs(s), static_cast<const BT *>(buf), \
+                       cells, ring_len, nf, tw, d_step, count, acc_re, acc_im)
+    if (buf_prec == 0) {
+        if (count <= 16) RF_PLANE_DFT(double, 16);
+        else if (count <= 32) RF_PLANE_DFT(double, 32);
+        else RF_PLANE_DFT(double, 64);
+    } else {
+        if (count <= 16) RF_PLANE_DFT(float, 16);
+        else if (count <= 32) RF_PLANE_DFT(float, 32);
+        else RF_PLANE_DFT(float, 64);
+    }
+#undef RF_PLANE_DFT
     return launched();
 }
 
