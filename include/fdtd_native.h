@@ -107,10 +107,15 @@ int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic);
 int fdtd_sim_set_profiling(fdtd_sim_t *s, int on);
 int fdtd_sim_kernel_times(fdtd_sim_t *s, double *h_ms, double *e_ms, long *steps);
 
-/* 1 when the run uses the fused leapfrog kernel (H, sources and E in one
- * pass over double-buffered fields; its time is reported as h_ms and e_ms
- * is 0), 0 for separate H and E kernels; decided at the first run. */
+/* 1 when the run uses the fused leapfrog step (H, sources and E in one
+ * pass over double-buffered fields on the CPML-free box, timed as h_ms,
+ * and node-wise shell kernels around it, timed as e_ms), 0 for separate
+ * H and E kernels; decided at the first run. */
 int fdtd_sim_fused(const fdtd_sim_t *s);
+/* The node box [lo, hi) per axis that the fused kernel owns (the CPML
+ * plain region with a one-node margin); the nodes around it are updated
+ * by the shell kernels.  Valid after the first run; 0 on success. */
+int fdtd_sim_fused_box(const fdtd_sim_t *s, int lo[3], int hi[3]);
 
 /* Bytes of device memory held by the handle. */
 size_t fdtd_sim_device_bytes(const fdtd_sim_t *s);

@@ -126,77 +126,119 @@ static void slab_bounds(const double *am, const double *ah, size_t n, int *lo, i
     *hi = (int)first_high;
 }
 
+/* Largest [plo, phi) inside the non-slab range [lo, hi) where kinv of both
+ * grids is exactly 1 in the device precision (empty if a kinv != 1 sits
+ * strictly inside it). */
+static int kinv_is_one(const double *k, size_t q, int prec) {
+    return prec ? (float)k[q] == 1.0f : k[q] == 1.0;
+}
+
+static void plain_range(const double *km, const double *kh, size_t n, int prec, int lo, int hi, int *plo,
+                        int *phi) {
+    int a = lo < 0 ? 0 : lo, b = hi > (int)n + 1 ? (int)n + 1 : hi;
+#define RF_PLAIN(q) (kinv_is_one(km, (size_t)(q), prec) && ((size_t)(q) >= n || kinv_is_one(kh, (size_t)(q), prec)))
+    while (a < b && !RF_PLAIN(a)) a++;
+    while (b > a && !RF_PLAIN(b - 1)) b--;
+    for (int q = a; q < b; q++)
+        if (!RF_PLAIN(q)) { a = b = 0; break; }
+#undef RF_PLAIN
+    *plo = a;
+    *phi = b;
+}
+
 static int push_owned(rf_cpml_dev_t *c, void *p) {
     if (c->n_owned >= 64) return -1;
     c->owned[c->n_owned++] = p;
     return 0;
 }
 
-static int upload_profile(rf_cpml_dev_t *c, const double *src, size_t n, int prec, const void **out) {
-    void *d = NULL;
-    size_t es = prec ? sizeof(float) : sizeof(double);
-    if (rf_malloc(&d, (n ? n : 1) * es) != 0 || push_owned(c, d) != 0) return -1;
-    c->bytes += n * es;
-    if (prec == 0) {
-        if (rf_h2d(d, src, n * sizeof(double), NULL) != 0 || rf_sync(NULL) != 0) return -1;
-    } else {
-        float *tmp = malloc((n ? n : 1) * sizeof(float));
-        if (!tmp) return -1;
-        for (size_t q = 0; q < n; q++) tmp[q] = (float)src[q];
-        int rc = rf_h2d(d, tmp, n * sizeof(float), NULL);
-        rf_sync(NULL);
-        free(tmp);
-        if (rc != 0) return -1;
-    }
-    *out = d;
-    return 0;
+/* Offset of psi memory (comp, s) inside one psi section, in elements;
+ * sec[a] is the size of one slab array for derivative axis a.  Order
+ * (0,0) (0,1) (1,0) (1,1) (2,0) (2,1), derivative axes 1 2 2 0 0 1. */
+static long long psi_offset(const long long sec[3], int comp, int s) {
+    const long long o[6] = {0, sec[1], sec[1] + sec[2], sec[1] + 2 * sec[2], sec[0] + sec[1] + 2 * sec[2],
+                            2 * sec[0] + sec[1] + 2 * sec[2]};
+    return o[2 * comp + s];
 }
 
-int rf_cpml_build(const cpml_t *p, int prec, int with_h2, rf_cpml_dev_t *c) {
+int rf_cpml_build(const cpml_t *p, int prec, rf_cpml_dev_t *c) {
     memset(c, 0, sizeof *c);
     size_t n[3] = {p->nx, p->ny, p->nz};
     cpml_t *pp = (cpml_t *)p;
+    size_t es = prec ? sizeof(float) : sizeof(double);
+
+    /* one coefficient block: per axis the six profiles kinv_m b_m a_m
+     * kinv_h b_h a_h, n+1 values each (the half-node ones padded) */
+    size_t tot = 0, coff[3];
     for (int ax = 0; ax < 3; ax++) {
-        const double *km = *profile_slot(pp, 0, 0, ax), *bm = *profile_slot(pp, 0, 1, ax),
-                     *am = *profile_slot(pp, 0, 2, ax);
-        const double *kh = *profile_slot(pp, 1, 0, ax), *bh = *profile_slot(pp, 1, 1, ax),
-                     *ah = *profile_slot(pp, 1, 2, ax);
-        if (upload_profile(c, km, n[ax] + 1, prec, &c->d.kinv_m[ax]) ||
-            upload_profile(c, bm, n[ax] + 1, prec, &c->d.b_m[ax]) ||
-            upload_profile(c, am, n[ax] + 1, prec, &c->d.a_m[ax]) ||
-            upload_profile(c, kh, n[ax], prec, &c->d.kinv_h[ax]) ||
-            upload_profile(c, bh, n[ax], prec, &c->d.b_h[ax]) ||
-            upload_profile(c, ah, n[ax], prec, &c->d.a_h[ax])) {
-            rf_cpml_release(c);
-            return -1;
+        coff[ax] = tot;
+        tot += 6 * (n[ax] + 1);
+    }
+    double *h64 = calloc(tot, sizeof *h64);
+    if (!h64) return -1;
+    for (int ax = 0; ax < 3; ax++)
+        for (int prof = 0; prof < 6; prof++) {
+            const double *src = *profile_slot(pp, prof / 3, prof % 3, ax);
+            size_t cnt = prof < 3 ? n[ax] + 1 : n[ax];
+            memcpy(h64 + coff[ax] + prof * (n[ax] + 1), src, cnt * sizeof(double));
         }
+    void *coef = NULL;
+    int rc = rf_malloc(&coef, tot * es) || push_owned(c, coef);
+    if (!rc) {
+        if (prec == 0) {
+            rc = rf_h2d(coef, h64, tot * sizeof(double), NULL) || rf_sync(NULL);
+        } else {
+            float *f32 = malloc(tot * sizeof *f32);
+            if (!f32) rc = -1;
+            else {
+                for (size_t q = 0; q < tot; q++) f32[q] = (float)h64[q];
+                rc = rf_h2d(coef, f32, tot * sizeof(float), NULL) || rf_sync(NULL);
+                free(f32);
+            }
+        }
+    }
+    if (rc) {
+        free(h64);
+        rf_cpml_release(c);
+        return -1;
+    }
+    c->bytes += tot * es;
+    char *cb = coef;
+    for (int ax = 0; ax < 3; ax++) {
+        size_t st = (n[ax] + 1) * es;
+        c->d.kinv_m[ax] = cb + coff[ax] * es;
+        c->d.b_m[ax] = cb + coff[ax] * es + st;
+        c->d.a_m[ax] = cb + coff[ax] * es + 2 * st;
+        c->d.kinv_h[ax] = cb + coff[ax] * es + 3 * st;
+        c->d.b_h[ax] = cb + coff[ax] * es + 4 * st;
+        c->d.a_h[ax] = cb + coff[ax] * es + 5 * st;
+        const double *km = h64 + coff[ax], *am = h64 + coff[ax] + 2 * (n[ax] + 1);
+        const double *kh = h64 + coff[ax] + 3 * (n[ax] + 1), *ah = h64 + coff[ax] + 5 * (n[ax] + 1);
         slab_bounds(am, ah, n[ax], &c->d.lo[ax], &c->d.hi[ax]);
         c->d.len[ax] = c->d.lo[ax] + ((int)n[ax] + 1 - c->d.hi[ax]);
+        plain_range(km, kh, n[ax], prec, c->d.lo[ax], c->d.hi[ax], &c->d.plo[ax], &c->d.phi[ax]);
     }
-    size_t es = prec ? sizeof(float) : sizeof(double);
+    free(h64);
+
+    /* psi: one zero-filled section each for E and H */
+    long long sec[3];
+    for (int ax = 0; ax < 3; ax++) {
+        long long ext[3] = {(long long)n[0] + 1, (long long)n[1] + 1, (long long)n[2] + 1};
+        ext[ax] = c->d.len[ax];
+        sec[ax] = ext[0] * ext[1] * ext[2];
+    }
+    size_t psec = (size_t)(2 * (sec[0] + sec[1] + sec[2]));
+    void *pe = NULL, *ph = NULL;
+    if (rf_malloc(&pe, psec * es) || push_owned(c, pe) || rf_malloc(&ph, psec * es) || push_owned(c, ph)) {
+        rf_cpml_release(c);
+        return -1;
+    }
+    c->bytes += 2 * psec * es;
     for (int comp = 0; comp < 3; comp++)
-        for (int s = 0; s < 2; s++) {
-            int ax = (comp + 1 + s) % 3;
-            size_t ext[3] = {n[0] + 1, n[1] + 1, n[2] + 1};
-            ext[ax] = (size_t)c->d.len[ax];
-            size_t cnt = ext[0] * ext[1] * ext[2];
-            if (!cnt) continue;
-            void *h = NULL, *h2 = NULL, *e = NULL;
-            if (rf_malloc(&h, cnt * es) || push_owned(c, h) || rf_malloc(&e, cnt * es) || push_owned(c, e)) {
-                rf_cpml_release(c);
-                return -1;
-            }
-            c->bytes += 2 * cnt * es;
-            if (with_h2) {   /* second H-psi set for the double-buffered fused step */
-                if (rf_malloc(&h2, cnt * es) || push_owned(c, h2)) {
-                    rf_cpml_release(c);
-                    return -1;
-                }
-                c->bytes += cnt * es;
-            }
-            c->d.psi_h[comp][s] = h;
-            c->psi_h2[comp][s] = h2;
-            c->d.psi_e[comp][s] = e;
+        for (int s2 = 0; s2 < 2; s2++) {
+            size_t off = (size_t)psi_offset(sec, comp, s2) * es;
+            c->d.psi_e[comp][s2] = (char *)pe + off;
+            c->d.psi_h[comp][s2] = (char *)ph + off;
         }
     return 0;
 }
@@ -210,7 +252,7 @@ rf_cpml_dev_t *rf_cpml_dev(cpml_t *p) {
     if (p->dev) return (rf_cpml_dev_t *)p->dev;
     rf_cpml_dev_t *c = calloc(1, sizeof *c);
     if (!c) return NULL;
-    if (rf_cpml_build(p, 0, 0, c) != 0) { free(c); return NULL; }
+    if (rf_cpml_build(p, 0, c) != 0) { free(c); return NULL; }
     p->dev = (struct cpml_device *)c;
     return c;
 }

@@ -56,13 +56,12 @@ int rf_grid_download(fdtd_grid_t *g, unsigned mask);
  * requested precision. */
 typedef struct {
     rf_cpml_t d;
-    void *psi_h2[3][2];       /* second H-psi set (with_h2), else NULL */
     void *owned[64];
     int n_owned;
     size_t bytes;
 } rf_cpml_dev_t;
 
-int rf_cpml_build(const cpml_t *p, int prec, int with_h2, rf_cpml_dev_t *out);
+int rf_cpml_build(const cpml_t *p, int prec, rf_cpml_dev_t *out);
 void rf_cpml_release(rf_cpml_dev_t *c);
 /* Compat mirror kept inside the cpml_t (FP64). */
 rf_cpml_dev_t *rf_cpml_dev(cpml_t *p);

@@ -1,24 +1,36 @@
 // Fused leapfrog Yee step for gfx950: H^{n+1/2} then E^{n+1} in one pass.
 //
-// A block owns a tile of TJ rows (y) x 64 lanes (z, the contiguous axis) of
-// node columns and marches along x.  At each plane i it
-//   1. stages E^n(i) for the tile plus a one-cell frame in LDS (the tile's
-//      own values come from the E^n(i+1) registers of the previous plane,
-//      so each E value crosses HBM once),
-//   2. computes H^{n+1/2}(i) on the tile and on the one-cell halo row
-//      (j0-1) and column (k0-1) that the E update needs, keeping the result
-//      in LDS and writing only the tile's own nodes to the B buffers,
-//   3. computes E^{n+1}(i) on the tile from the LDS H plane and the H of
-//      plane i-1 held in registers, and writes it to B.
-// Reading state n from the A buffers and writing state n+1 to B makes the
-// halo recomputation race-free; per node per step the kernel moves 6 reads
-// and 6 writes instead of the 9 + 9 of separate H and E sweeps.
+// One wavefront owns a tile of R rows (y) by 62 columns (z, the contiguous
+// axis) of nodes and marches along x through a chunk of planes.  Lane l
+// holds column k0-1+l: lanes 1..62 own their column, lanes 0 and 63 are
+// halo columns whose values reach the k-1 / k+1 neighbours through DPP
+// wavefront shifts.  Registers hold E^n on rows j0-1 .. j0+R of the current
+// and the next plane and the (hy, hz) of the previous plane, so a field value
+// crosses HBM once per step in each direction: 6 reads and 6 writes per node
+// instead of the 9 + 9 of separate H and E sweeps.  H^{n+1/2} is computed on
+// rows j0-1 .. j0+R-1 (row j0-1 and lane 0 are the halo that the E update of
+// the tile reads) and E^{n+1} on rows j0 .. j0+R-1.
 //
-// The arithmetic of every update is the same expression, in the same order,
-// as the two-kernel path (yee_kernels.hip), which follows
-// engine/fdtd/grid.c:70-89, cpml.h:49-50, cpml.c:185-260 and
-// materials.c:975-1055; compiled with -ffp-contract=off the FP64 variant
-// reproduces the reference bits.
+// State n is read from the A buffers and state n+1 written to the B buffers,
+// so recomputed halos never see a neighbour's new values and no wavefront
+// waits for another: the only barrier is the one after the block stages the
+// material table in LDS.  psi_h is double-buffered for the same reason;
+// psi_e and the ADE state are touched only by the owner of the node and are
+// updated in place.  Blocks are numbered XCD-major so that the tiles one XCD
+// runs at a time are neighbours and share halo rows in its L2.
+//
+// The kernel owns the box of nodes that lies inside the CPML plain range on
+// every axis (kinv = 1 and psi stays 0 there, so the stretched difference
+// equals the plain one up to the sign of a zero) with a one-node margin, so
+// it evaluates no CPML term; the shell kernels at the end of this file
+// update the nodes around that box.  The common path is branch-free:
+// sources are applied only on planes whose tile their bounding box
+// touches, dispersive and tensor edges only where the material flags say so.
+//
+// Every update is the same expression, in the same order, as the two-kernel
+// path (yee_kernels.hip), which follows engine/fdtd/grid.c:70-89,
+// cpml.h:49-50, cpml.c:185-260 and materials.c:975-1055; compiled with
+// -ffp-contract=off the FP64 variant reproduces the reference values.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -32,79 +44,59 @@ namespace {
 
 using namespace rfk;
 
-constexpr int TK = 64;   // lanes along z
+constexpr int WL = 64;       // lanes per wavefront
+constexpr int KW = WL - 2;   // columns owned per wavefront
+constexpr int WPB = 4;       // wavefronts per block
+constexpr int N_XCD = 8;     // blocks are dealt round-robin to the XCDs
 
-// Wrap a coordinate one step outside [0, n) on a periodic axis; -1 when the
-// coordinate is outside the stored box of a non-periodic axis ([0, n]).
-__device__ __forceinline__ int wrap_or_invalid(int q, int n, bool periodic) {
-    if (periodic) {
-        if (q < 0) return q + n;
-        if (q >= n) return q - n;
-        return q;
-    }
-    return (q < 0 || q > n) ? -1 : q;
+// DPP wavefront shifts: lane l receives lane l+1 (from_upper) or lane l-1
+// (from_lower); the edge lane receives 0 and is a halo lane whose result is
+// never stored.
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+
+__device__ __forceinline__ float from_upper(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+__device__ __forceinline__ float from_lower(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+__device__ __forceinline__ double from_upper(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = static_cast<unsigned>(dpp_shl1(static_cast<int>(b)));
+    const unsigned hi = static_cast<unsigned>(dpp_shl1(static_cast<int>(b >> 32)));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ double from_lower(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = static_cast<unsigned>(dpp_shr1(static_cast<int>(b)));
+    const unsigned hi = static_cast<unsigned>(dpp_shr1(static_cast<int>(b >> 32)));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
-// cpml_term with separate psi read/write slabs (write == nullptr: halo
-// recomputation, nothing stored).
-template <typename T>
-__device__ __forceinline__ T stretch_rw(T diff, const rf_cpml_t &c, const Geo &g, bool h, int axis, int q,
-                                        const T *psi_rd, T *psi_wr, int i, int j, int k, T dx) {
-    const T *kinv = static_cast<const T *>(h ? c.kinv_h[axis] : c.kinv_m[axis]);
-    const T *bb = static_cast<const T *>(h ? c.b_h[axis] : c.b_m[axis]);
-    const T *aa = static_cast<const T *>(h ? c.a_h[axis] : c.a_m[axis]);
-    const int cq = slab_coord(c, axis, q);
-    T psi = T(0);
-    if (cq >= 0) {
-        const long long s = slab_index(c, g, axis, cq, i, j, k);
-        psi = bb[q] * psi_rd[s] + aa[q] * diff;
-        if (psi_wr) psi_wr[s] = psi;
-    }
-    return diff * kinv[q] + dx * psi;
+// The cold data of the step, read through a pointer the compiler cannot
+// see through, so its loads stay inside the slab / flagged-edge branches
+// that use them instead of being hoisted into registers for the whole loop.
+__device__ __forceinline__ const rf_leap_cold_t *cold_view(const rf_leap_cold_t *p) {
+    asm volatile("" : "+s"(p));
+    return p;
 }
 
-// Applies the pre-E point and plane sources of this (node, component), in
-// insertion order, to the old value (source_apply before the E update).
+// Sources of the device table that hit node (i,j,k), component c, phase
+// `after`, in table order (point sources, then plane sources, as the
+// separate source kernels apply them).
 template <typename T>
-__device__ __forceinline__ T pre_sources(const rf_leap_t &a, T v, long long idx, int c, int i, int j, int k,
-                                         const Geo &g, long long step) {
-    for (int s = 0; s < a.n_pts; s++) {
-        const rf_point_src_t &p = a.pts[s];
-        if (p.idx == idx && p.comp == c) {
-            const double w = p.wave[step];
-            v = p.hard ? static_cast<T>(w) : static_cast<T>(static_cast<double>(v) + w);
+__device__ __forceinline__ T apply_sources(const rf_leap_src_t *tab, int n, T v, int c, int i, int j, int k,
+                                        int gnx, int gny, int gnz, long long step, int after) {
+    for (int s = 0; s < n; s++) {
+        const rf_leap_src_t &p = tab[s];
+        if (p.after != after || p.comp != c) continue;
+        bool hit;
+        if (p.plane) {
+            const int ax = p.axis;
+            const int x = ax == 0 ? i : (ax == 1 ? j : k);
+            const bool in_x = ax == 0 || i < gnx, in_y = ax == 1 || j < gny, in_z = ax == 2 || k < gnz;
+            hit = x == p.index && in_x && in_y && in_z;
+        } else {
+            hit = p.i == i && p.j == j && p.k == k;
         }
-    }
-    for (int s = 0; s < a.n_planes; s++) {
-        if (a.plane_after[s]) continue;
-        const rf_plane_src_t &p = a.planes[s];
-        if (p.comp != c) continue;
-        const int x[3] = {i, j, k};
-        const int n[3] = {g.nx, g.ny, g.nz};
-        const int ax = p.axis, b0 = (ax + 1) % 3, b1 = (ax + 2) % 3;
-        int pi = p.index;
-        if ((a.g.per >> ax) & 1) pi %= n[ax];
-        if (x[ax] == pi && x[b0] < n[b0] && x[b1] < n[b1]) {
-            const double w = p.wave[step];
-            v = p.hard ? static_cast<T>(w) : static_cast<T>(static_cast<double>(v) + w);
-        }
-    }
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T post_sources(const rf_leap_t &a, T v, int c, int i, int j, int k, const Geo &g,
-                                          long long step) {
-    for (int s = 0; s < a.n_planes; s++) {
-        if (!a.plane_after[s]) continue;
-        const rf_plane_src_t &p = a.planes[s];
-        if (p.comp != c) continue;
-        const int x[3] = {i, j, k};
-        const int n[3] = {g.nx, g.ny, g.nz};
-        const int ax = p.axis, b0 = (ax + 1) % 3, b1 = (ax + 2) % 3;
-        int pi = p.index;
-        if ((a.g.per >> ax) & 1) pi %= n[ax];
-        if (x[ax] == pi && x[b0] < n[b0] && x[b1] < n[b1]) {
+        if (hit) {
             const double w = p.wave[step];
             v = p.hard ? static_cast<T>(w) : static_cast<T>(static_cast<double>(v) + w);
         }
@@ -112,11 +104,397 @@ __device__ __forceinline__ T post_sources(const rf_leap_t &a, T v, int c, int i,
     return v;
 }
 
-// Material update of one E component (same arithmetic as e_component of the
-// two-kernel path); returns the new value, or the old one for tensor edges.
+// Coefficients of one material entry: Ca, Cb per component and the flags
+// of the three components packed one byte each.  Entries below `ncache`
+// come from the block's LDS copy of the table, the rest from HBM.
+template <typename T>
+struct Coefs {
+    T ca[3], cb[3];
+    uint32_t fl;
+};
+
+template <typename T>
+__device__ __forceinline__ Coefs<T> entry_coefs(const void *entries, const T *s_ca, const T *s_cb,
+                                                const uint32_t *s_fl, int ncache, unsigned tag) {
+    Coefs<T> o;
+    if (static_cast<int>(tag) < ncache) {
+        o.fl = s_fl[tag];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            o.ca[c] = s_ca[3 * tag + c];
+            o.cb[c] = s_cb[3 * tag + c];
+        }
+    } else {
+        const typename EntryOf<T>::type &en = static_cast<const typename EntryOf<T>::type *>(entries)[tag];
+        o.fl = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            o.ca[c] = en.Ca[c];
+            o.cb[c] = en.Cb[c];
+        }
+    }
+    return o;
+}
+
+// Flagged edge (tensor: keep the old value for the tensor kernel;
+// dispersive: trapezoidal ADE), same arithmetic as e_component of the
+// two-kernel path.
+template <typename T>
+__device__ __forceinline__ T e_flagged(int c, uint32_t fl, T ca, T cb, T e_old, T dA, T dB,
+                                       const rf_leap_cold_t *cold, unsigned tag, int i, int j, int k) {
+    using Entry = typename EntryOf<T>::type;
+    using Ade = typename EntryOf<T>::ade;
+    if (fl & RF_EDGE_TENSOR) return e_old;
+    if (fl & RF_EDGE_ADE) {
+        const rf_leap_cold_t &m = *cold_view(cold);
+        const Entry &en = static_cast<const Entry *>(m.entries)[tag];
+        const Ade &ad = static_cast<const Ade *>(m.ade)[en.ade[c]];
+        const long long cells = (long long)m.bn[0] * m.bn[1] * m.bn[2];
+        const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
+        return ade_edge<T>(ad, ca, cb, e_old, dA, dB, static_cast<T *>(m.J), static_cast<T *>(m.P), 3 * cells,
+                           (long long)c * cells + cell);
+    }
+    return ca * e_old + cb * (dA - dB);
+}
+
+// LDS bytes per cached material entry.
+template <typename T>
+constexpr int coef_bytes() { return 6 * static_cast<int>(sizeof(T)) + 4; }
+
+template <typename T, typename TagT, int R>
+__global__ void __launch_bounds__(WL *WPB)
+k_leapfrog(rf_leap_t a) {
+    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+    // the first a.ncache entries of the material table, staged in LDS
+    // before any wavefront of the block may leave
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    const int ncache = a.ncache;
+    T *s_ca = reinterpret_cast<T *>(s_raw);
+    T *s_cb = s_ca + 3 * ncache;
+    uint32_t *s_fl = reinterpret_cast<uint32_t *>(s_cb + 3 * ncache);
+    if constexpr (TAGS) {
+        using Entry = typename EntryOf<T>::type;
+        const Entry *ents = static_cast<const Entry *>(a.entries);
+        for (int e = threadIdx.x; e < ncache; e += blockDim.x) {
+            const Entry &en = ents[e];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                s_ca[3 * e + c] = en.Ca[c];
+                s_cb[3 * e + c] = en.Cb[c];
+            }
+            s_fl[e] = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+        }
+        __syncthreads();
+    }
+
+    const int lane = threadIdx.x & (WL - 1);
+    const int wave_in_block = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = (blockIdx.x % N_XCD) * a.xcd_q + blockIdx.x / N_XCD;
+    const int wid = lb * WPB + wave_in_block;
+    if (wid >= a.n_waves) return;
+    const int tk = wid % a.ntk;
+    const int tj = (wid / a.ntk) % a.ntj;
+    const int tc = wid / (a.ntk * a.ntj);
+
+    const Geo g = geo_of(a.g);
+    // the box of nodes this launch owns: [ib0, ib1) per axis
+    const int k0 = a.ib0[2] + tk * KW, j0 = a.ib0[1] + tj * R;
+    const int jend = a.ib1[1], kend = a.ib1[2];
+    const int i_begin = a.ib0[0] + tc * a.chunk;
+    const int i_end = min(i_begin + a.chunk, a.ib1[0]);
+    const int sy = static_cast<int>(g.sy);
+
+    // this lane's column: true coordinate, stored coordinate (wrapped on a
+    // periodic axis, clamped into the stored box otherwise), ownership
+    const int kt = k0 - 1 + lane;
+    const int kc = g.pz ? (kt + g.nz) % g.nz : min(max(kt, 0), g.nz);
+    const bool col_own = lane >= 1 && lane <= KW && kt < kend;
+    // rows j0-1 .. j0+R, stored coordinates
+    int jr[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) {
+        const int jt = j0 - 1 + r;
+        jr[r] = g.py ? (jt + g.ny) % g.ny : min(max(jt, 0), g.ny);
+    }
+    // sources: does their bounding box meet this tile's rows and columns
+    const bool src_tile = a.n_srcs > 0 && a.src_box[2] < j0 + R && a.src_box[3] >= j0 &&
+                          a.src_box[4] < k0 + KW && a.src_box[5] >= k0;
+
+    const T *EA0 = static_cast<const T *>(a.ea[0]), *EA1 = static_cast<const T *>(a.ea[1]),
+            *EA2 = static_cast<const T *>(a.ea[2]);
+    const T *HA0 = static_cast<const T *>(a.ha[0]), *HA1 = static_cast<const T *>(a.ha[1]),
+            *HA2 = static_cast<const T *>(a.ha[2]);
+    T *EB0 = static_cast<T *>(a.eb[0]), *EB1 = static_cast<T *>(a.eb[1]), *EB2 = static_cast<T *>(a.eb[2]);
+    T *HB0 = static_cast<T *>(a.hb[0]), *HB1 = static_cast<T *>(a.hb[1]), *HB2 = static_cast<T *>(a.hb[2]);
+    const T cb = static_cast<T>(a.cb);
+    const T vac = static_cast<T>(a.vac_ca);
+
+    // E^{n+1}(i_begin) needs H^{n+1/2}(i_begin - 1): a warm-up plane
+    // computes it without storing (plane nx-1 when x is periodic).
+    const bool warm = i_begin > 0 || g.px;
+    int i = warm ? (i_begin == 0 ? g.nx - 1 : i_begin - 1) : i_begin;
+    const int iters = (i_end - i_begin) + (warm ? 1 : 0);
+    const long long step = *a.d_step;
+
+    T cx[R + 2], cy[R + 2], cz[R + 2];   // E^n, plane i
+    T nx[R + 2], ny[R + 2], nz[R + 2];   // E^n, plane i+1
+    T py[R + 1], pz[R + 1];              // H^{n+1/2} hy, hz of plane i-1
+    {
+        const long long base = (long long)i * g.sx;
+#pragma unroll
+        for (int r = 0; r < R + 2; r++) {
+            const int o = jr[r] * sy + kc;
+            cx[r] = EA0[base + o];
+            cy[r] = r <= R ? EA1[base + o] : T(0);
+            cz[r] = EA2[base + o];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r <= R; r++) py[r] = pz[r] = T(0);
+
+    for (int it = 0; it < iters; it++) {
+        const bool first = warm && it == 0;
+        int ip = i + 1;
+        if (g.px) {
+            if (ip == g.nx) ip = 0;
+        } else if (ip > g.nx) {
+            ip = g.nx;   // plane nx updates only hx, which needs no i+1
+        }
+        const long long base = (long long)i * g.sx;
+        const long long nbase = (long long)ip * g.sx;
+        // row j0+R feeds only hx, hz of row j0+R-1 (ez, ex at j+1)
+#pragma unroll
+        for (int r = 0; r < R + 2; r++) {
+            const int o = jr[r] * sy + kc;
+            nx[r] = EA0[nbase + o];
+            ny[r] = r <= R ? EA1[nbase + o] : T(0);
+            nz[r] = EA2[nbase + o];
+        }
+        T hx[R + 1], hy[R + 1], hz[R + 1];
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            const int o = jr[r] * sy + kc;
+            hx[r] = HA0[base + o];
+            hy[r] = r > 0 ? HA1[base + o] : T(0);   // the halo row needs hx, hz only
+            hz[r] = HA2[base + o];
+        }
+        // material tags of the owned rows, issued with the field loads
+        unsigned tg[R + 1];
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            tg[r] = 0;
+            if constexpr (TAGS)
+                if (r >= 1) tg[r] = load_tag<TagT>(a.tag, base + jr[r] * sy + kc);
+        }
+
+        // ---- H^{n+1/2}(i), rows j0-1 .. j0+R-1 (grid.c:70-89, cpml.c:185-215)
+        const bool hx_i = i < g.nx + !g.px, hyz_i = i < g.nx;
+#pragma unroll
+        for (int r = 0; r <= R; r++) {
+            const int jc = jr[r];
+            const T eyk = from_upper(cy[r]);   // ey(i, j, k+1)
+            const T exk = from_upper(cx[r]);   // ex(i, j, k+1)
+            const bool own = !first && r >= 1 && j0 - 1 + r < jend && col_own;
+            const bool do_x = hx_i && jc < g.ny && kc < g.nz;
+            const bool do_y = r >= 1 && hyz_i && jc < g.ny + !g.py && kc < g.nz;
+            const bool do_z = hyz_i && jc < g.ny && kc < g.nz + !g.pz;
+            T dAx = cz[r + 1] - cz[r], dBx = eyk - cy[r];
+            T dAy = exk - cx[r], dBy = nz[r] - cz[r];
+            T dAz = ny[r] - cy[r], dBz = cx[r + 1] - cx[r];
+            const T nhx = hx[r] - cb * (dAx - dBx);
+            const T nhy = hy[r] - cb * (dAy - dBy);
+            const T nhz = hz[r] - cb * (dAz - dBz);
+            hx[r] = do_x ? nhx : hx[r];
+            hy[r] = do_y ? nhy : hy[r];
+            hz[r] = do_z ? nhz : hz[r];
+            if (own) {
+                const int o = jc * sy + kc;
+                HB0[base + o] = hx[r];
+                HB1[base + o] = hy[r];
+                HB2[base + o] = hz[r];
+            }
+        }
+
+        // ---- sources and E^{n+1}(i), rows j0 .. j0+R-1 (materials.c:975-1058)
+        if (!first) {
+            const bool i_ok = g.px || i >= 1;
+            const bool src = src_tile && a.src_box[0] <= i && a.src_box[1] >= i;
+#pragma unroll
+            for (int r = 1; r <= R; r++) {
+                const T hxk = from_lower(hx[r]);   // hx(i, j, k-1)
+                const T hyk = from_lower(hy[r]);   // hy(i, j, k-1)
+                const int jc = jr[r];
+                if (j0 - 1 + r < jend && col_own) {
+                    const long long idx = base + jc * sy + kc;
+                    const bool in = i < g.nx && jc < g.ny && kc < g.nz;
+                    const bool j_ok = g.py || jc >= 1, k_ok = g.pz || kc >= 1;
+                    const bool do_x = in && j_ok && k_ok, do_y = in && i_ok && k_ok, do_z = in && i_ok && j_ok;
+                    T e0 = cx[r], e1 = cy[r], e2 = cz[r];
+                    if (src) {
+                        e0 = apply_sources<T>(a.srcs, a.n_srcs, e0, 0, i, jc, kc, g.nx, g.ny, g.nz, step, 0);
+                        e1 = apply_sources<T>(a.srcs, a.n_srcs, e1, 1, i, jc, kc, g.nx, g.ny, g.nz, step, 0);
+                        e2 = apply_sources<T>(a.srcs, a.n_srcs, e2, 2, i, jc, kc, g.nx, g.ny, g.nz, step, 0);
+                    }
+                    T dAx = hz[r] - hz[r - 1], dBx = hy[r] - hyk;
+                    T dAy = hx[r] - hxk, dBy = hz[r] - pz[r];
+                    T dAz = hy[r] - py[r], dBz = hx[r] - hx[r - 1];
+                    if constexpr (!TAGS) {
+                        const T n0 = T(1) * e0 + vac * (dAx - dBx);
+                        const T n1 = T(1) * e1 + vac * (dAy - dBy);
+                        const T n2 = T(1) * e2 + vac * (dAz - dBz);
+                        e0 = do_x ? n0 : e0;
+                        e1 = do_y ? n1 : e1;
+                        e2 = do_z ? n2 : e2;
+                    } else {
+                        const unsigned tag = tg[r];
+                        const Coefs<T> kf = entry_coefs<T>(a.entries, s_ca, s_cb, s_fl, ncache, tag);
+                        if (kf.fl == 0) {
+                            const T n0 = kf.ca[0] * e0 + kf.cb[0] * (dAx - dBx);
+                            const T n1 = kf.ca[1] * e1 + kf.cb[1] * (dAy - dBy);
+                            const T n2 = kf.ca[2] * e2 + kf.cb[2] * (dAz - dBz);
+                            e0 = do_x ? n0 : e0;
+                            e1 = do_y ? n1 : e1;
+                            e2 = do_z ? n2 : e2;
+                        } else {
+                            if (do_x) e0 = e_flagged<T>(0, kf.fl & 0xffu, kf.ca[0], kf.cb[0], e0, dAx, dBx, a.cold, tag, i, jc, kc);
+                            if (do_y) e1 = e_flagged<T>(1, (kf.fl >> 8) & 0xffu, kf.ca[1], kf.cb[1], e1, dAy, dBy, a.cold, tag, i, jc, kc);
+                            if (do_z) e2 = e_flagged<T>(2, (kf.fl >> 16) & 0xffu, kf.ca[2], kf.cb[2], e2, dAz, dBz, a.cold, tag, i, jc, kc);
+                        }
+                    }
+                    if (src) {
+                        e0 = apply_sources<T>(a.srcs, a.n_srcs, e0, 0, i, jc, kc, g.nx, g.ny, g.nz, step, 1);
+                        e1 = apply_sources<T>(a.srcs, a.n_srcs, e1, 1, i, jc, kc, g.nx, g.ny, g.nz, step, 1);
+                        e2 = apply_sources<T>(a.srcs, a.n_srcs, e2, 2, i, jc, kc, g.nx, g.ny, g.nz, step, 1);
+                    }
+                    EB0[idx] = e0;
+                    EB1[idx] = e1;
+                    EB2[idx] = e2;
+                }
+            }
+        }
+
+#pragma unroll
+        for (int r = 1; r <= R; r++) {
+            py[r] = hy[r];
+            pz[r] = hz[r];
+        }
+#pragma unroll
+        for (int r = 0; r < R + 2; r++) {
+            cx[r] = nx[r];
+            cy[r] = ny[r];
+            cz[r] = nz[r];
+        }
+        i = first ? i_begin : i + 1;
+    }
+}
+
+template <typename T, typename TagT, int R>
+int launch_tag(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
+    const size_t lds = std::is_same<TagT, NoTag>::value ? 0 : (size_t)a->ncache * coef_bytes<T>();
+    hipLaunchKernelGGL((k_leapfrog<T, TagT, R>), dim3(blocks), dim3(WL * WPB), lds, st, *a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T, int R>
+int launch_rows(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
+    if (!a->tag) return launch_tag<T, NoTag, R>(a, blocks, st);
+    switch (a->tag_bytes) {
+    case 1: return launch_tag<T, uint8_t, R>(a, blocks, st);
+    case 2: return launch_tag<T, uint16_t, R>(a, blocks, st);
+    case 4: return launch_tag<T, uint32_t, R>(a, blocks, st);
+    }
+    return -1;
+}
+
+// ------------------------------------------------------------- shell
+// The nodes around the fused box (CPML slabs and their one-node margin)
+// are updated one node per thread, H then E, with the expressions of the
+// two-kernel path (yee_kernels.hip): H from the A buffers into B, then E
+// from E^n (A) and H^{n+1/2} (B, written by the fused kernel and the shell
+// H kernel) into B.  psi and the ADE state are in place: no other launch
+// touches the slab memories.
+
+// Node of flat shell index t (false past the end).  The box is chosen with
+// compile-time indices so the argument struct is never indexed at run time.
+__device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, int &i, int &j, int &k) {
+    if (t >= sh.start[sh.n]) return false;
+    long long s0 = 0;
+    int l0 = 0, l1 = 0, l2 = 0, n1 = 1, n2 = 1;
+#pragma unroll
+    for (int b = 0; b < 6; b++) {
+        if (b < sh.n && t >= sh.start[b]) {
+            s0 = sh.start[b];
+            l0 = sh.lo[b][0];
+            l1 = sh.lo[b][1];
+            l2 = sh.lo[b][2];
+            n1 = sh.hi[b][1] - sh.lo[b][1];
+            n2 = sh.hi[b][2] - sh.lo[b][2];
+        }
+    }
+    const long long r = t - s0;
+    k = l2 + (int)(r % n2);
+    const long long q = r / n2;
+    j = l1 + (int)(q % n1);
+    i = l0 + (int)(q / n1);
+    return true;
+}
+
+template <typename T, bool CPML>
+__global__ void __launch_bounds__(256)
+k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__restrict__ ey,
+          const T *__restrict__ ez, const T *__restrict__ hxa, const T *__restrict__ hya,
+          const T *__restrict__ hza, T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb, T dx,
+          rf_cpml_t cp) {
+    const Geo g = geo_of(gg);
+    int i, j, k;
+    if (!shell_node(sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+    const bool do_x = (i < g.nx + !g.px) && j < g.ny && k < g.nz;
+    const bool do_y = i < g.nx && (j < g.ny + !g.py) && k < g.nz;
+    const bool do_z = i < g.nx && j < g.ny && (k < g.nz + !g.pz);
+    // +1 neighbours with periodic wrap (grid.h:26-28)
+    const long long di = (g.px && i + 1 == g.nx) ? -(long long)i * g.sx : g.sx;
+    const long long dj = (g.py && j + 1 == g.ny) ? -(long long)j * g.sy : g.sy;
+    const long long dk = (g.pz && k + 1 == g.nz) ? -(long long)k : 1;
+    T hx = hxa[idx], hy = hya[idx], hz = hza[idx];
+    const T exc = ex[idx], eyc = ey[idx], ezc = ez[idx];
+    if (do_x) {   // grid.c:70, cpml.c:185-189
+        T dA = ez[idx + dj] - ezc;
+        T dB = ey[idx + dk] - eyc;
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[0][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[0][1]), i, j, k, dx);
+        }
+        hx -= cb * (dA - dB);
+    }
+    if (do_y) {   // grid.c:78, cpml.c:198-202
+        T dA = ex[idx + dk] - exc;
+        T dB = ez[idx + di] - ezc;
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[1][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[1][1]), i, j, k, dx);
+        }
+        hy -= cb * (dA - dB);
+    }
+    if (do_z) {   // grid.c:86, cpml.c:211-215
+        T dA = ey[idx + di] - eyc;
+        T dB = ex[idx + dj] - exc;
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[2][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[2][1]), i, j, k, dx);
+        }
+        hz -= cb * (dA - dB);
+    }
+    hxb[idx] = hx;
+    hyb[idx] = hy;
+    hzb[idx] = hz;
+}
+
+// E update of one component on a shell node (e_component of the
+// two-kernel path, returning the value).
 template <typename T, typename TagT>
-__device__ __forceinline__ T e_new_value(int c, T e_old, T dA, T dB, const rf_mat_t &m, unsigned tag, int i,
-                                         int j, int k) {
+__device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_mat_t &m, unsigned tag, int i,
+                                           int j, int k) {
     using Entry = typename EntryOf<T>::type;
     using Ade = typename EntryOf<T>::ade;
     if constexpr (std::is_same<TagT, NoTag>::value) {
@@ -137,274 +515,202 @@ __device__ __forceinline__ T e_new_value(int c, T e_old, T dA, T dB, const rf_ma
     }
 }
 
-template <typename T, typename TagT, bool CPML, int TJ>
-__global__ void __launch_bounds__(TK *(TJ + 2))
-k_leapfrog(rf_leap_t a) {
-    __shared__ T sE[3][TJ + 2][TK + 2];
-    __shared__ T sH[3][TJ + 1][TK + 1];
-
-    const Geo g = geo_of(a.g);
-    const int lane = threadIdx.x;
-    const int w = threadIdx.y;
-    const int j0 = blockIdx.y * TJ;
-    const int k0 = blockIdx.x * TK;
-    const int jmax = g.py ? g.ny - 1 : g.ny;
-    const int kmax = g.pz ? g.nz - 1 : g.nz;
-    const int imax = g.px ? g.nx - 1 : g.nx;
-
-    const T *EA[3] = {static_cast<const T *>(a.ea[0]), static_cast<const T *>(a.ea[1]),
-                      static_cast<const T *>(a.ea[2])};
-    const T *HA[3] = {static_cast<const T *>(a.ha[0]), static_cast<const T *>(a.ha[1]),
-                      static_cast<const T *>(a.ha[2])};
-    T *EB[3] = {static_cast<T *>(a.eb[0]), static_cast<T *>(a.eb[1]), static_cast<T *>(a.eb[2])};
-    T *HB[3] = {static_cast<T *>(a.hb[0]), static_cast<T *>(a.hb[1]), static_cast<T *>(a.hb[2])};
-    const T cb = static_cast<T>(a.cb), dx = static_cast<T>(a.dx);
-
-    // ---- role of this thread -------------------------------------------
-    // position: a node whose H (and, when owned, E) this thread computes;
-    // (r, c): its cell in the LDS planes (row 0 = j0-1, col 0 = k0-1).
-    int pj = -1, pk = -1, r = 0, c = 0;
-    bool pos = false, owned = false;
-    // a second LDS cell this thread refreshes from E^n(i) every plane
-    int xr = -1, xc = -1, xj = -1, xk = -1;
-    if (w < TJ) {
-        pj = j0 + w;
-        pk = k0 + lane;
-        r = w + 1;
-        c = lane + 1;
-        owned = pos = pj <= jmax && pk <= kmax;
-        if (!pos) {   // a periodic wrap cell just past the tile end still feeds neighbours
-            const int wj = wrap_or_invalid(pj, g.ny, g.py), wk = wrap_or_invalid(pk, g.nz, g.pz);
-            if (wj >= 0 && wk >= 0 && (pj <= jmax + 1) && (pk <= kmax + 1)) {
-                xr = r; xc = c; xj = wj; xk = wk;
-            }
-        }
-    } else if (w == TJ) {
-        pj = j0 - 1;
-        if (pj < 0) pj = g.py ? g.ny - 1 : -1;
-        pk = k0 + lane;
-        r = 0;
-        c = lane + 1;
-        pos = pj >= 0 && pk <= kmax;
-        // the frame row below the tile: (j0+TJ, k0+lane)
-        const int wj = wrap_or_invalid(j0 + TJ, g.ny, g.py), wk = wrap_or_invalid(k0 + lane, g.nz, g.pz);
-        if (wj >= 0 && wk >= 0 && j0 + TJ <= jmax + 1 && k0 + lane <= kmax + 1) {
-            xr = TJ + 1; xc = lane + 1; xj = wj; xk = wk;
-        }
-    } else {
-        if (lane >= 1 && lane <= TJ) {         // halo column positions (j0+lane-1, k0-1)
-            pj = j0 + lane - 1;
-            pk = k0 - 1;
-            if (pk < 0) pk = g.pz ? g.nz - 1 : -1;
-            r = lane;
-            c = 0;
-            pos = pj <= jmax && pk >= 0;
-        } else if (lane == 0 || lane == TJ + 1) {   // frame corners of column k0-1
-            const int wj = wrap_or_invalid(j0 - 1 + lane, g.ny, g.py), wk = wrap_or_invalid(k0 - 1, g.nz, g.pz);
-            if (wj >= 0 && wk >= 0) { xr = lane; xc = 0; xj = wj; xk = wk; }
-        } else if (lane >= 32 && lane <= 33 + TJ) {   // frame column k0+64
-            const int rr = lane - 32;
-            const int wj = wrap_or_invalid(j0 - 1 + rr, g.ny, g.py), wk = wrap_or_invalid(k0 + TK, g.nz, g.pz);
-            if (wj >= 0 && wk >= 0 && k0 + TK <= kmax + 1) { xr = rr; xc = TK + 1; xj = wj; xk = wk; }
-        }
+template <typename T, typename TagT, bool CPML>
+__global__ void __launch_bounds__(256)
+k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__restrict__ eya,
+          const T *__restrict__ eza, const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
+          T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_mat_t m,
+          const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
+    const Geo g = geo_of(gg);
+    int i, j, k;
+    if (!shell_node(sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+    // E ranges (grid.c:95-117): own axis from 0, the other axes from 1
+    // unless periodic (PEC planes at 0 and n are never updated)
+    const bool in = i < g.nx && j < g.ny && k < g.nz;
+    const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
+    const bool do_x = in && j_ok && k_ok, do_y = in && i_ok && k_ok, do_z = in && i_ok && j_ok;
+    // -1 neighbours with wrap at 0 (materials.c:808-810)
+    const long long di = (i == 0) ? (long long)(g.nx - 1) * g.sx : -g.sx;
+    const long long dj = (j == 0) ? (long long)(g.ny - 1) * g.sy : -g.sy;
+    const long long dk = (k == 0) ? (long long)(g.nz - 1) : -1;
+    const long long step = *d_step;
+    T e0 = exa[idx], e1 = eya[idx], e2 = eza[idx];
+    if (n_srcs) {
+        e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 0);
+        e1 = apply_sources<T>(srcs, n_srcs, e1, 1, i, j, k, g.nx, g.ny, g.nz, step, 0);
+        e2 = apply_sources<T>(srcs, n_srcs, e2, 2, i, j, k, g.nx, g.ny, g.nz, step, 0);
     }
-
-    // ---- x range of this block ------------------------------------------
-    const int chunk = a.chunk > 0 ? a.chunk : imax + 1;
-    const int i_begin = blockIdx.z * chunk;
-    if (i_begin > imax) return;
-    const int i_end = min(i_begin + chunk, imax + 1);
-    // E^{n+1}(i_begin) needs H^{n+1/2}(i_begin - 1): compute it first
-    // without storing anything (plane nx-1 when x is periodic).
-    const bool warm = i_begin > 0 || g.px;
-    int i = warm ? (i_begin == 0 ? g.nx - 1 : i_begin - 1) : i_begin;
-    const int iters = (i_end - i_begin) + (warm ? 1 : 0);
-
-    const long long step = *a.d_step;
     unsigned tag = 0;
-    T ecur[3] = {T(0), T(0), T(0)}, enx[3];
-    T hp_y = T(0), hp_z = T(0);
-    if (pos) {
-        const long long idx = (long long)i * g.sx + (long long)pj * g.sy + pk;
-        for (int q = 0; q < 3; q++) ecur[q] = EA[q][idx];
+    if constexpr (!std::is_same<TagT, NoTag>::value) tag = load_tag<TagT>(m.tag, idx);
+    if (do_x) {   // materials.c:975-1000
+        T dA = hz[idx] - hz[idx + dj];
+        T dB = hy[idx] - hy[idx + dk];
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[0][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
+        }
+        e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k);
     }
-
-    for (int it = 0; it < iters; it++) {
-        const bool first = warm && it == 0;
-        int ip = i + 1;
-        bool has_ip = true;
-        if (ip > g.nx || (g.px && ip == g.nx)) {
-            if (g.px) ip = 0; else has_ip = false;
+    if (do_y) {   // materials.c:1015-1040
+        T dA = hx[idx] - hx[idx + dk];
+        T dB = hz[idx] - hz[idx + di];
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[1][1]), i, j, k, dx);
         }
-        // 1. stage E^n(i) and fetch E^n(i+1) for the next plane
-        if (pos) {
-            for (int q = 0; q < 3; q++) sE[q][r][c] = ecur[q];
-            if (has_ip) {
-                const long long idxn = (long long)ip * g.sx + (long long)pj * g.sy + pk;
-                for (int q = 0; q < 3; q++) enx[q] = EA[q][idxn];
-            } else {
-                for (int q = 0; q < 3; q++) enx[q] = T(0);
-            }
-        }
-        if (xr >= 0) {
-            const long long idx = (long long)i * g.sx + (long long)xj * g.sy + xk;
-            for (int q = 0; q < 3; q++) sE[q][xr][xc] = EA[q][idx];
-        }
-        __syncthreads();
-
-        // 2. H^{n+1/2}(i) at this position
-        if (pos) {
-            const long long idx = (long long)i * g.sx + (long long)pj * g.sy + pk;
-            const bool do_x = (i < g.nx + !g.px) && pj < g.ny && pk < g.nz;
-            const bool do_y = i < g.nx && (pj < g.ny + !g.py) && pk < g.nz;
-            const bool do_z = i < g.nx && pj < g.ny && (pk < g.nz + !g.pz);
-            const bool store = owned && !first;
-            // halo nodes compute only what the tile's E update reads: Hx
-            // on both, Hz on the halo row (Ex), Hy on the halo column (Ex)
-            const bool need_y = owned || (w == TJ + 1);
-            const bool need_z = owned || (w == TJ);
-            T hn[3] = {T(0), T(0), T(0)};
-            {
-                T h = HA[0][idx];
-                if (do_x) {
-                    T dA = sE[2][r + 1][c] - sE[2][r][c];
-                    T dB = sE[1][r][c + 1] - sE[1][r][c];
-                    if (CPML) {
-                        dA = stretch_rw<T>(dA, a.cp, g, true, 1, pj, static_cast<const T *>(a.cp.psi_h[0][0]),
-                                           store ? static_cast<T *>(a.psi_hb[0][0]) : nullptr, i, pj, pk, dx);
-                        dB = stretch_rw<T>(dB, a.cp, g, true, 2, pk, static_cast<const T *>(a.cp.psi_h[0][1]),
-                                           store ? static_cast<T *>(a.psi_hb[0][1]) : nullptr, i, pj, pk, dx);
-                    }
-                    h -= cb * (dA - dB);
-                }
-                hn[0] = h;
-            }
-            if (need_y) {
-                T h = HA[1][idx];
-                if (do_y) {
-                    T dA = sE[0][r][c + 1] - sE[0][r][c];
-                    T dB = enx[2] - sE[2][r][c];
-                    if (CPML) {
-                        dA = stretch_rw<T>(dA, a.cp, g, true, 2, pk, static_cast<const T *>(a.cp.psi_h[1][0]),
-                                           store ? static_cast<T *>(a.psi_hb[1][0]) : nullptr, i, pj, pk, dx);
-                        dB = stretch_rw<T>(dB, a.cp, g, true, 0, i, static_cast<const T *>(a.cp.psi_h[1][1]),
-                                           store ? static_cast<T *>(a.psi_hb[1][1]) : nullptr, i, pj, pk, dx);
-                    }
-                    h -= cb * (dA - dB);
-                }
-                hn[1] = h;
-            }
-            if (need_z) {
-                T h = HA[2][idx];
-                if (do_z) {
-                    T dA = enx[1] - sE[1][r][c];
-                    T dB = sE[0][r + 1][c] - sE[0][r][c];
-                    if (CPML) {
-                        dA = stretch_rw<T>(dA, a.cp, g, true, 0, i, static_cast<const T *>(a.cp.psi_h[2][0]),
-                                           store ? static_cast<T *>(a.psi_hb[2][0]) : nullptr, i, pj, pk, dx);
-                        dB = stretch_rw<T>(dB, a.cp, g, true, 1, pj, static_cast<const T *>(a.cp.psi_h[2][1]),
-                                           store ? static_cast<T *>(a.psi_hb[2][1]) : nullptr, i, pj, pk, dx);
-                    }
-                    h -= cb * (dA - dB);
-                }
-                hn[2] = h;
-            }
-            if (store)
-                for (int q = 0; q < 3; q++) HB[q][idx] = hn[q];
-            const int hr = r, hc = c;   // H plane uses the same (row, col) frame origin
-            sH[0][hr][hc] = hn[0];
-            if (need_y) sH[1][hr][hc] = hn[1];
-            if (need_z) sH[2][hr][hc] = hn[2];
-        }
-        __syncthreads();
-
-        // 3. E^{n+1}(i) on the tile
-        if (owned) {
-            const T hx_ = sH[0][r][c], hy_ = sH[1][r][c], hz_ = sH[2][r][c];
-            if (!first) {
-                const long long idx = (long long)i * g.sx + (long long)pj * g.sy + pk;
-                if constexpr (!std::is_same<TagT, NoTag>::value) tag = load_tag<TagT>(a.mat.tag, idx);
-                const bool in_x = i < g.nx, in_y = pj < g.ny, in_z = pk < g.nz;
-                const bool i_ok = g.px || i >= 1, j_ok = g.py || pj >= 1, k_ok = g.pz || pk >= 1;
-                const bool do_x = in_x && in_y && in_z && j_ok && k_ok;
-                const bool do_y = in_x && in_y && in_z && i_ok && k_ok;
-                const bool do_z = in_x && in_y && in_z && i_ok && j_ok;
-                T en[3];
-                for (int q = 0; q < 3; q++) en[q] = pre_sources<T>(a, ecur[q], idx, q, i, pj, pk, g, step);
-                if (do_x) {   // materials.c:975-1000
-                    T dA = hz_ - sH[2][r - 1][c];
-                    T dB = hy_ - sH[1][r][c - 1];
-                    if (CPML) {
-                        dA = stretch_rw<T>(dA, a.cp, g, false, 1, pj, static_cast<const T *>(a.cp.psi_e[0][0]),
-                                           static_cast<T *>(a.cp.psi_e[0][0]), i, pj, pk, dx);
-                        dB = stretch_rw<T>(dB, a.cp, g, false, 2, pk, static_cast<const T *>(a.cp.psi_e[0][1]),
-                                           static_cast<T *>(a.cp.psi_e[0][1]), i, pj, pk, dx);
-                    }
-                    en[0] = e_new_value<T, TagT>(0, en[0], dA, dB, a.mat, tag, i, pj, pk);
-                }
-                if (do_y) {   // materials.c:1015-1040
-                    T dA = hx_ - sH[0][r][c - 1];
-                    T dB = hz_ - hp_z;
-                    if (CPML) {
-                        dA = stretch_rw<T>(dA, a.cp, g, false, 2, pk, static_cast<const T *>(a.cp.psi_e[1][0]),
-                                           static_cast<T *>(a.cp.psi_e[1][0]), i, pj, pk, dx);
-                        dB = stretch_rw<T>(dB, a.cp, g, false, 0, i, static_cast<const T *>(a.cp.psi_e[1][1]),
-                                           static_cast<T *>(a.cp.psi_e[1][1]), i, pj, pk, dx);
-                    }
-                    en[1] = e_new_value<T, TagT>(1, en[1], dA, dB, a.mat, tag, i, pj, pk);
-                }
-                if (do_z) {   // materials.c:1050-1058
-                    T dA = hy_ - hp_y;
-                    T dB = hx_ - sH[0][r - 1][c];
-                    if (CPML) {
-                        dA = stretch_rw<T>(dA, a.cp, g, false, 0, i, static_cast<const T *>(a.cp.psi_e[2][0]),
-                                           static_cast<T *>(a.cp.psi_e[2][0]), i, pj, pk, dx);
-                        dB = stretch_rw<T>(dB, a.cp, g, false, 1, pj, static_cast<const T *>(a.cp.psi_e[2][1]),
-                                           static_cast<T *>(a.cp.psi_e[2][1]), i, pj, pk, dx);
-                    }
-                    en[2] = e_new_value<T, TagT>(2, en[2], dA, dB, a.mat, tag, i, pj, pk);
-                }
-                for (int q = 0; q < 3; q++) EB[q][idx] = post_sources<T>(a, en[q], q, i, pj, pk, g, step);
-            }
-            hp_y = hy_;
-            hp_z = hz_;
-        }
-        for (int q = 0; q < 3; q++) ecur[q] = enx[q];
-        i = first ? i_begin : i + 1;
-        __syncthreads();
+        e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k);
     }
+    if (do_z) {   // materials.c:1050-1058
+        T dA = hy[idx] - hy[idx + di];
+        T dB = hx[idx] - hx[idx + dj];
+        if (CPML) {
+            dA = stretch<T>(dA, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[2][0]), i, j, k, dx);
+            dB = stretch<T>(dB, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[2][1]), i, j, k, dx);
+        }
+        e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k);
+    }
+    if (n_srcs) {
+        e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 1);
+        e1 = apply_sources<T>(srcs, n_srcs, e1, 1, i, j, k, g.nx, g.ny, g.nz, step, 1);
+        e2 = apply_sources<T>(srcs, n_srcs, e2, 2, i, j, k, g.nx, g.ny, g.nz, step, 1);
+    }
+    exb[idx] = e0;
+    eyb[idx] = e1;
+    ezb[idx] = e2;
 }
 
-template <typename T, typename TagT, int TJ>
-int launch_tag(int has_cpml, const rf_leap_t *a, hipStream_t st) {
-    const int imax = (a->g.per & 1) ? a->g.n[0] - 1 : a->g.n[0];
-    const int chunk = a->chunk > 0 ? a->chunk : imax + 1;
-    const int jn = (a->g.per & 2) ? a->g.n[1] : a->g.n[1] + 1;
-    const int kn = (a->g.per & 4) ? a->g.n[2] : a->g.n[2] + 1;
-    dim3 grd((kn + TK - 1) / TK, (jn + TJ - 1) / TJ, (imax + 1 + chunk - 1) / chunk);
-    dim3 blk(TK, TJ + 2, 1);
-    if (has_cpml)
-        hipLaunchKernelGGL((k_leapfrog<T, TagT, true, TJ>), grd, blk, 0, st, *a);
-    else
-        hipLaunchKernelGGL((k_leapfrog<T, TagT, false, TJ>), grd, blk, 0, st, *a);
+template <typename T>
+int shell_h_t(const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6], void *const hb[3], double cb,
+              double dx, const rf_cpml_t *cp, hipStream_t st) {
+    const long long n = sh->start[sh->n];
+    const dim3 grd((unsigned)((n + 255) / 256));
+    rf_cpml_t c = cp ? *cp : rf_cpml_t{};
+#define RF_SHELL_H(CP)                                                                                         \
+    hipLaunchKernelGGL((k_shell_h<T, CP>), grd, dim3(256), 0, st, *g, *sh, static_cast<const T *>(fa[0]),      \
+                       static_cast<const T *>(fa[1]), static_cast<const T *>(fa[2]),                           \
+                       static_cast<const T *>(fa[3]), static_cast<const T *>(fa[4]),                           \
+                       static_cast<const T *>(fa[5]), static_cast<T *>(hb[0]), static_cast<T *>(hb[1]),        \
+                       static_cast<T *>(hb[2]), static_cast<T>(cb), static_cast<T>(dx), c)
+    if (cp) RF_SHELL_H(true);
+    else RF_SHELL_H(false);
+#undef RF_SHELL_H
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <typename T, int TJ>
-int launch_prec(int has_cpml, const rf_leap_t *a, hipStream_t st) {
-    if (!a->mat.tag) return launch_tag<T, NoTag, TJ>(has_cpml, a, st);
-    switch (a->mat.tag_bytes) {
-    case 1: return launch_tag<T, uint8_t, TJ>(has_cpml, a, st);
-    case 2: return launch_tag<T, uint16_t, TJ>(has_cpml, a, st);
-    case 4: return launch_tag<T, uint32_t, TJ>(has_cpml, a, st);
+template <typename T, typename TagT>
+int shell_e_tag(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void *const hb[3], void *const eb[3],
+                double dx, const rf_cpml_t *cp, const rf_mat_t *m, const rf_leap_src_t *srcs, int n_srcs,
+                const long long *d_step, hipStream_t st) {
+    const long long n = sh->start[sh->n];
+    const dim3 grd((unsigned)((n + 255) / 256));
+    rf_cpml_t c = cp ? *cp : rf_cpml_t{};
+#define RF_SHELL_E(CP)                                                                                         \
+    hipLaunchKernelGGL((k_shell_e<T, TagT, CP>), grd, dim3(256), 0, st, *g, *sh,                              \
+                       static_cast<const T *>(ea[0]), static_cast<const T *>(ea[1]),                           \
+                       static_cast<const T *>(ea[2]), static_cast<const T *>(hb[0]),                           \
+                       static_cast<const T *>(hb[1]), static_cast<const T *>(hb[2]), static_cast<T *>(eb[0]),  \
+                       static_cast<T *>(eb[1]), static_cast<T *>(eb[2]), static_cast<T>(dx), c, *m, srcs,      \
+                       n_srcs, d_step)
+    if (cp) RF_SHELL_E(true);
+    else RF_SHELL_E(false);
+#undef RF_SHELL_E
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+int shell_e_t(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void *const hb[3], void *const eb[3],
+              double dx, const rf_cpml_t *cp, const rf_mat_t *m, const rf_leap_src_t *srcs, int n_srcs,
+              const long long *d_step, hipStream_t st) {
+    if (!m->tag) return shell_e_tag<T, NoTag>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
+    switch (m->tag_bytes) {
+    case 1: return shell_e_tag<T, uint8_t>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
+    case 2: return shell_e_tag<T, uint16_t>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
+    case 4: return shell_e_tag<T, uint32_t>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
     }
     return -1;
 }
 
 }  // namespace
 
-extern "C" int rf_launch_leapfrog(int prec, int has_cpml, const rf_leap_t *a, rf_stream_t *s) {
-    if (a->n_pts > RF_LEAP_MAX_PTS || a->n_planes > RF_LEAP_MAX_PLANES) return -1;
+extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves) {
+    for (int q = 0; q < 3; q++) {
+        const int top = ((a->g.per >> q) & 1) ? a->g.n[q] : a->g.n[q] + 1;
+        if (a->ib0[q] < 0 || a->ib1[q] > top) return -1;
+    }
+    const long long planes = a->ib1[0] - a->ib0[0];
+    const long long jn = a->ib1[1] - a->ib0[1];
+    const long long kn = a->ib1[2] - a->ib0[2];
+    if (rows <= 0) rows = 4;
+    if (prec == 0 ? !(rows == 2 || rows == 4) : !(rows == 4 || rows == 8)) return -1;
+    a->rows = rows;
+    if (planes <= 0 || jn <= 0 || kn <= 0) {   /* empty box: nothing to launch */
+        a->ntk = a->ntj = a->n_waves = a->xcd_q = 0;
+        a->chunk = 1;
+        return 0;
+    }
+    a->ntk = (int)((kn + KW - 1) / KW);
+    a->ntj = (int)((jn + rows - 1) / rows);
+    const long long tiles = (long long)a->ntk * a->ntj;
+    if (chunk <= 0) {
+        if (target_waves <= 0) target_waves = 4096;
+        long long chunks = (target_waves + tiles - 1) / tiles;
+        if (chunks < 1) chunks = 1;
+        chunk = (int)((planes + chunks - 1) / chunks);
+        if (chunk < 4) chunk = 4;
+    }
+    if (chunk > planes) chunk = (int)planes;
+    a->chunk = chunk;
+    const long long nch = (planes + chunk - 1) / chunk;
+    const long long waves = tiles * nch;
+    if (waves > (1LL << 30)) return -1;
+    a->n_waves = (int)waves;
+    const long long blocks = (waves + WPB - 1) / WPB;
+    a->xcd_q = (int)((blocks + N_XCD - 1) / N_XCD);
+    // up to 40 KiB of LDS per block for the material table
+    const int cap = (40 * 1024) / (prec == 0 ? coef_bytes<double>() : coef_bytes<float>());
+    a->ncache = a->tag ? (a->n_entries < cap ? a->n_entries : cap) : 0;
+    return 0;
+}
+
+extern "C" int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s) {
+    if (a->n_waves == 0) return 0;
+    if (a->n_waves < 0 || a->ntk <= 0 || a->ntj <= 0 || a->chunk <= 0) return -1;
+    if (a->n_srcs > 0 && !a->srcs) return -1;
+    if (a->tag && !a->cold) return -1;
     hipStream_t st = rf_raw_stream(s);
-    return prec == 0 ? launch_prec<double, 8>(has_cpml, a, st) : launch_prec<float, 8>(has_cpml, a, st);
+    const unsigned blocks = static_cast<unsigned>(a->xcd_q) * N_XCD;
+    if (prec == 0) {
+        switch (a->rows) {
+        case 2: return launch_rows<double, 2>(a, blocks, st);
+        case 4: return launch_rows<double, 4>(a, blocks, st);
+        }
+        return -1;
+    }
+    switch (a->rows) {
+    case 4: return launch_rows<float, 4>(a, blocks, st);
+    case 8: return launch_rows<float, 8>(a, blocks, st);
+    }
+    return -1;
+}
+
+extern "C" int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6],
+                                 void *const hb[3], double cb, double dx, const rf_cpml_t *cpml, rf_stream_t *s) {
+    if (sh->n < 0 || sh->n > 6) return -1;
+    if (sh->n == 0 || sh->start[sh->n] == 0) return 0;
+    hipStream_t st = rf_raw_stream(s);
+    return prec == 0 ? shell_h_t<double>(g, sh, fa, hb, cb, dx, cpml, st)
+                     : shell_h_t<float>(g, sh, fa, hb, cb, dx, cpml, st);
+}
+
+extern "C" int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3],
+                                 void *const hb[3], void *const eb[3], double dx, const rf_cpml_t *cpml,
+                                 const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs,
+                                 const long long *d_step, rf_stream_t *s) {
+    if (sh->n < 0 || sh->n > 6) return -1;
+    if (sh->n == 0 || sh->start[sh->n] == 0) return 0;
+    if (n_srcs > 0 && !srcs) return -1;
+    hipStream_t st = rf_raw_stream(s);
+    return prec == 0 ? shell_e_t<double>(g, sh, ea, hb, eb, dx, cpml, mat, srcs, n_srcs, d_step, st)
+                     : shell_e_t<float>(g, sh, ea, hb, eb, dx, cpml, mat, srcs, n_srcs, d_step, st);
 }

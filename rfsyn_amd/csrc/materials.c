@@ -632,6 +632,7 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
             tags[idx] = (uint32_t)t;
         }
     d->n_entries = set.n;
+    d->d.n_entries = (int)set.n;
 
     /* ADE box: nodes whose entry has a dispersive component */
     int any_ade = 0;

@@ -62,6 +62,10 @@ typedef struct {
     const void *kinv_m[3], *b_m[3], *a_m[3];    /* n+1 entries per axis */
     const void *kinv_h[3], *b_h[3], *a_h[3];    /* n entries per axis */
     int lo[3], hi[3], len[3];
+    /* plain range per axis: for q in [plo, phi) both kinv profiles are 1
+     * and q lies outside the slab, so the stretched difference equals the
+     * plain one (up to the sign of a zero) */
+    int plo[3], phi[3];
     /* psi per component c and derivative s (axis (c+1+s)%3) */
     void *psi_h[3][2];
     void *psi_e[3][2];
@@ -111,6 +115,7 @@ typedef struct {
     int n_slots;
     void *J, *P;                /* n_slots * 3 * cells each */
     double vac_ca;              /* vacuum Cb = dt/(eps0 dx) when tag == NULL */
+    int n_entries;              /* entries in the table */
 } rf_mat_t;
 
 /* One tensor edge: position, component, kind and neighbour weights.
@@ -142,6 +147,7 @@ typedef struct {
     int comp;                   /* 0..2 */
     int hard;
     const double *wave;         /* value per absolute step */
+    int i, j, k;                /* node of idx */
 } rf_point_src_t;
 
 typedef struct {
@@ -149,36 +155,84 @@ typedef struct {
     const double *wave;         /* value per absolute step */
 } rf_plane_src_t;
 
+/* One source of the fused step, in application order: the pre-E entries
+ * (point sources in insertion order, then plane sources) and after them
+ * the post-E plane sources. */
+typedef struct {
+    int plane;                  /* 0 = point node (i,j,k), 1 = plane axis/index */
+    int comp, hard, after;
+    int axis, index;            /* plane; index already wrapped on a periodic axis */
+    int i, j, k;                /* point */
+    const double *wave;         /* value per absolute step */
+} rf_leap_src_t;
+
+/* Data of the fused step that only its dispersive-edge path reads, kept
+ * in device memory so the hot loop does not hold it in registers: the ADE
+ * tables and state. */
+typedef struct {
+    const void *entries, *ade;
+    void *J, *P;
+    int b0[3], bn[3];
+} rf_leap_cold_t;
+
 /* Fused leapfrog step: one kernel computes H^{n+1/2} and then E^{n+1}
- * from the "A" buffers (E^n, H^{n-1/2}) into the "B" buffers, marching
- * along x over (j,k) tiles and recomputing the one-cell H halo of each
- * tile.  psi_h is double-buffered (read cp.psi_h, write psi_hb) because
- * halo recomputation reads a neighbour's psi; psi_e and the ADE state are
- * updated in place.  Every node of B is written (copies where a component
- * is not updated), so B is a complete state.  Pre-E point and plane
- * sources and post-E plane sources are applied inside the kernel in the
- * order of the step loop. */
-#define RF_LEAP_MAX_PTS 4
-#define RF_LEAP_MAX_PLANES 4
+ * from the "A" buffers (E^n, H^{n-1/2}) into the "B" buffers for the nodes
+ * of the box [ib0, ib1), which must lie inside the CPML plain ranges with
+ * a one-node margin (no CPML term is evaluated).  A wavefront owns `rows`
+ * rows (y) by 62 columns (z) of the box and marches along x over `chunk`
+ * planes, recomputing the one-cell H halo of its tile from the A buffers.
+ * The ADE state is updated in place.  Every node of the box is written to
+ * B (copies where a component is not updated).  The sources of the device
+ * table `srcs` are applied in its order; src_box bounds the nodes they
+ * touch (i0,i1,j0,j1,k0,k1 inclusive).  rf_leap_plan fills the tiling
+ * fields (rows, chunk: 0 = automatic). */
 typedef struct {
     rf_geom_t g;
     const void *ea[3], *ha[3];
     void *eb[3], *hb[3];
     double cb, dx;
-    rf_cpml_t cp;
-    void *psi_hb[3][2];
-    rf_mat_t mat;
-    int n_pts;
-    rf_point_src_t pts[RF_LEAP_MAX_PTS];
-    int n_planes;
-    rf_plane_src_t planes[RF_LEAP_MAX_PLANES];
-    int plane_after[RF_LEAP_MAX_PLANES];
+    int ib0[3], ib1[3];         /* owned node box */
+    const void *tag;            /* material tags, NULL = vacuum */
+    int tag_bytes, n_entries;
+    const void *entries;
+    double vac_ca;
+    const rf_leap_cold_t *cold; /* device */
+    const rf_leap_src_t *srcs;  /* device table */
+    int n_srcs;
+    int src_box[6];
     const long long *d_step;
-    int chunk;                  /* x planes per block (0: all) */
+    /* tiling, filled by rf_leap_plan */
+    int rows, chunk;            /* rows per wavefront, x planes per wavefront */
+    int ntk, ntj;               /* tiles along z and y */
+    int n_waves, xcd_q;         /* wavefronts; blocks per XCD */
+    int ncache;                 /* material entries staged in LDS per block */
 } rf_leap_t;
 
-/* has_cpml selects the CPML variant; tag_bytes 0 = vacuum (mat.vac_ca). */
-int rf_launch_leapfrog(int prec, int has_cpml, const rf_leap_t *a, rf_stream_t *s);
+/* Tiling of the fused step over a->ib0..ib1; rows: 2 or 4 (FP64), 4 or 8
+ * (FP32), 0 = default; chunk 0 = enough x chunks for about target_waves
+ * wavefronts.  0 on success, -1 for an unsupported request. */
+int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves);
+/* tag NULL = vacuum (vac_ca); an empty box launches nothing. */
+int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s);
+
+/* The nodes around the fused step's box: up to 6 disjoint boxes [lo, hi),
+ * flattened with k fastest; box b holds flat indices [start[b],
+ * start[b+1]). */
+typedef struct {
+    int n;
+    int lo[6][3], hi[6][3];
+    long long start[7];
+} rf_shell_t;
+
+/* Shell H: H^{n+1/2} of the shell nodes from E^n and H^{n-1/2} of the A
+ * buffers (fa: ex ey ez hx hy hz) into hb, psi_h in place. */
+int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6], void *const hb[3],
+                      double cb, double dx, const rf_cpml_t *cpml, rf_stream_t *s);
+/* Shell E: sources and E^{n+1} of the shell nodes from E^n (ea) and
+ * H^{n+1/2} (hb) into eb, psi_e and the ADE state in place. */
+int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void *const hb[3],
+                      void *const eb[3], double dx, const rf_cpml_t *cpml, const rf_mat_t *mat,
+                      const rf_leap_src_t *srcs, int n_srcs, const long long *d_step, rf_stream_t *s);
 
 /* ----------------------------------------------------------- launches
  * `prec` selects the field precision: 0 = double, 1 = float.  All

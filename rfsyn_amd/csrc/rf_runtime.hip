@@ -47,7 +47,12 @@ int rf_malloc(void **p, size_t bytes) {
     if (rf_init() != 0) return -1;
     if (bytes == 0) bytes = 256;
     RF_CHECK(hipMalloc(p, bytes));
-    hipError_t e = hipMemset(*p, 0, bytes);
+    // The library's streams are non-blocking and do not wait for the null
+    // stream, so the fill must be complete before the buffer is handed out:
+    // a fill still queued behind earlier large fills would otherwise land
+    // after the caller's first upload into the buffer and erase it.
+    hipError_t e = hipMemsetAsync(*p, 0, bytes, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         (void)hipFree(*p);
         *p = nullptr;

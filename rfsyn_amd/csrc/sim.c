@@ -20,6 +20,7 @@
 #include "constants.h"
 
 #include <math.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -105,6 +106,13 @@ struct fdtd_sim {
     int fused;
     int parity;
     void *f2[6];
+    int leap_rows, leap_chunk, leap_waves;   /* tiling requests (0: automatic) */
+    rf_leap_src_t *d_srcs;   /* device source table of the fused step */
+    rf_leap_cold_t *d_cold;  /* its cold data (ADE) */
+    int ib0[3], ib1[3];      /* node box of the fused kernel */
+    rf_shell_t shell;        /* the nodes around it (CPML slabs + margin) */
+    int n_srcs, src_box[6];
+    int post_in_kernel;      /* post-E plane sources inside the fused kernel */
     rf_point_src_t *h_pts;   /* host copy of the point sources (device wave tables) */
 
     /* kernel timing with HIP events (profiling mode launches directly) */
@@ -171,7 +179,7 @@ fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml, const mater
         if (dalloc(s, &s->f[q], s->cells * s->es)) goto fail;
     if (upload_fields(s, g)) goto fail;
     if (pml) {
-        if (rf_cpml_build(pml, s->prec, 1, &s->cp) != 0) goto fail;
+        if (rf_cpml_build(pml, s->prec, &s->cp) != 0) goto fail;
         s->has_cpml = 1;
         s->bytes += s->cp.bytes;
     }
@@ -206,6 +214,8 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     free(s->pt_wave);
     free(s->pts);
     rf_free(s->d_pts);
+    rf_free(s->d_srcs);
+    rf_free(s->d_cold);
     free(s->planes);
     free(s->probes);
     free(s->freqs_all);
@@ -340,13 +350,57 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index, const doub
     return (int)s->n_mons - 1;
 }
 
-/* The fused leapfrog kernel carries up to RF_LEAP_MAX_PTS point and
- * RF_LEAP_MAX_PLANES plane sources in its arguments; RFSYN_FUSED=0 in the
+/* The fused leapfrog kernel is the default step; RFSYN_FUSED=0 in the
  * environment selects the separate H and E kernels instead. */
 static int want_fused(const fdtd_sim_t *s) {
+    (void)s;
     const char *env = getenv("RFSYN_FUSED");
-    if (env && env[0] == '0') return 0;
-    return s->n_pts <= RF_LEAP_MAX_PTS && s->n_planes <= RF_LEAP_MAX_PLANES;
+    return !(env && env[0] == '0');
+}
+
+/* The fused kernel owns the nodes whose coordinate on every axis lies in
+ * the CPML plain range with a one-node margin below it (so the halo it
+ * recomputes is plain too); the rest (CPML slabs and margin) form up to 6
+ * shell boxes updated node by node. */
+static void plan_regions(fdtd_sim_t *s) {
+    const size_t n[3] = {s->nx, s->ny, s->nz};
+    int top[3];
+    for (int a = 0; a < 3; a++) {
+        top[a] = ((s->per >> a) & 1) ? (int)n[a] : (int)n[a] + 1;
+        s->ib0[a] = 0;
+        s->ib1[a] = top[a];
+        if (s->has_cpml && !((s->per >> a) & 1)) {
+            const int plo = s->cp.d.plo[a], phi = s->cp.d.phi[a];
+            s->ib0[a] = plo == 0 ? 0 : plo + 1;
+            s->ib1[a] = phi < top[a] ? phi : top[a];
+            if (s->ib1[a] < s->ib0[a]) s->ib1[a] = s->ib0[a];
+        }
+    }
+    rf_shell_t *sh = &s->shell;
+    memset(sh, 0, sizeof *sh);
+    const int *b0 = s->ib0, *b1 = s->ib1;
+    int box[6][2][3] = {
+        {{0, 0, 0}, {b0[0], top[1], top[2]}},
+        {{b1[0], 0, 0}, {top[0], top[1], top[2]}},
+        {{b0[0], 0, 0}, {b1[0], b0[1], top[2]}},
+        {{b0[0], b1[1], 0}, {b1[0], top[1], top[2]}},
+        {{b0[0], b0[1], 0}, {b1[0], b1[1], b0[2]}},
+        {{b0[0], b0[1], b1[2]}, {b1[0], b1[1], top[2]}},
+    };
+    long long at = 0;
+    for (int q = 0; q < 6; q++) {
+        long long cnt = 1;
+        for (int a = 0; a < 3; a++) cnt *= box[q][1][a] > box[q][0][a] ? box[q][1][a] - box[q][0][a] : 0;
+        if (!cnt) continue;
+        for (int a = 0; a < 3; a++) {
+            sh->lo[sh->n][a] = box[q][0][a];
+            sh->hi[sh->n][a] = box[q][1][a];
+        }
+        sh->start[sh->n] = at;
+        at += cnt;
+        sh->n++;
+    }
+    sh->start[sh->n] = at;
 }
 
 /* Device arrays for probes and monitors, allocated once at the first run. */
@@ -360,6 +414,28 @@ static int finalize(fdtd_sim_t *s) {
                 return -1;
         if (rf_sync(s->st)) return -1;
         if (s->K & 1) s->K++;   /* a captured block must return to its buffer parity */
+        s->post_in_kernel = !(s->has_mat && s->mat.n_tensor);
+        rf_leap_cold_t cold;
+        memset(&cold, 0, sizeof cold);
+        if (s->has_mat) {
+            const rf_mat_t *m = &s->mat.d;
+            cold.entries = m->entries;
+            cold.ade = m->ade;
+            cold.J = m->J;
+            cold.P = m->P;
+            for (int a = 0; a < 3; a++) {
+                cold.b0[a] = m->b0[a];
+                cold.bn[a] = m->bn[a];
+            }
+        }
+        if (dalloc(s, (void **)&s->d_cold, sizeof cold) || rf_h2d(s->d_cold, &cold, sizeof cold, s->st) ||
+            rf_sync(s->st))
+            return -1;
+        plan_regions(s);
+        const char *e;
+        s->leap_rows = (e = getenv("RFSYN_LEAP_ROWS")) ? atoi(e) : 0;
+        s->leap_chunk = (e = getenv("RFSYN_LEAP_CHUNK")) ? atoi(e) : 0;
+        s->leap_waves = (e = getenv("RFSYN_LEAP_WAVES")) ? atoi(e) : 0;
     }
     int P = (int)s->n_probes;
     size_t cap = 0;
@@ -426,6 +502,61 @@ static int finalize(fdtd_sim_t *s) {
     return 0;
 }
 
+/* The fused step's source table in application order (point sources,
+ * pre-E plane sources, then the post-E plane sources it applies itself)
+ * and the bounding box of the nodes they touch. */
+static int build_src_table(fdtd_sim_t *s) {
+    size_t n = s->n_pts + s->n_planes;
+    rf_leap_src_t *t = calloc(n ? n : 1, sizeof *t);
+    if (!t) return -1;
+    int m = 0;
+    int box[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
+    const int nn[3] = {(int)s->nx, (int)s->ny, (int)s->nz};
+    for (size_t p = 0; p < s->n_pts; p++) {
+        rf_leap_src_t *e = &t[m++];
+        e->plane = 0;
+        e->comp = s->h_pts[p].comp;
+        e->hard = s->h_pts[p].hard;
+        e->after = 0;
+        e->i = s->h_pts[p].i;
+        e->j = s->h_pts[p].j;
+        e->k = s->h_pts[p].k;
+        e->wave = s->h_pts[p].wave;
+        const int x[3] = {e->i, e->j, e->k};
+        for (int a = 0; a < 3; a++) {
+            if (x[a] < box[2 * a]) box[2 * a] = x[a];
+            if (x[a] > box[2 * a + 1]) box[2 * a + 1] = x[a];
+        }
+    }
+    for (int pass = 0; pass < 2; pass++)
+        for (size_t p = 0; p < s->n_planes; p++) {
+            const splane_t *pl = &s->planes[p];
+            if ((pl->after_e != 0) != pass || (pass && !s->post_in_kernel)) continue;
+            rf_leap_src_t *e = &t[m++];
+            e->plane = 1;
+            e->comp = pl->d.comp;
+            e->hard = pl->d.hard;
+            e->after = pass;
+            e->axis = pl->d.axis;
+            e->index = pl->d.index;
+            if ((s->per >> e->axis) & 1) e->index %= nn[e->axis];
+            e->wave = pl->d.wave;
+            for (int a = 0; a < 3; a++) {
+                const int lo = a == e->axis ? e->index : 0, hi = a == e->axis ? e->index : nn[a] - 1;
+                if (lo < box[2 * a]) box[2 * a] = lo;
+                if (hi > box[2 * a + 1]) box[2 * a + 1] = hi;
+            }
+        }
+    int rc = 0;
+    if (m && !s->d_srcs) rc = dalloc(s, (void **)&s->d_srcs, n * sizeof *t);
+    if (!rc && m) rc = rf_h2d(s->d_srcs, t, (size_t)m * sizeof *t, s->st) || rf_sync(s->st);
+    free(t);
+    if (rc) return -1;
+    s->n_srcs = m;
+    memcpy(s->src_box, box, sizeof box);
+    return 0;
+}
+
 /* Waveform tables cover steps [0, need); grown geometrically. */
 static int ensure_waves(fdtd_sim_t *s, size_t need) {
     if (need <= s->wave_cap) return 0;
@@ -460,6 +591,9 @@ static int ensure_waves(fdtd_sim_t *s, size_t need) {
             hp[p].idx = (long long)fdtd_index(&tmp, s->pts[p].i, s->pts[p].j, s->pts[p].k);
             hp[p].comp = (int)s->pts[p].component;
             hp[p].hard = s->pts[p].mode == SRC_HARD;
+            hp[p].i = (int)(hp[p].idx / s->geom.sx);
+            hp[p].j = (int)((hp[p].idx % s->geom.sx) / s->geom.sy);
+            hp[p].k = (int)(hp[p].idx % s->geom.sy);
             hp[p].wave = s->waves[s->pt_wave[p]].d_tab;
         }
         int rc = 0;
@@ -470,6 +604,7 @@ static int ensure_waves(fdtd_sim_t *s, size_t need) {
         if (rc) return -1;
     }
     for (size_t p = 0; p < s->n_planes; p++) s->planes[p].d.wave = s->waves[s->planes[p].wave].d_tab;
+    if (s->fused && build_src_table(s)) return -1;
     /* sources pass their tables by value: captured blocks are stale */
     for (int q = 0; q < 2; q++) {
         rf_graph_destroy(s->graphs[q]);
@@ -478,20 +613,13 @@ static int ensure_waves(fdtd_sim_t *s, size_t need) {
     return 0;
 }
 
-/* x planes per block of the fused kernel: enough blocks to fill the
- * chip several times over, each marching at least 16 planes. */
-static int leap_chunk(const fdtd_sim_t *s) {
-    const int TJ = 8;
-    long long kn = (s->per & 4) ? (long long)s->nz : (long long)s->nz + 1;
-    long long jn = (s->per & 2) ? (long long)s->ny : (long long)s->ny + 1;
-    long long in = (s->per & 1) ? (long long)s->nx : (long long)s->nx + 1;
-    long long tiles = ((kn + 63) / 64) * ((jn + TJ - 1) / TJ);
-    long long chunks = (2048 + tiles - 1) / tiles;
-    long long chunk = (in + chunks - 1) / chunks;
-    if (chunk < 16) chunk = 16;
-    return (int)chunk;
+static int mark(fdtd_sim_t *s, int q) {
+    if (!s->profiling) return 0;
+    return rf_event_record(s->ev[s->ev_used + (size_t)q], s->st);
 }
 
+/* The fused step: the leapfrog kernel on the plain box (timed in event
+ * slots 0-1) and the shell kernels around it (slots 2-3). */
 static int enqueue_fused(fdtd_sim_t *s) {
     rf_leap_t a;
     memset(&a, 0, sizeof a);
@@ -504,55 +632,60 @@ static int enqueue_fused(fdtd_sim_t *s) {
     }
     a.cb = s->dt / (MU0 * s->dx);
     a.dx = s->dx;
-    if (s->has_cpml) {
-        a.cp = s->cp.d;
-        for (int c = 0; c < 3; c++)
-            for (int q = 0; q < 2; q++) a.psi_hb[c][q] = s->cp.psi_h2[c][q];
+    for (int q = 0; q < 3; q++) {
+        a.ib0[q] = s->ib0[q];
+        a.ib1[q] = s->ib1[q];
     }
-    a.mat = s->has_mat ? s->mat.d : s->vac;
-    a.n_pts = (int)s->n_pts;
-    for (size_t p = 0; p < s->n_pts; p++) a.pts[p] = s->h_pts[p];
-    a.n_planes = (int)s->n_planes;
-    for (size_t p = 0; p < s->n_planes; p++) {
-        a.planes[p] = s->planes[p].d;
-        a.plane_after[p] = s->planes[p].after_e;
-    }
+    const rf_mat_t *m = s->has_mat ? &s->mat.d : &s->vac;
+    a.tag = m->tag;
+    a.tag_bytes = m->tag_bytes;
+    a.n_entries = m->n_entries;
+    a.entries = m->entries;
+    a.vac_ca = m->vac_ca;
+    a.cold = s->d_cold;
+    a.srcs = s->d_srcs;
+    a.n_srcs = s->n_srcs;
+    memcpy(a.src_box, s->src_box, sizeof a.src_box);
     a.d_step = s->d_step;
-    a.chunk = leap_chunk(s);
-    if (rf_launch_leapfrog(s->prec, s->has_cpml, &a, s->st)) return -1;
+    if (rf_leap_plan(s->prec, &a, s->leap_rows, s->leap_chunk, s->leap_waves)) {
+        rf_set_error("fused step: unsupported tiling (rows %d, chunk %d)", s->leap_rows, s->leap_chunk);
+        return -1;
+    }
+    if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
+    if (s->shell.n) {
+        const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
+        if (rf_launch_shell_h(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, a.cb, s->dx, cp, s->st) ||
+            rf_launch_shell_e(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, s->f2, s->dx, cp, m, s->d_srcs,
+                              s->n_srcs, s->d_step, s->st))
+            return -1;
+    }
+    if (mark(s, 3)) return -1;
     /* B now holds state n+1 */
     for (int q = 0; q < 6; q++) {
         void *t = s->f[q];
         s->f[q] = s->f2[q];
         s->f2[q] = t;
     }
-    if (s->has_cpml)
-        for (int c = 0; c < 3; c++)
-            for (int q = 0; q < 2; q++) {
-                void *t = s->cp.d.psi_h[c][q];
-                s->cp.d.psi_h[c][q] = s->cp.psi_h2[c][q];
-                s->cp.psi_h2[c][q] = t;
-            }
     s->parity ^= 1;
     return 0;
 }
 
-static int mark(fdtd_sim_t *s, int q) {
-    if (!s->profiling) return 0;
-    return rf_event_record(s->ev[s->ev_used + (size_t)q], s->st);
-}
 
 static int enqueue_step(fdtd_sim_t *s) {
     const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
     const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
     double cb = s->dt / (MU0 * s->dx);
     if (s->fused) {
-        /* one kernel for H, sources and E; timed as the "H" slot */
-        if (mark(s, 0) || enqueue_fused(s) || mark(s, 1) || mark(s, 2) || mark(s, 3)) return -1;
+        if (enqueue_fused(s)) return -1;
         if (s->profiling) s->ev_used += 4;
         if (s->has_mat && s->mat.n_tensor &&
             rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
             return -1;
+        /* with tensor edges the post-E sources follow the tensor update */
+        for (size_t p = 0; p < s->n_planes && !s->post_in_kernel; p++)
+            if (s->planes[p].after_e &&
+                rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
+                return -1;
     } else {
         if (mark(s, 0)) return -1;
         if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
@@ -703,6 +836,15 @@ size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }
 
 int fdtd_sim_fused(const fdtd_sim_t *s) { return s ? s->fused : -1; }
 
+int fdtd_sim_fused_box(const fdtd_sim_t *s, int lo[3], int hi[3]) {
+    if (!s || !lo || !hi || !s->started || !s->fused) return -1;
+    for (int a = 0; a < 3; a++) {
+        lo[a] = s->ib0[a];
+        hi[a] = s->ib1[a];
+    }
+    return 0;
+}
+
 long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n) {
     if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out)) return -1;
     size_t have = s->steps < s->probes[probe].td_cap ? s->steps : s->probes[probe].td_cap;
@@ -768,7 +910,15 @@ int fdtd_sim_download(fdtd_sim_t *s, fdtd_grid_t *g) {
 
 int fdtd_sim_upload(fdtd_sim_t *s, const fdtd_grid_t *g) {
     if (!same_shape(s, g)) return -1;
-    return upload_fields(s, g);
+    if (upload_fields(s, g)) return -1;
+    /* the fused step leaves nodes it does not update (the padding of a
+     * periodic axis) as they are in B: keep B equal to A there */
+    if (s->fused && s->started) {
+        for (int q = 0; q < 6; q++)
+            if (rf_d2d(s->f2[q], s->f[q], s->cells * s->es, s->st)) return -1;
+        if (rf_sync(s->st)) return -1;
+    }
+    return 0;
 }
 
 int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic) {

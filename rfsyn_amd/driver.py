@@ -138,10 +138,11 @@ class NativeRun:
                                          C.cast(C.pointer(params), C.c_void_p))
         assert self.lib.fdtd_sim_add_point_source(self.sim, C.byref(src)) == 0
 
-    def add_plane_source(self, comp, axis, index, wf_name, params, amp=1.0, after_e=0):
+    def add_plane_source(self, comp, axis, index, wf_name, params, amp=1.0, after_e=0, hard=False):
         self._keep.append(params)
         wf = capi.waveform_ptr(self.lib, wf_name)
-        assert self.lib.fdtd_sim_add_plane_source(self.sim, comp, axis, index, capi.SRC_SOFT, after_e, amp,
+        mode = capi.SRC_HARD if hard else capi.SRC_SOFT
+        assert self.lib.fdtd_sim_add_plane_source(self.sim, comp, axis, index, mode, after_e, amp,
                                                   wf, C.cast(C.pointer(params), C.c_void_p)) == 0
 
     def add_probe(self, comp, pos, td_cap, freqs=None):
