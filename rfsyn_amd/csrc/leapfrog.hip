@@ -104,6 +104,34 @@ __device__ __forceinline__ T apply_sources(const rf_leap_src_t *tab, int n, T v,
     return v;
 }
 
+// A source of the table with its value at the current step, staged in
+// LDS at the start of the kernel.
+struct LSrc {
+    double w;
+    int plane, comp, hard, after, axis, index, i, j, k;
+};
+constexpr int MAX_LSRC = 16;
+
+template <typename T>
+__device__ __forceinline__ T apply_lsrc(const LSrc *tab, int n, T v, int c, int i, int j, int k, int gnx, int gny,
+                                        int gnz, int after) {
+    for (int s = 0; s < n; s++) {
+        const LSrc &p = tab[s];
+        if (p.after != after || p.comp != c) continue;
+        bool hit;
+        if (p.plane) {
+            const int ax = p.axis;
+            const int x = ax == 0 ? i : (ax == 1 ? j : k);
+            const bool in_x = ax == 0 || i < gnx, in_y = ax == 1 || j < gny, in_z = ax == 2 || k < gnz;
+            hit = x == p.index && in_x && in_y && in_z;
+        } else {
+            hit = p.i == i && p.j == j && p.k == k;
+        }
+        if (hit) v = p.hard ? static_cast<T>(p.w) : static_cast<T>(static_cast<double>(v) + p.w);
+    }
+    return v;
+}
+
 // Coefficients of one material entry: Ca, Cb per component and the flags
 // of the three components packed one byte each.  Entries below `ncache`
 // come from the block's LDS copy of the table, the rest from HBM.
@@ -136,42 +164,165 @@ __device__ __forceinline__ Coefs<T> entry_coefs(const void *entries, const T *s_
     return o;
 }
 
-// Flagged edge (tensor: keep the old value for the tensor kernel;
-// dispersive: trapezoidal ADE), same arithmetic as e_component of the
-// two-kernel path.
-template <typename T>
-__device__ __forceinline__ T e_flagged(int c, uint32_t fl, T ca, T cb, T e_old, T dA, T dB,
-                                       const rf_leap_cold_t *cold, unsigned tag, int i, int j, int k) {
-    using Entry = typename EntryOf<T>::type;
-    using Ade = typename EntryOf<T>::ade;
-    if (fl & RF_EDGE_TENSOR) return e_old;
-    if (fl & RF_EDGE_ADE) {
-        const rf_leap_cold_t &m = *cold_view(cold);
-        const Entry &en = static_cast<const Entry *>(m.entries)[tag];
-        const Ade &ad = static_cast<const Ade *>(m.ade)[en.ade[c]];
-        const long long cells = (long long)m.bn[0] * m.bn[1] * m.bn[2];
-        const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
-        return ade_edge<T>(ad, ca, cb, e_old, dA, dB, static_cast<T *>(m.J), static_cast<T *>(m.P), 3 * cells,
-                           (long long)c * cells + cell);
+// Trapezoidal ADE update of one edge with at most two poles whose state
+// (j[s], p[s]) is already in registers: the arithmetic, in order, of
+// ade_edge (yee_device.h).  Returns E^{n+1} and the new state.
+template <typename T, typename Ade>
+__device__ __forceinline__ T ade2(const Ade &ad, T ca, T cb, T e_old, T dA, T dB, const T j[2], const T p[2],
+                                  T nj[2], T np[2]) {
+    T u[2];
+    T sum_u = T(0);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (s < ad.n_poles) {
+            u[s] = ad.has_p[s] ? ad.kj[s] * j[s] - ad.kp[s] * p[s] : ad.kj[s] * j[s];
+            sum_u += u[s];
+        }
     }
-    return ca * e_old + cb * (dA - dB);
+    const T e_new = ca * e_old + cb * (dA - dB) - ad.Cu * sum_u;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (s < ad.n_poles) {
+            const T S = u[s] + ad.v[s] * (e_new + e_old);
+            nj[s] = S - j[s];
+            np[s] = ad.has_p[s] ? p[s] + ad.hdt * S : p[s];
+        }
+    }
+    return e_new;
 }
 
-// LDS bytes per cached material entry.
+// The three components of a node whose entry has flags: tensor edges keep
+// the old value (the tensor kernel updates them), dispersive edges take
+// the trapezoidal ADE step with their state loaded for all components
+// before any is updated, the rest the plain update.  ADE coefficient sets
+// below n_adec come from LDS.
 template <typename T>
-constexpr int coef_bytes() { return 6 * static_cast<int>(sizeof(T)) + 4; }
+__device__ __forceinline__ void e_flagged3(T e[3], const T dA[3], const T dB[3], const bool on[3],
+                                           const T ca[3], const T cb[3], uint32_t fl, uint32_t ai,
+                                           const typename EntryOf<T>::ade *s_ade, int n_adec, int n_slots,
+                                           const rf_leap_cold_t *cold, int i, int j, int k) {
+    using Ade = typename EntryOf<T>::ade;
+    bool ade_c[3];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const uint32_t f = (fl >> (8 * c)) & 0xffu;
+        ade_c[c] = on[c] && (f & RF_EDGE_ADE) && !(f & RF_EDGE_TENSOR);
+        any = any || ade_c[c];
+        if (on[c] && !(f & (RF_EDGE_ADE | RF_EDGE_TENSOR))) e[c] = ca[c] * e[c] + cb[c] * (dA[c] - dB[c]);
+    }
+    if (!any) return;
+    const rf_leap_cold_t &m = *cold_view(cold);
+    T *J = static_cast<T *>(m.J), *P = static_cast<T *>(m.P);
+    const long long st = m.ade_stride;
+    const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
+    if (n_slots <= 2) {
+        T jv[3][2], pv[3][2];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                jv[c][s] = pv[c][s] = T(0);
+                if (ade_c[c] && s < n_slots) {
+                    const long long o = s * 3 * st + c * st + cell;
+                    jv[c][s] = J[o];
+                    pv[c][s] = P[o];
+                }
+            }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (!ade_c[c]) continue;
+            const int id = (ai >> (8 * c)) & 0xff;
+            const Ade &ad = id < n_adec ? s_ade[id] : static_cast<const Ade *>(m.ade)[id];
+            T nj[2], np[2];
+            e[c] = ade2<T>(ad, ca[c], cb[c], e[c], dA[c], dB[c], jv[c], pv[c], nj, np);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                if (s < ad.n_poles) {
+                    const long long o = s * 3 * st + c * st + cell;
+                    J[o] = nj[s];
+                    if (ad.has_p[s]) P[o] = np[s];
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (!ade_c[c]) continue;
+            const int id = (ai >> (8 * c)) & 0xff;
+            const Ade &ad = id < n_adec ? s_ade[id] : static_cast<const Ade *>(m.ade)[id];
+            e[c] = ade_edge<T>(ad, ca[c], cb[c], e[c], dA[c], dB[c], J, P, 3 * st, (long long)c * st + cell);
+        }
+    }
+}
 
-template <typename T, typename TagT, int R>
+// Packed ADE set ids of an entry read from the table in HBM.
+template <typename T>
+__device__ __forceinline__ uint32_t entry_ade_ids(const void *entries, unsigned tag) {
+    const typename EntryOf<T>::type &en = static_cast<const typename EntryOf<T>::type *>(entries)[tag];
+    uint32_t ai = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) ai |= (static_cast<uint32_t>(en.ade[c]) & 0xffu) << (8 * c);
+    return ai;
+}
+
+// LDS bytes per cached material entry: Ca, Cb per component, packed flags
+// and packed ADE set ids.
+template <typename T>
+constexpr int coef_bytes() { return 6 * static_cast<int>(sizeof(T)) + 8; }
+
+// Per-lane CPML profile values along z (the lane axis): loaded once per
+// wavefront, since a lane keeps its column for the whole march.
+template <typename T>
+struct ZProf {
+    T kh, bh, ah, km, bm, am;
+    int cq;        // compressed slab coordinate, -1 outside the psi slab
+    bool on;       // outside the plain range: the stretched term applies
+};
+
+// One z CPML term (cpml.h:46-51) for a lane outside the plain range, with
+// the psi value already loaded (0 outside the psi slab).
+template <typename T>
+__device__ __forceinline__ T zterm(T diff, T kinv, T b, T aa, bool slab, T old, T &psi_new, T dx) {
+    const T psi = slab ? b * old + aa * diff : T(0);
+    psi_new = psi;
+    return diff * kinv + dx * psi;
+}
+
+template <typename T, typename TagT, bool ZC, int R>
 __global__ void __launch_bounds__(WL *WPB)
 k_leapfrog(rf_leap_t a) {
     constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
-    // the first a.ncache entries of the material table, staged in LDS
-    // before any wavefront of the block may leave
+    using Ade = typename EntryOf<T>::ade;
+    // the ADE coefficient sets and the first a.ncache entries of the
+    // material table, staged in LDS before any wavefront of the block may
+    // leave
     extern __shared__ __align__(16) unsigned char s_raw[];
-    const int ncache = a.ncache;
-    T *s_ca = reinterpret_cast<T *>(s_raw);
+    const int ncache = a.ncache, n_adec = a.n_adec;
+    const int nls = a.n_srcs <= MAX_LSRC ? a.n_srcs : 0;   // more sources: read from HBM
+    LSrc *s_src = reinterpret_cast<LSrc *>(s_raw);
+    Ade *s_ade = reinterpret_cast<Ade *>(s_raw + (size_t)nls * sizeof(LSrc));
+    T *s_ca = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(s_ade) + (size_t)n_adec * sizeof(Ade));
     T *s_cb = s_ca + 3 * ncache;
     uint32_t *s_fl = reinterpret_cast<uint32_t *>(s_cb + 3 * ncache);
+    uint32_t *s_ai = s_fl + ncache;
+    const long long step = *a.d_step;
+    for (int e = threadIdx.x; e < nls; e += blockDim.x) {
+        const rf_leap_src_t &p = a.srcs[e];
+        LSrc q;
+        q.w = p.wave[step];
+        q.plane = p.plane;
+        q.comp = p.comp;
+        q.hard = p.hard;
+        q.after = p.after;
+        q.axis = p.axis;
+        q.index = p.index;
+        q.i = p.i;
+        q.j = p.j;
+        q.k = p.k;
+        s_src[e] = q;
+    }
+    if (!TAGS && nls > 0) __syncthreads();
     if constexpr (TAGS) {
         using Entry = typename EntryOf<T>::type;
         const Entry *ents = static_cast<const Entry *>(a.entries);
@@ -183,23 +334,46 @@ k_leapfrog(rf_leap_t a) {
                 s_cb[3 * e + c] = en.Cb[c];
             }
             s_fl[e] = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+            uint32_t ai = 0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) ai |= (static_cast<uint32_t>(en.ade[c]) & 0xffu) << (8 * c);
+            s_ai[e] = ai;
         }
+        const Ade *ades = static_cast<const Ade *>(a.ade);
+        for (int e = threadIdx.x; e < n_adec; e += blockDim.x) s_ade[e] = ades[e];
         __syncthreads();
     }
 
     const int lane = threadIdx.x & (WL - 1);
-    const int wave_in_block = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lb = (blockIdx.x % N_XCD) * a.xcd_q + blockIdx.x / N_XCD;
-    const int wid = lb * WPB + wave_in_block;
-    if (wid >= a.n_waves) return;
-    const int tk = wid % a.ntk;
-    const int tj = (wid / a.ntk) % a.ntj;
-    const int tc = wid / (a.ntk * a.ntj);
-
+    // Work queue: the items (tile, x chunk) are split in N_XCD contiguous
+    // ranges, one per XCD (blocks are dealt round-robin to the XCDs), so
+    // the tiles an XCD runs at a time are neighbours that share halo rows
+    // in its L2; each wavefront takes the next item of its XCD's counter
+    // until the range is exhausted, which balances tiles of unequal cost
+    // (dispersive, source and CPML columns).  The counters of this launch
+    // are a.ctr[a.qpar][*]; the other set is zeroed here for the next one.
+    const int xcd = blockIdx.x % N_XCD;
+    if (blockIdx.x == 0 && threadIdx.x < N_XCD) a.ctr[(1 - a.qpar) * N_XCD + threadIdx.x] = 0u;
+    const int item_lo = (int)(((long long)a.n_items * xcd) / N_XCD);
+    const int item_hi = (int)(((long long)a.n_items * (xcd + 1)) / N_XCD);
+    unsigned *my_ctr = a.ctr + a.qpar * N_XCD + xcd;
     const Geo g = geo_of(a.g);
-    // the box of nodes this launch owns: [ib0, ib1) per axis
-    const int k0 = a.ib0[2] + tk * KW, j0 = a.ib0[1] + tj * R;
-    const int jend = a.ib1[1], kend = a.ib1[2];
+    for (;;) {
+    unsigned ticket = 0;
+    if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
+    const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
+    if (item >= item_hi) break;
+    const int tk = item % a.ntk;
+    const int tj = (item / a.ntk) % a.ntj;
+    const int tc = item / (a.ntk * a.ntj);
+
+    // the box of nodes this launch owns, [ib0, ib1) per axis; along z the
+    // tiles restart at kbrk (the start of the upper CPML slab)
+    const int nt0 = a.ntk0;
+    const int k0 = tk < nt0 ? a.ib0[2] + tk * KW : a.kbrk + (tk - nt0) * KW;
+    const int kend = tk < nt0 ? min(k0 + KW, a.kbrk) : a.ib1[2];
+    const int j0 = a.ib0[1] + tj * R;
+    const int jend = a.ib1[1];
     const int i_begin = a.ib0[0] + tc * a.chunk;
     const int i_end = min(i_begin + a.chunk, a.ib1[0]);
     const int sy = static_cast<int>(g.sy);
@@ -216,9 +390,29 @@ k_leapfrog(rf_leap_t a) {
         const int jt = j0 - 1 + r;
         jr[r] = g.py ? (jt + g.ny) % g.ny : min(max(jt, 0), g.ny);
     }
-    // sources: does their bounding box meet this tile's rows and columns
+    // z CPML profile of this column (grid.c stretched differences along z
+    // for hx, hy, ex, ey); any such lane in the wavefront
+    ZProf<T> zp;
+    bool zs = false;
+    if constexpr (ZC) {
+        zp.on = kc < a.zplo || kc >= a.zphi;
+        zp.cq = kc < a.zlo ? kc : (kc >= a.zhi ? a.zlo + kc - a.zhi : -1);
+        const int qh = min(kc, g.nz - 1);   // half-grid profiles have nz entries
+        zp.kh = zp.on ? static_cast<const T *>(a.zk[0])[qh] : T(1);
+        zp.bh = zp.on ? static_cast<const T *>(a.zk[1])[qh] : T(0);
+        zp.ah = zp.on ? static_cast<const T *>(a.zk[2])[qh] : T(0);
+        zp.km = zp.on ? static_cast<const T *>(a.zk[3])[kc] : T(1);
+        zp.bm = zp.on ? static_cast<const T *>(a.zk[4])[kc] : T(0);
+        zp.am = zp.on ? static_cast<const T *>(a.zk[5])[kc] : T(0);
+        zs = __ballot(zp.on) != 0;
+    }
+    T *zpsi = static_cast<T *>(a.zpsi);
+    const int zr = 2 * a.zpar, zw = 2 - 2 * a.zpar;   // H psi slots read / written
+    // sources: does their bounding box meet this tile's rows and columns;
+    // the components they touch before and after the E update
     const bool src_tile = a.n_srcs > 0 && a.src_box[2] < j0 + R && a.src_box[3] >= j0 &&
                           a.src_box[4] < k0 + KW && a.src_box[5] >= k0;
+    const bool src_lane = kc >= a.src_box[4] && kc <= a.src_box[5];
 
     const T *EA0 = static_cast<const T *>(a.ea[0]), *EA1 = static_cast<const T *>(a.ea[1]),
             *EA2 = static_cast<const T *>(a.ea[2]);
@@ -226,7 +420,7 @@ k_leapfrog(rf_leap_t a) {
             *HA2 = static_cast<const T *>(a.ha[2]);
     T *EB0 = static_cast<T *>(a.eb[0]), *EB1 = static_cast<T *>(a.eb[1]), *EB2 = static_cast<T *>(a.eb[2]);
     T *HB0 = static_cast<T *>(a.hb[0]), *HB1 = static_cast<T *>(a.hb[1]), *HB2 = static_cast<T *>(a.hb[2]);
-    const T cb = static_cast<T>(a.cb);
+    const T cb = static_cast<T>(a.cb), dx = static_cast<T>(a.dx);
     const T vac = static_cast<T>(a.vac_ca);
 
     // E^{n+1}(i_begin) needs H^{n+1/2}(i_begin - 1): a warm-up plane
@@ -234,7 +428,6 @@ k_leapfrog(rf_leap_t a) {
     const bool warm = i_begin > 0 || g.px;
     int i = warm ? (i_begin == 0 ? g.nx - 1 : i_begin - 1) : i_begin;
     const int iters = (i_end - i_begin) + (warm ? 1 : 0);
-    const long long step = *a.d_step;
 
     T cx[R + 2], cy[R + 2], cz[R + 2];   // E^n, plane i
     T nx[R + 2], ny[R + 2], nz[R + 2];   // E^n, plane i+1
@@ -278,6 +471,25 @@ k_leapfrog(rf_leap_t a) {
             hy[r] = r > 0 ? HA1[base + o] : T(0);   // the halo row needs hx, hz only
             hz[r] = HA2[base + o];
         }
+        // z-psi of the slab lanes, issued with the field loads: H of hx and
+        // hy (read slots) on rows 0..R, E of ex and ey on rows 1..R
+        T zh0[R + 1], zh1[R + 1], ze0[R + 1], ze1[R + 1];
+#pragma unroll
+        for (int r = 0; r <= R; r++) zh0[r] = zh1[r] = ze0[r] = ze1[r] = T(0);
+        if constexpr (ZC) {
+            if (zs && zp.cq >= 0) {
+#pragma unroll
+                for (int r = 0; r <= R; r++) {
+                    const long long zo = (((long long)i * (g.ny + 1) + jr[r]) * a.zlen + zp.cq) * 6;
+                    zh0[r] = zpsi[zo + zr];
+                    zh1[r] = zpsi[zo + zr + 1];
+                    if (r >= 1 && !first) {
+                        ze0[r] = zpsi[zo + 4];
+                        ze1[r] = zpsi[zo + 5];
+                    }
+                }
+            }
+        }
         // material tags of the owned rows, issued with the field loads
         unsigned tg[R + 1];
 #pragma unroll
@@ -298,9 +510,24 @@ k_leapfrog(rf_leap_t a) {
             const bool do_x = hx_i && jc < g.ny && kc < g.nz;
             const bool do_y = r >= 1 && hyz_i && jc < g.ny + !g.py && kc < g.nz;
             const bool do_z = hyz_i && jc < g.ny && kc < g.nz + !g.pz;
-            T dAx = cz[r + 1] - cz[r], dBx = eyk - cy[r];
-            T dAy = exk - cx[r], dBy = nz[r] - cz[r];
-            T dAz = ny[r] - cy[r], dBz = cx[r + 1] - cx[r];
+            const T dAx = cz[r + 1] - cz[r];
+            T dBx = eyk - cy[r];
+            T dAy = exk - cx[r];
+            const T dBy = nz[r] - cz[r];
+            const T dAz = ny[r] - cy[r], dBz = cx[r + 1] - cx[r];
+            if constexpr (ZC) {
+                if (zs && zp.on && (do_x || do_y)) {   // dBx, dAy are z differences
+                    const bool slab = zp.cq >= 0;
+                    T px_, py_;
+                    if (do_x) dBx = zterm<T>(dBx, zp.kh, zp.bh, zp.ah, slab, zh0[r], px_, dx);
+                    if (do_y) dAy = zterm<T>(dAy, zp.kh, zp.bh, zp.ah, slab, zh1[r], py_, dx);
+                    if (slab && own) {
+                        const long long zo = (((long long)i * (g.ny + 1) + jc) * a.zlen + zp.cq) * 6;
+                        if (do_x) zpsi[zo + zw] = px_;
+                        if (do_y) zpsi[zo + zw + 1] = py_;
+                    }
+                }
+            }
             const T nhx = hx[r] - cb * (dAx - dBx);
             const T nhy = hy[r] - cb * (dAy - dBy);
             const T nhz = hz[r] - cb * (dAz - dBz);
@@ -328,47 +555,66 @@ k_leapfrog(rf_leap_t a) {
                     const long long idx = base + jc * sy + kc;
                     const bool in = i < g.nx && jc < g.ny && kc < g.nz;
                     const bool j_ok = g.py || jc >= 1, k_ok = g.pz || kc >= 1;
-                    const bool do_x = in && j_ok && k_ok, do_y = in && i_ok && k_ok, do_z = in && i_ok && j_ok;
-                    T e0 = cx[r], e1 = cy[r], e2 = cz[r];
-                    if (src) {
-                        e0 = apply_sources<T>(a.srcs, a.n_srcs, e0, 0, i, jc, kc, g.nx, g.ny, g.nz, step, 0);
-                        e1 = apply_sources<T>(a.srcs, a.n_srcs, e1, 1, i, jc, kc, g.nx, g.ny, g.nz, step, 0);
-                        e2 = apply_sources<T>(a.srcs, a.n_srcs, e2, 2, i, jc, kc, g.nx, g.ny, g.nz, step, 0);
+                    const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
+                    T e[3] = {cx[r], cy[r], cz[r]};
+                    if (src && src_lane) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+                            if ((a.src_pre >> c) & 1)
+                                e[c] = nls == a.n_srcs
+                                           ? apply_lsrc<T>(s_src, nls, e[c], c, i, jc, kc, g.nx, g.ny, g.nz, 0)
+                                           : apply_sources<T>(a.srcs, a.n_srcs, e[c], c, i, jc, kc, g.nx, g.ny,
+                                                              g.nz, step, 0);
                     }
-                    T dAx = hz[r] - hz[r - 1], dBx = hy[r] - hyk;
-                    T dAy = hx[r] - hxk, dBy = hz[r] - pz[r];
-                    T dAz = hy[r] - py[r], dBz = hx[r] - hx[r - 1];
+                    T dA[3] = {hz[r] - hz[r - 1], hx[r] - hxk, hy[r] - py[r]};
+                    T dB[3] = {hy[r] - hyk, hz[r] - pz[r], hx[r] - hx[r - 1]};
+                    if constexpr (ZC) {
+                        if (zs && zp.on && (on[0] || on[1])) {   // dB[0], dA[1] are z differences
+                            const bool slab = zp.cq >= 0;
+                            T px_, py_;
+                            if (on[0]) dB[0] = zterm<T>(dB[0], zp.km, zp.bm, zp.am, slab, ze0[r], px_, dx);
+                            if (on[1]) dA[1] = zterm<T>(dA[1], zp.km, zp.bm, zp.am, slab, ze1[r], py_, dx);
+                            if (slab) {
+                                const long long zo = (((long long)i * (g.ny + 1) + jc) * a.zlen + zp.cq) * 6;
+                                if (on[0]) zpsi[zo + 4] = px_;
+                                if (on[1]) zpsi[zo + 5] = py_;
+                            }
+                        }
+                    }
                     if constexpr (!TAGS) {
-                        const T n0 = T(1) * e0 + vac * (dAx - dBx);
-                        const T n1 = T(1) * e1 + vac * (dAy - dBy);
-                        const T n2 = T(1) * e2 + vac * (dAz - dBz);
-                        e0 = do_x ? n0 : e0;
-                        e1 = do_y ? n1 : e1;
-                        e2 = do_z ? n2 : e2;
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            const T n = T(1) * e[c] + vac * (dA[c] - dB[c]);
+                            e[c] = on[c] ? n : e[c];
+                        }
                     } else {
                         const unsigned tag = tg[r];
                         const Coefs<T> kf = entry_coefs<T>(a.entries, s_ca, s_cb, s_fl, ncache, tag);
                         if (kf.fl == 0) {
-                            const T n0 = kf.ca[0] * e0 + kf.cb[0] * (dAx - dBx);
-                            const T n1 = kf.ca[1] * e1 + kf.cb[1] * (dAy - dBy);
-                            const T n2 = kf.ca[2] * e2 + kf.cb[2] * (dAz - dBz);
-                            e0 = do_x ? n0 : e0;
-                            e1 = do_y ? n1 : e1;
-                            e2 = do_z ? n2 : e2;
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                const T n = kf.ca[c] * e[c] + kf.cb[c] * (dA[c] - dB[c]);
+                                e[c] = on[c] ? n : e[c];
+                            }
                         } else {
-                            if (do_x) e0 = e_flagged<T>(0, kf.fl & 0xffu, kf.ca[0], kf.cb[0], e0, dAx, dBx, a.cold, tag, i, jc, kc);
-                            if (do_y) e1 = e_flagged<T>(1, (kf.fl >> 8) & 0xffu, kf.ca[1], kf.cb[1], e1, dAy, dBy, a.cold, tag, i, jc, kc);
-                            if (do_z) e2 = e_flagged<T>(2, (kf.fl >> 16) & 0xffu, kf.ca[2], kf.cb[2], e2, dAz, dBz, a.cold, tag, i, jc, kc);
+                            const uint32_t ai = static_cast<int>(tag) < ncache ? s_ai[tag]
+                                                                                : entry_ade_ids<T>(a.entries, tag);
+                            e_flagged3<T>(e, dA, dB, on, kf.ca, kf.cb, kf.fl, ai, s_ade, n_adec, a.n_slots, a.cold,
+                                          i, jc, kc);
                         }
                     }
-                    if (src) {
-                        e0 = apply_sources<T>(a.srcs, a.n_srcs, e0, 0, i, jc, kc, g.nx, g.ny, g.nz, step, 1);
-                        e1 = apply_sources<T>(a.srcs, a.n_srcs, e1, 1, i, jc, kc, g.nx, g.ny, g.nz, step, 1);
-                        e2 = apply_sources<T>(a.srcs, a.n_srcs, e2, 2, i, jc, kc, g.nx, g.ny, g.nz, step, 1);
+                    if (src && src_lane) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+                            if ((a.src_post >> c) & 1)
+                                e[c] = nls == a.n_srcs
+                                           ? apply_lsrc<T>(s_src, nls, e[c], c, i, jc, kc, g.nx, g.ny, g.nz, 1)
+                                           : apply_sources<T>(a.srcs, a.n_srcs, e[c], c, i, jc, kc, g.nx, g.ny,
+                                                              g.nz, step, 1);
                     }
-                    EB0[idx] = e0;
-                    EB1[idx] = e1;
-                    EB2[idx] = e2;
+                    EB0[idx] = e[0];
+                    EB1[idx] = e[1];
+                    EB2[idx] = e[2];
                 }
             }
         }
@@ -386,24 +632,7 @@ k_leapfrog(rf_leap_t a) {
         }
         i = first ? i_begin : i + 1;
     }
-}
-
-template <typename T, typename TagT, int R>
-int launch_tag(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
-    const size_t lds = std::is_same<TagT, NoTag>::value ? 0 : (size_t)a->ncache * coef_bytes<T>();
-    hipLaunchKernelGGL((k_leapfrog<T, TagT, R>), dim3(blocks), dim3(WL * WPB), lds, st, *a);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-template <typename T, int R>
-int launch_rows(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
-    if (!a->tag) return launch_tag<T, NoTag, R>(a, blocks, st);
-    switch (a->tag_bytes) {
-    case 1: return launch_tag<T, uint8_t, R>(a, blocks, st);
-    case 2: return launch_tag<T, uint16_t, R>(a, blocks, st);
-    case 4: return launch_tag<T, uint32_t, R>(a, blocks, st);
     }
-    return -1;
 }
 
 // ------------------------------------------------------------- shell
@@ -414,12 +643,19 @@ int launch_rows(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
 // H kernel) into B.  psi and the ADE state are in place: no other launch
 // touches the slab memories.
 
+__device__ __forceinline__ unsigned div_magic(unsigned n, unsigned m, unsigned s1, unsigned s2) {
+    const unsigned t = __umulhi(m, n);
+    return (t + ((n - t) >> s1)) >> s2;
+}
+
 // Node of flat shell index t (false past the end).  The box is chosen with
-// compile-time indices so the argument struct is never indexed at run time.
+// compile-time indices so the argument struct is never indexed at run time;
+// the index inside the box is split with multiply-high divisions.
 __device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, int &i, int &j, int &k) {
     if (t >= sh.start[sh.n]) return false;
     long long s0 = 0;
-    int l0 = 0, l1 = 0, l2 = 0, n1 = 1, n2 = 1;
+    int l0 = 0, l1 = 0, l2 = 0;
+    unsigned nk = 1, nj = 1, mk = 0, mj = 0, k1 = 0, k2 = 0, j1 = 0, j2 = 0;
 #pragma unroll
     for (int b = 0; b < 6; b++) {
         if (b < sh.n && t >= sh.start[b]) {
@@ -427,16 +663,42 @@ __device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, in
             l0 = sh.lo[b][0];
             l1 = sh.lo[b][1];
             l2 = sh.lo[b][2];
-            n1 = sh.hi[b][1] - sh.lo[b][1];
-            n2 = sh.hi[b][2] - sh.lo[b][2];
+            nj = static_cast<unsigned>(sh.hi[b][1] - sh.lo[b][1]);
+            nk = static_cast<unsigned>(sh.hi[b][2] - sh.lo[b][2]);
+            mk = sh.mk[b];
+            mj = sh.mj[b];
+            k1 = sh.sk[b][0];
+            k2 = sh.sk[b][1];
+            j1 = sh.sj[b][0];
+            j2 = sh.sj[b][1];
         }
     }
-    const long long r = t - s0;
-    k = l2 + (int)(r % n2);
-    const long long q = r / n2;
-    j = l1 + (int)(q % n1);
-    i = l0 + (int)(q / n1);
+    const unsigned r = static_cast<unsigned>(t - s0);
+    const unsigned q = div_magic(r, mk, k1, k2);
+    const unsigned p = div_magic(q, mj, j1, j2);
+    k = l2 + static_cast<int>(r - q * nk);
+    j = l1 + static_cast<int>(q - p * nj);
+    i = l0 + static_cast<int>(p);
     return true;
+}
+
+// z CPML term of a shell node with its psi in the z-psi store (same
+// arithmetic as stretch, yee_device.h): read slot rd, write slot wr.
+template <typename T>
+__device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, bool h, int i, int j, int k,
+                                      const rf_zpsi_t &z, int rd, int wr, T dx) {
+    const T *kinv = static_cast<const T *>(h ? c.kinv_h[2] : c.kinv_m[2]);
+    const T *bb = static_cast<const T *>(h ? c.b_h[2] : c.b_m[2]);
+    const T *aa = static_cast<const T *>(h ? c.a_h[2] : c.a_m[2]);
+    const int cq = slab_coord(c, 2, k);
+    T psi = T(0);
+    if (cq >= 0) {
+        T *zp = static_cast<T *>(z.zpsi);
+        const long long zo = (((long long)i * (g.ny + 1) + j) * z.zlen + cq) * 6;
+        psi = bb[k] * zp[zo + rd] + aa[k] * diff;
+        zp[zo + wr] = psi;
+    }
+    return diff * kinv[k] + dx * psi;
 }
 
 template <typename T, bool CPML>
@@ -444,7 +706,7 @@ __global__ void __launch_bounds__(256)
 k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__restrict__ ey,
           const T *__restrict__ ez, const T *__restrict__ hxa, const T *__restrict__ hya,
           const T *__restrict__ hza, T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb, T dx,
-          rf_cpml_t cp) {
+          rf_cpml_t cp, rf_zpsi_t zs) {
     const Geo g = geo_of(gg);
     int i, j, k;
     if (!shell_node(sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
@@ -463,7 +725,10 @@ k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__rest
         T dB = ey[idx + dk] - eyc;
         if (CPML) {
             dA = stretch<T>(dA, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[0][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[0][1]), i, j, k, dx);
+            if (zs.zpsi)
+                dB = zstretch<T>(dB, cp, g, true, i, j, k, zs, 2 * zs.zpar, 2 - 2 * zs.zpar, dx);
+            else
+                dB = stretch<T>(dB, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[0][1]), i, j, k, dx);
         }
         hx -= cb * (dA - dB);
     }
@@ -471,7 +736,10 @@ k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__rest
         T dA = ex[idx + dk] - exc;
         T dB = ez[idx + di] - ezc;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[1][0]), i, j, k, dx);
+            if (zs.zpsi)
+                dA = zstretch<T>(dA, cp, g, true, i, j, k, zs, 2 * zs.zpar + 1, 3 - 2 * zs.zpar, dx);
+            else
+                dA = stretch<T>(dA, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[1][0]), i, j, k, dx);
             dB = stretch<T>(dB, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[1][1]), i, j, k, dx);
         }
         hy -= cb * (dA - dB);
@@ -505,11 +773,11 @@ __device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_
         if (fl & RF_EDGE_TENSOR) return e_old;
         if (fl & RF_EDGE_ADE) {
             const Ade &ad = static_cast<const Ade *>(m.ade)[en.ade[c]];
-            const long long cells = (long long)m.bn[0] * m.bn[1] * m.bn[2];
+            const long long st = m.ade_stride;
             const long long cell =
                 ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
             return ade_edge<T>(ad, en.Ca[c], en.Cb[c], e_old, dA, dB, static_cast<T *>(m.J),
-                               static_cast<T *>(m.P), 3 * cells, (long long)c * cells + cell);
+                               static_cast<T *>(m.P), 3 * st, (long long)c * st + cell);
         }
         return en.Ca[c] * e_old + en.Cb[c] * (dA - dB);
     }
@@ -519,8 +787,8 @@ template <typename T, typename TagT, bool CPML>
 __global__ void __launch_bounds__(256)
 k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__restrict__ eya,
           const T *__restrict__ eza, const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
-          T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_mat_t m,
-          const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
+          T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_zpsi_t zs,
+          rf_mat_t m, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
     const Geo g = geo_of(gg);
     int i, j, k;
     if (!shell_node(sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
@@ -548,7 +816,10 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
         T dB = hy[idx] - hy[idx + dk];
         if (CPML) {
             dA = stretch<T>(dA, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[0][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
+            if (zs.zpsi)
+                dB = zstretch<T>(dB, cp, g, false, i, j, k, zs, 4, 4, dx);
+            else
+                dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
         }
         e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k);
     }
@@ -556,7 +827,10 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
         T dA = hx[idx] - hx[idx + dk];
         T dB = hz[idx] - hz[idx + di];
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
+            if (zs.zpsi)
+                dA = zstretch<T>(dA, cp, g, false, i, j, k, zs, 5, 5, dx);
+            else
+                dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
             dB = stretch<T>(dB, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[1][1]), i, j, k, dx);
         }
         e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k);
@@ -582,16 +856,17 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
 
 template <typename T>
 int shell_h_t(const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6], void *const hb[3], double cb,
-              double dx, const rf_cpml_t *cp, hipStream_t st) {
+              double dx, const rf_cpml_t *cp, const rf_zpsi_t *z, hipStream_t st) {
     const long long n = sh->start[sh->n];
     const dim3 grd((unsigned)((n + 255) / 256));
     rf_cpml_t c = cp ? *cp : rf_cpml_t{};
+    rf_zpsi_t zz = z ? *z : rf_zpsi_t{};
 #define RF_SHELL_H(CP)                                                                                         \
     hipLaunchKernelGGL((k_shell_h<T, CP>), grd, dim3(256), 0, st, *g, *sh, static_cast<const T *>(fa[0]),      \
                        static_cast<const T *>(fa[1]), static_cast<const T *>(fa[2]),                           \
                        static_cast<const T *>(fa[3]), static_cast<const T *>(fa[4]),                           \
                        static_cast<const T *>(fa[5]), static_cast<T *>(hb[0]), static_cast<T *>(hb[1]),        \
-                       static_cast<T *>(hb[2]), static_cast<T>(cb), static_cast<T>(dx), c)
+                       static_cast<T *>(hb[2]), static_cast<T>(cb), static_cast<T>(dx), c, zz)
     if (cp) RF_SHELL_H(true);
     else RF_SHELL_H(false);
 #undef RF_SHELL_H
@@ -600,17 +875,18 @@ int shell_h_t(const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6], void 
 
 template <typename T, typename TagT>
 int shell_e_tag(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void *const hb[3], void *const eb[3],
-                double dx, const rf_cpml_t *cp, const rf_mat_t *m, const rf_leap_src_t *srcs, int n_srcs,
-                const long long *d_step, hipStream_t st) {
+                double dx, const rf_cpml_t *cp, const rf_zpsi_t *z, const rf_mat_t *m, const rf_leap_src_t *srcs,
+                int n_srcs, const long long *d_step, hipStream_t st) {
     const long long n = sh->start[sh->n];
     const dim3 grd((unsigned)((n + 255) / 256));
     rf_cpml_t c = cp ? *cp : rf_cpml_t{};
+    rf_zpsi_t zz = z ? *z : rf_zpsi_t{};
 #define RF_SHELL_E(CP)                                                                                         \
     hipLaunchKernelGGL((k_shell_e<T, TagT, CP>), grd, dim3(256), 0, st, *g, *sh,                              \
                        static_cast<const T *>(ea[0]), static_cast<const T *>(ea[1]),                           \
                        static_cast<const T *>(ea[2]), static_cast<const T *>(hb[0]),                           \
                        static_cast<const T *>(hb[1]), static_cast<const T *>(hb[2]), static_cast<T *>(eb[0]),  \
-                       static_cast<T *>(eb[1]), static_cast<T *>(eb[2]), static_cast<T>(dx), c, *m, srcs,      \
+                       static_cast<T *>(eb[1]), static_cast<T *>(eb[2]), static_cast<T>(dx), c, zz, *m, srcs,  \
                        n_srcs, d_step)
     if (cp) RF_SHELL_E(true);
     else RF_SHELL_E(false);
@@ -620,13 +896,72 @@ int shell_e_tag(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], voi
 
 template <typename T>
 int shell_e_t(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void *const hb[3], void *const eb[3],
-              double dx, const rf_cpml_t *cp, const rf_mat_t *m, const rf_leap_src_t *srcs, int n_srcs,
-              const long long *d_step, hipStream_t st) {
-    if (!m->tag) return shell_e_tag<T, NoTag>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
+              double dx, const rf_cpml_t *cp, const rf_zpsi_t *z, const rf_mat_t *m, const rf_leap_src_t *srcs,
+              int n_srcs, const long long *d_step, hipStream_t st) {
+    if (!m->tag) return shell_e_tag<T, NoTag>(g, sh, ea, hb, eb, dx, cp, z, m, srcs, n_srcs, d_step, st);
     switch (m->tag_bytes) {
-    case 1: return shell_e_tag<T, uint8_t>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
-    case 2: return shell_e_tag<T, uint16_t>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
-    case 4: return shell_e_tag<T, uint32_t>(g, sh, ea, hb, eb, dx, cp, m, srcs, n_srcs, d_step, st);
+    case 1: return shell_e_tag<T, uint8_t>(g, sh, ea, hb, eb, dx, cp, z, m, srcs, n_srcs, d_step, st);
+    case 2: return shell_e_tag<T, uint16_t>(g, sh, ea, hb, eb, dx, cp, z, m, srcs, n_srcs, d_step, st);
+    case 4: return shell_e_tag<T, uint32_t>(g, sh, ea, hb, eb, dx, cp, z, m, srcs, n_srcs, d_step, st);
+    }
+    return -1;
+}
+
+template <typename T>
+size_t leap_lds(const rf_leap_t *a) {
+    const size_t nls = a->n_srcs <= MAX_LSRC ? (size_t)a->n_srcs : 0;
+    return nls * sizeof(LSrc) + (size_t)a->n_adec * sizeof(typename EntryOf<T>::ade) +
+           (size_t)a->ncache * coef_bytes<T>();
+}
+
+template <typename T, typename TagT, int R>
+int launch_tag(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
+    const size_t lds = leap_lds<T>(a);
+    if (a->zc)
+        hipLaunchKernelGGL((k_leapfrog<T, TagT, true, R>), dim3(blocks), dim3(WL * WPB), lds, st, *a);
+    else
+        hipLaunchKernelGGL((k_leapfrog<T, TagT, false, R>), dim3(blocks), dim3(WL * WPB), lds, st, *a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Wavefronts of the variant `a` selects that the device holds at once.
+template <typename T, typename TagT, int R>
+int capacity_tag(const rf_leap_t *a) {
+    int nb = 0, cus = 0, dev = 0;
+    const size_t lds = leap_lds<T>(a);
+    hipError_t e = a->zc ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, true, R>), WL * WPB, lds)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, false, R>), WL * WPB, lds);
+    if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nb <= 0)
+        return 0;
+    return nb * cus * WPB;
+}
+
+template <typename T, int R>
+int capacity_rows(const rf_leap_t *a) {
+    if (!a->tag) return capacity_tag<T, NoTag, R>(a);
+    switch (a->tag_bytes) {
+    case 1: return capacity_tag<T, uint8_t, R>(a);
+    case 2: return capacity_tag<T, uint16_t, R>(a);
+    case 4: return capacity_tag<T, uint32_t, R>(a);
+    }
+    return 0;
+}
+
+int leap_capacity(int prec, const rf_leap_t *a) {
+    if (prec == 0) return a->rows == 2 ? capacity_rows<double, 2>(a) : capacity_rows<double, 4>(a);
+    return a->rows == 4 ? capacity_rows<float, 4>(a) : capacity_rows<float, 8>(a);
+}
+
+template <typename T, int R>
+int launch_rows(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
+    if (!a->tag) return launch_tag<T, NoTag, R>(a, blocks, st);
+    switch (a->tag_bytes) {
+    case 1: return launch_tag<T, uint8_t, R>(a, blocks, st);
+    case 2: return launch_tag<T, uint16_t, R>(a, blocks, st);
+    case 4: return launch_tag<T, uint32_t, R>(a, blocks, st);
     }
     return -1;
 }
@@ -641,45 +976,73 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
     const long long planes = a->ib1[0] - a->ib0[0];
     const long long jn = a->ib1[1] - a->ib0[1];
     const long long kn = a->ib1[2] - a->ib0[2];
+    if (a->kbrk < a->ib0[2] || a->kbrk > a->ib1[2]) return -1;
     if (rows <= 0) rows = 4;
     if (prec == 0 ? !(rows == 2 || rows == 4) : !(rows == 4 || rows == 8)) return -1;
     a->rows = rows;
+    // up to 40 KiB of LDS per block for the ADE sets (at most 16) and the
+    // material table (plus the staged sources)
+    const size_t ade_sz = prec == 0 ? sizeof(rf_ade64_t) : sizeof(rf_ade32_t);
+    a->n_adec = a->tag ? (a->n_ade < 16 ? a->n_ade : 16) : 0;
+    const int cap = (int)((40 * 1024 - a->n_adec * ade_sz) / (prec == 0 ? coef_bytes<double>() : coef_bytes<float>()));
+    a->ncache = a->tag ? (a->n_entries < cap ? a->n_entries : cap) : 0;
     if (planes <= 0 || jn <= 0 || kn <= 0) {   /* empty box: nothing to launch */
-        a->ntk = a->ntj = a->n_waves = a->xcd_q = 0;
+        a->ntk = a->ntk0 = a->ntj = a->n_items = a->blocks = 0;
         a->chunk = 1;
         return 0;
     }
-    a->ntk = (int)((kn + KW - 1) / KW);
+    a->ntk0 = (a->kbrk - a->ib0[2] + KW - 1) / KW;
+    a->ntk = a->ntk0 + (a->ib1[2] - a->kbrk + KW - 1) / KW;
     a->ntj = (int)((jn + rows - 1) / rows);
     const long long tiles = (long long)a->ntk * a->ntj;
+    // resident wavefronts of the variant this launch selects
+    static long long cache_key[8];
+    static int cache_val[8], cache_n = 0;
+    const size_t lds = prec == 0 ? leap_lds<double>(a) : leap_lds<float>(a);
+    const long long key = ((long long)lds << 16) | (prec << 12) | (a->tag_bytes << 8) | (a->zc << 7) | (rows << 3);
+    int resident = 0;
+    for (int q = 0; q < cache_n; q++)
+        if (cache_key[q] == key) resident = cache_val[q];
+    if (!resident) {
+        resident = leap_capacity(prec, a);
+        if (resident <= 0) resident = 4096;
+        if (cache_n < 8) {
+            cache_key[cache_n] = key;
+            cache_val[cache_n++] = resident;
+        }
+    }
+    if (target_waves > 0) resident = target_waves;
     if (chunk <= 0) {
-        if (target_waves <= 0) target_waves = 4096;
-        long long chunks = (target_waves + tiles - 1) / tiles;
+        // about three items per resident wavefront: enough for the queue
+        // to even out unequal tiles, few enough that the warm-up plane of
+        // each item stays a small share
+        long long chunks = (3LL * resident + tiles - 1) / tiles;
         if (chunks < 1) chunks = 1;
         chunk = (int)((planes + chunks - 1) / chunks);
-        if (chunk < 4) chunk = 4;
+        if (chunk < 8) chunk = 8;
     }
     if (chunk > planes) chunk = (int)planes;
     a->chunk = chunk;
     const long long nch = (planes + chunk - 1) / chunk;
-    const long long waves = tiles * nch;
-    if (waves > (1LL << 30)) return -1;
-    a->n_waves = (int)waves;
-    const long long blocks = (waves + WPB - 1) / WPB;
-    a->xcd_q = (int)((blocks + N_XCD - 1) / N_XCD);
-    // up to 40 KiB of LDS per block for the material table
-    const int cap = (40 * 1024) / (prec == 0 ? coef_bytes<double>() : coef_bytes<float>());
-    a->ncache = a->tag ? (a->n_entries < cap ? a->n_entries : cap) : 0;
+    const long long items = tiles * nch;
+    if (items > (1LL << 30)) return -1;
+    a->n_items = (int)items;
+    long long blocks = (resident + WPB - 1) / WPB;
+    const long long need = (items + WPB - 1) / WPB;
+    if (blocks > need) blocks = need;
+    blocks = (blocks + N_XCD - 1) / N_XCD * N_XCD;
+    a->blocks = (int)blocks;
     return 0;
 }
 
 extern "C" int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s) {
-    if (a->n_waves == 0) return 0;
-    if (a->n_waves < 0 || a->ntk <= 0 || a->ntj <= 0 || a->chunk <= 0) return -1;
+    if (a->n_items == 0) return 0;
+    if (a->n_items < 0 || a->ntk <= 0 || a->ntj <= 0 || a->chunk <= 0 || a->blocks <= 0 || !a->ctr) return -1;
     if (a->n_srcs > 0 && !a->srcs) return -1;
     if (a->tag && !a->cold) return -1;
+    if (a->zc && (!a->zpsi || a->zlen <= 0)) return -1;
     hipStream_t st = rf_raw_stream(s);
-    const unsigned blocks = static_cast<unsigned>(a->xcd_q) * N_XCD;
+    const unsigned blocks = static_cast<unsigned>(a->blocks);
     if (prec == 0) {
         switch (a->rows) {
         case 2: return launch_rows<double, 2>(a, blocks, st);
@@ -695,22 +1058,25 @@ extern "C" int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s) 
 }
 
 extern "C" int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6],
-                                 void *const hb[3], double cb, double dx, const rf_cpml_t *cpml, rf_stream_t *s) {
+                                 void *const hb[3], double cb, double dx, const rf_cpml_t *cpml,
+                                 const rf_zpsi_t *z, rf_stream_t *s) {
     if (sh->n < 0 || sh->n > 6) return -1;
     if (sh->n == 0 || sh->start[sh->n] == 0) return 0;
+    if (z && z->zpsi && !cpml) return -1;
     hipStream_t st = rf_raw_stream(s);
-    return prec == 0 ? shell_h_t<double>(g, sh, fa, hb, cb, dx, cpml, st)
-                     : shell_h_t<float>(g, sh, fa, hb, cb, dx, cpml, st);
+    return prec == 0 ? shell_h_t<double>(g, sh, fa, hb, cb, dx, cpml, z, st)
+                     : shell_h_t<float>(g, sh, fa, hb, cb, dx, cpml, z, st);
 }
 
 extern "C" int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3],
                                  void *const hb[3], void *const eb[3], double dx, const rf_cpml_t *cpml,
-                                 const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs,
+                                 const rf_zpsi_t *z, const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs,
                                  const long long *d_step, rf_stream_t *s) {
     if (sh->n < 0 || sh->n > 6) return -1;
     if (sh->n == 0 || sh->start[sh->n] == 0) return 0;
     if (n_srcs > 0 && !srcs) return -1;
+    if (z && z->zpsi && !cpml) return -1;
     hipStream_t st = rf_raw_stream(s);
-    return prec == 0 ? shell_e_t<double>(g, sh, ea, hb, eb, dx, cpml, mat, srcs, n_srcs, d_step, st)
-                     : shell_e_t<float>(g, sh, ea, hb, eb, dx, cpml, mat, srcs, n_srcs, d_step, st);
+    return prec == 0 ? shell_e_t<double>(g, sh, ea, hb, eb, dx, cpml, z, mat, srcs, n_srcs, d_step, st)
+                     : shell_e_t<float>(g, sh, ea, hb, eb, dx, cpml, z, mat, srcs, n_srcs, d_step, st);
 }

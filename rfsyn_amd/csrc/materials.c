@@ -657,7 +657,12 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
         d->d.n_slots = slots;
         size_t cells = (size_t)d->d.bn[0] * d->d.bn[1] * d->d.bn[2];
         size_t es = prec ? sizeof(float) : sizeof(double);
-        size_t sb = (size_t)(slots ? slots : 1) * 3 * cells * es;
+        /* 64-element aligned arrays plus an odd number of 64-element
+         * blocks, so the 3 * slots state streams of a node do not all
+         * start at one offset modulo a large power of two */
+        size_t stride = (cells + 63) / 64 * 64 + 64 * 37;
+        d->d.ade_stride = (long long)stride;
+        size_t sb = (size_t)(slots ? slots : 1) * 3 * stride * es;
         if (dev_upload(d, NULL, sb, &d->d.J) || dev_upload(d, NULL, sb, &d->d.P)) goto out;
     }
 
@@ -708,6 +713,7 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
         }
         d->d.entries = dp;
         size_t na = m->count ? m->count : 1;
+        d->d.n_ade = (int)na;
         if (prec == 0) {
             rf_ade64_t *a = calloc(na, sizeof *a);
             if (!a) goto out;

@@ -110,10 +110,14 @@ typedef struct {
     int tag_bytes;              /* 1, 2 or 4 */
     const void *entries;        /* rf_entry64_t or rf_entry32_t */
     const void *ade;            /* rf_ade64_t or rf_ade32_t */
-    /* ADE state box [b0, b0+bn) per axis; state arrays [slot][c][cell] */
+    /* ADE state box [b0, b0+bn) per axis; state arrays [slot][c][cell],
+     * one (slot, c) array every ade_stride elements (>= cells, padded so
+     * the arrays do not start at the same offset modulo large powers of 2) */
     int b0[3], bn[3];
     int n_slots;
-    void *J, *P;                /* n_slots * 3 * cells each */
+    int n_ade;                  /* ADE coefficient sets in `ade` */
+    long long ade_stride;
+    void *J, *P;                /* n_slots * 3 * ade_stride each */
     double vac_ca;              /* vacuum Cb = dt/(eps0 dx) when tag == NULL */
     int n_entries;              /* entries in the table */
 } rf_mat_t;
@@ -173,6 +177,7 @@ typedef struct {
     const void *entries, *ade;
     void *J, *P;
     int b0[3], bn[3];
+    long long ade_stride;
 } rf_leap_cold_t;
 
 /* Fused leapfrog step: one kernel computes H^{n+1/2} and then E^{n+1}
@@ -192,25 +197,43 @@ typedef struct {
     void *eb[3], *hb[3];
     double cb, dx;
     int ib0[3], ib1[3];         /* owned node box */
+    int kbrk;                   /* z tiles restart here (ib0[2] <= kbrk <= ib1[2]) */
     const void *tag;            /* material tags, NULL = vacuum */
     int tag_bytes, n_entries;
     const void *entries;
+    const void *ade;            /* ADE coefficient sets (rf_ade32_t / rf_ade64_t) */
+    int n_ade, n_slots;         /* sets; poles of the state arrays */
     double vac_ca;
     const rf_leap_cold_t *cold; /* device */
     const rf_leap_src_t *srcs;  /* device table */
     int n_srcs;
     int src_box[6];
+    int src_pre, src_post;      /* component bits with sources before / after the E update */
     const long long *d_step;
+    /* CPML along z inside the box: plain range, psi slab [0,zlo) u [zhi,nz],
+     * the z profiles of rf_cpml_t (kinv_h b_h a_h kinv_m b_m a_m) and the
+     * z-psi store: 6 values per (i, j, compressed k) node, H of hx and hy
+     * in slots 2*zpar (read) and 2-2*zpar (written), then E of ex and ey */
+    int zc;
+    int zplo, zphi, zlo, zhi, zlen;
+    const void *zk[6];
+    void *zpsi;
+    int zpar;
+    /* work queue: 2 x 8 zero-initialised device counters, the set
+     * qpar used by this launch (alternate it from launch to launch) */
+    unsigned *ctr;
+    int qpar;
     /* tiling, filled by rf_leap_plan */
-    int rows, chunk;            /* rows per wavefront, x planes per wavefront */
-    int ntk, ntj;               /* tiles along z and y */
-    int n_waves, xcd_q;         /* wavefronts; blocks per XCD */
-    int ncache;                 /* material entries staged in LDS per block */
+    int rows, chunk;            /* rows per item, x planes per item */
+    int ntk, ntk0, ntj;         /* tiles along z (ntk0 below kbrk) and y */
+    int n_items, blocks;        /* work items; blocks launched (resident) */
+    int ncache, n_adec;         /* material entries and ADE sets staged in LDS */
 } rf_leap_t;
 
 /* Tiling of the fused step over a->ib0..ib1; rows: 2 or 4 (FP64), 4 or 8
- * (FP32), 0 = default; chunk 0 = enough x chunks for about target_waves
- * wavefronts.  0 on success, -1 for an unsupported request. */
+ * (FP32), 0 = default; chunk 0 = about three work items per resident
+ * wavefront (target_waves > 0 overrides the residency the device
+ * reports).  0 on success, -1 for an unsupported request. */
 int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves);
 /* tag NULL = vacuum (vac_ca); an empty box launches nothing. */
 int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s);
@@ -221,18 +244,31 @@ int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s);
 typedef struct {
     int n;
     int lo[6][3], hi[6][3];
-    long long start[7];
+    long long start[7];         /* each box holds fewer than 2^32 nodes */
+    /* division by the box's k and j extents: q = (t + ((n - t) >> s1)) >> s2
+     * with t = mulhi(m, n) (round-up multiply-high method) */
+    unsigned mk[6], mj[6];
+    unsigned sk[6][2], sj[6][2];
 } rf_shell_t;
 
+/* z-psi store of the fused step (see rf_leap_t): NULL = the z CPML terms
+ * use the psi arrays of rf_cpml_t. */
+typedef struct {
+    void *zpsi;
+    int zpar, zlen;
+} rf_zpsi_t;
+
 /* Shell H: H^{n+1/2} of the shell nodes from E^n and H^{n-1/2} of the A
- * buffers (fa: ex ey ez hx hy hz) into hb, psi_h in place. */
+ * buffers (fa: ex ey ez hx hy hz) into hb, psi_h in place (z terms in the
+ * z-psi store when given). */
 int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6], void *const hb[3],
-                      double cb, double dx, const rf_cpml_t *cpml, rf_stream_t *s);
+                      double cb, double dx, const rf_cpml_t *cpml, const rf_zpsi_t *z, rf_stream_t *s);
 /* Shell E: sources and E^{n+1} of the shell nodes from E^n (ea) and
  * H^{n+1/2} (hb) into eb, psi_e and the ADE state in place. */
 int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void *const hb[3],
-                      void *const eb[3], double dx, const rf_cpml_t *cpml, const rf_mat_t *mat,
-                      const rf_leap_src_t *srcs, int n_srcs, const long long *d_step, rf_stream_t *s);
+                      void *const eb[3], double dx, const rf_cpml_t *cpml, const rf_zpsi_t *z,
+                      const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
+                      rf_stream_t *s);
 
 /* ----------------------------------------------------------- launches
  * `prec` selects the field precision: 0 = double, 1 = float.  All

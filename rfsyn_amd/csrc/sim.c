@@ -110,6 +110,11 @@ struct fdtd_sim {
     rf_leap_src_t *d_srcs;   /* device source table of the fused step */
     rf_leap_cold_t *d_cold;  /* its cold data (ADE) */
     int ib0[3], ib1[3];      /* node box of the fused kernel */
+    int kbrk;                /* its z tiles restart at the upper z slab */
+    int zc, zpar;            /* z CPML inside the box; z-psi buffer parity */
+    void *zpsi;              /* z-psi store of the box (and shell) nodes */
+    unsigned *d_ctr;         /* work-queue counters of the fused kernel */
+    int src_pre, src_post;   /* sourced components before / after E */
     rf_shell_t shell;        /* the nodes around it (CPML slabs + margin) */
     int n_srcs, src_box[6];
     int post_in_kernel;      /* post-E plane sources inside the fused kernel */
@@ -216,6 +221,8 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     rf_free(s->d_pts);
     rf_free(s->d_srcs);
     rf_free(s->d_cold);
+    rf_free(s->zpsi);
+    rf_free(s->d_ctr);
     free(s->planes);
     free(s->probes);
     free(s->freqs_all);
@@ -358,18 +365,33 @@ static int want_fused(const fdtd_sim_t *s) {
     return !(env && env[0] == '0');
 }
 
+/* Multiply-high division by d >= 1 for 32-bit n (round-up method of
+ * Granlund and Montgomery, "Division by invariant integers using
+ * multiplication", 1994): l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1,
+ * n / d = (t + ((n - t) >> min(l,1))) >> max(l-1,0), t = mulhi(m, n). */
+static void magic_u32(unsigned d, unsigned *m, unsigned s[2]) {
+    unsigned l = 0;
+    while ((1ull << l) < d) l++;
+    *m = (unsigned)((((1ull << l) - d) << 32) / d + 1);
+    s[0] = l < 1 ? l : 1;
+    s[1] = l > 0 ? l - 1 : 0;
+}
+
 /* The fused kernel owns the nodes whose coordinate on every axis lies in
  * the CPML plain range with a one-node margin below it (so the halo it
  * recomputes is plain too); the rest (CPML slabs and margin) form up to 6
  * shell boxes updated node by node. */
-static void plan_regions(fdtd_sim_t *s) {
+static int plan_regions(fdtd_sim_t *s) {
     const size_t n[3] = {s->nx, s->ny, s->nz};
     int top[3];
+    /* z CPML is evaluated inside the fused kernel (per lane) when z is a
+     * non-periodic axis with a psi slab; x and y slabs go to the shell */
+    s->zc = s->has_cpml && !((s->per >> 2) & 1) && s->cp.d.len[2] > 0;
     for (int a = 0; a < 3; a++) {
         top[a] = ((s->per >> a) & 1) ? (int)n[a] : (int)n[a] + 1;
         s->ib0[a] = 0;
         s->ib1[a] = top[a];
-        if (s->has_cpml && !((s->per >> a) & 1)) {
+        if (s->has_cpml && !((s->per >> a) & 1) && !(a == 2 && s->zc)) {
             const int plo = s->cp.d.plo[a], phi = s->cp.d.phi[a];
             s->ib0[a] = plo == 0 ? 0 : plo + 1;
             s->ib1[a] = phi < top[a] ? phi : top[a];
@@ -378,6 +400,11 @@ static void plan_regions(fdtd_sim_t *s) {
     }
     rf_shell_t *sh = &s->shell;
     memset(sh, 0, sizeof *sh);
+    s->kbrk = s->ib1[2];
+    if (s->zc) {
+        int hz = s->cp.d.hi[2];
+        s->kbrk = hz < s->ib0[2] ? s->ib0[2] : (hz > s->ib1[2] ? s->ib1[2] : hz);
+    }
     const int *b0 = s->ib0, *b1 = s->ib1;
     int box[6][2][3] = {
         {{0, 0, 0}, {b0[0], top[1], top[2]}},
@@ -392,15 +419,19 @@ static void plan_regions(fdtd_sim_t *s) {
         long long cnt = 1;
         for (int a = 0; a < 3; a++) cnt *= box[q][1][a] > box[q][0][a] ? box[q][1][a] - box[q][0][a] : 0;
         if (!cnt) continue;
+        if (cnt >= (1LL << 32)) return -1;
         for (int a = 0; a < 3; a++) {
             sh->lo[sh->n][a] = box[q][0][a];
             sh->hi[sh->n][a] = box[q][1][a];
         }
         sh->start[sh->n] = at;
+        magic_u32((unsigned)(box[q][1][2] - box[q][0][2]), &sh->mk[sh->n], sh->sk[sh->n]);
+        magic_u32((unsigned)(box[q][1][1] - box[q][0][1]), &sh->mj[sh->n], sh->sj[sh->n]);
         at += cnt;
         sh->n++;
     }
     sh->start[sh->n] = at;
+    return 0;
 }
 
 /* Device arrays for probes and monitors, allocated once at the first run. */
@@ -427,11 +458,18 @@ static int finalize(fdtd_sim_t *s) {
                 cold.b0[a] = m->b0[a];
                 cold.bn[a] = m->bn[a];
             }
+            cold.ade_stride = m->ade_stride;
         }
         if (dalloc(s, (void **)&s->d_cold, sizeof cold) || rf_h2d(s->d_cold, &cold, sizeof cold, s->st) ||
             rf_sync(s->st))
             return -1;
-        plan_regions(s);
+        if (plan_regions(s)) {
+            rf_set_error("a shell box of the fused step holds 2^32 nodes or more");
+            return -1;
+        }
+        if (s->zc && dalloc(s, &s->zpsi, (s->nx + 1) * (s->ny + 1) * (size_t)s->cp.d.len[2] * 6 * s->es))
+            return -1;
+        if (dalloc(s, (void **)&s->d_ctr, 16 * sizeof(unsigned))) return -1;
         const char *e;
         s->leap_rows = (e = getenv("RFSYN_LEAP_ROWS")) ? atoi(e) : 0;
         s->leap_chunk = (e = getenv("RFSYN_LEAP_CHUNK")) ? atoi(e) : 0;
@@ -547,6 +585,11 @@ static int build_src_table(fdtd_sim_t *s) {
                 if (hi > box[2 * a + 1]) box[2 * a + 1] = hi;
             }
         }
+    s->src_pre = s->src_post = 0;
+    for (int q = 0; q < m; q++) {
+        if (t[q].after) s->src_post |= 1 << t[q].comp;
+        else s->src_pre |= 1 << t[q].comp;
+    }
     int rc = 0;
     if (m && !s->d_srcs) rc = dalloc(s, (void **)&s->d_srcs, n * sizeof *t);
     if (!rc && m) rc = rf_h2d(s->d_srcs, t, (size_t)m * sizeof *t, s->st) || rf_sync(s->st);
@@ -636,13 +679,38 @@ static int enqueue_fused(fdtd_sim_t *s) {
         a.ib0[q] = s->ib0[q];
         a.ib1[q] = s->ib1[q];
     }
+    a.kbrk = s->kbrk;
+    if (s->zc) {
+        const rf_cpml_t *c = &s->cp.d;
+        a.zc = 1;
+        a.zplo = c->plo[2];
+        a.zphi = c->phi[2];
+        a.zlo = c->lo[2];
+        a.zhi = c->hi[2];
+        a.zlen = c->len[2];
+        a.zk[0] = c->kinv_h[2];
+        a.zk[1] = c->b_h[2];
+        a.zk[2] = c->a_h[2];
+        a.zk[3] = c->kinv_m[2];
+        a.zk[4] = c->b_m[2];
+        a.zk[5] = c->a_m[2];
+        a.zpsi = s->zpsi;
+        a.zpar = s->zpar;
+    }
     const rf_mat_t *m = s->has_mat ? &s->mat.d : &s->vac;
     a.tag = m->tag;
     a.tag_bytes = m->tag_bytes;
     a.n_entries = m->n_entries;
     a.entries = m->entries;
+    a.ade = m->ade;
+    a.n_ade = m->n_ade;
+    a.n_slots = m->n_slots;
+    a.src_pre = s->src_pre;
+    a.src_post = s->src_post;
     a.vac_ca = m->vac_ca;
     a.cold = s->d_cold;
+    a.ctr = s->d_ctr;
+    a.qpar = s->parity;
     a.srcs = s->d_srcs;
     a.n_srcs = s->n_srcs;
     memcpy(a.src_box, s->src_box, sizeof a.src_box);
@@ -654,8 +722,10 @@ static int enqueue_fused(fdtd_sim_t *s) {
     if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
     if (s->shell.n) {
         const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
-        if (rf_launch_shell_h(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, a.cb, s->dx, cp, s->st) ||
-            rf_launch_shell_e(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, s->f2, s->dx, cp, m, s->d_srcs,
+        rf_zpsi_t z = {s->zpsi, s->zpar, s->zc ? s->cp.d.len[2] : 0};
+        const rf_zpsi_t *zp = s->zc ? &z : NULL;
+        if (rf_launch_shell_h(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, a.cb, s->dx, cp, zp, s->st) ||
+            rf_launch_shell_e(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, s->f2, s->dx, cp, zp, m, s->d_srcs,
                               s->n_srcs, s->d_step, s->st))
             return -1;
     }
@@ -667,6 +737,7 @@ static int enqueue_fused(fdtd_sim_t *s) {
         s->f2[q] = t;
     }
     s->parity ^= 1;
+    s->zpar ^= 1;
     return 0;
 }
 
