@@ -1,20 +1,24 @@
 """Benchmark of the rfsyn FDTD core on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
 
 One step is one full Yee step (H update, sources, E update with CPML,
 materials and dispersive ADE, probes and plane monitors) over the whole
-grid.  W untimed steps run first; then K steps are timed with the wall
-clock around fdtd_sim_run (which synchronises its stream), with the H and
-E kernels each bracketed by HIP events on the simulation stream.  Inputs
-are resident in HBM before the timed region starts.  The run then
-continues to the workload's full step count in both precisions so the
-FP32 probe DFTs can be compared with the FP64 ones.  Prints one JSON line.
+grid.  The default workload is the 512x512x256 periodic dispersive stack
+(c3).  W untimed steps run first; then K steps are timed with the wall
+clock around fdtd_sim_run (which synchronises its stream), with the step's
+kernels bracketed by HIP events on the simulation stream.  Inputs are
+resident in HBM before the timed region starts.  The run then continues to
+the workload's full step count in both precisions so the FP32 probe DFTs
+can be compared with the FP64 ones.  Prints one JSON line.
 
-Roofline: the dominant kernel's algorithmic bytes per launch (every field
-value it must read or write once, psi memories only in CPML slab cells,
-ADE state only in dispersive cells, one material tag per node) divided by
-its average event-timed duration, against the 8 TB/s HBM3E peak.
+Roofline: the dominant kernel's algorithmic bytes per launch divided by its
+average event-timed duration, against the 8 TB/s HBM3E peak.  The bytes
+follow the two-sweep accounting of a Yee step (18 values per cell, a
+material tag, 8 psi values per cell and CPML axis, 6 per Drude cell, 9 per
+Lorentz pole and cell, 6 per tensor cell) over the cells the launch
+updates; "step" gives the same accounting for the whole grid against the
+wall time per step.
 
 CPU baseline: c1 runs the rfsyn reference compiled from its own sources
 (oracle/_ref, kind "reference") on one pinned core over the same
@@ -68,12 +72,40 @@ def kernel_bytes(n, s, npml_axes, tag_bytes, ade_values=0):
     return h, e
 
 
-def leapfrog_bytes(box_nodes, s, tag_bytes, ade_values=0):
-    """Algorithmic HBM bytes of one fused leapfrog launch (H, sources and E
-    of one step on the CPML-free box): read the six components of state n
-    and write the six of state n+1 (12 values per node), one material tag
-    per node and the ADE state values (read and written)."""
-    return s * (12 * box_nodes + ade_values) + tag_bytes * box_nodes
+def box_cells(box, n):
+    """Cells (coordinate < n on every axis) inside a node box [(lo, hi)] * 3."""
+    out = 1
+    for a, (lo, hi) in enumerate(box):
+        out *= max(0, min(hi, n[a]) - lo)
+    return out
+
+
+def box_slab_cells(box, n, npml_axes):
+    """Sum over axes of the cells of the box whose coordinate along that
+    axis lies in its CPML slab (depth d on both faces; 0 = none)."""
+    out = 0
+    for a in range(3):
+        d = npml_axes[a]
+        if not d:
+            continue
+        lo, hi = box[a][0], min(box[a][1], n[a])
+        inside = max(0, min(hi, d) - lo) + max(0, hi - max(lo, n[a] - d))
+        others = 1
+        for b in range(3):
+            if b != a:
+                others *= max(0, min(box[b][1], n[b]) - box[b][0])
+        out += inside * others
+    return out
+
+
+def step_bytes(cells, slab, s, tag_bytes, drude=0, lorentz_pole_cells=0, tensor=0):
+    """Algorithmic bytes of one Yee step over `cells` cells in the two-sweep
+    accounting: 18 values per cell (each component read and written in its
+    own sweep and read once as curl source in the other), a material tag,
+    8 psi values per cell and CPML axis, 6 per Drude cell (J read and
+    written, 3 components), 9 per Lorentz pole and cell, 6 per tensor cell."""
+    return (s * (18 * cells + 8 * slab + 6 * drude + 9 * lorentz_pole_cells + 6 * tensor)
+            + tag_bytes * cells)
 
 
 # ------------------------------------------------------------ workloads
@@ -157,6 +189,7 @@ class C3:
         # Drude: J per component read+write (6 values); 2 Lorentz poles:
         # J and P per pole per component read+write (24 values)
         self.ade_values = 6 * drude_cells + 24 * lorentz_cells
+        self.drude_cells, self.lorentz_pole_cells = drude_cells, 2 * lorentz_cells
         self.config = {"workload": "c3: 512x512x256 periodic Drude/Lorentz stack, 16-cell z CPML, "
                                    "2 plane monitors x 128 freqs, 6000 steps",
                        "grid": list(self.n), "dx_m": 0.5e-3, "courant": 0.99, "layers": lay}
@@ -244,7 +277,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")
@@ -288,19 +321,25 @@ def main():
     r = res[wl.headline]
     value = cells * args.steps / r["wall"] / 1e6
     s = 8 if wl.headline == capi.FDTD_FP64 else 4
+    drude = getattr(wl, "drude_cells", 0)
+    lorentz = getattr(wl, "lorentz_pole_cells", 0)
     if r["fused"]:
-        # the fused launch on the CPML-free box (H slot of the events); the
-        # shell kernels around it are timed in the E slot
-        box_nodes = int(np.prod([hi - lo for lo, hi in r["box"]]))
-        byt = leapfrog_bytes(box_nodes, s, wl.tag_bytes, wl.ade_values)
+        # the fused launch on its box (H slot of the events); the shell
+        # kernels around it (x/y CPML slabs) are timed in the E slot.  Its
+        # algorithmic bytes: the two-sweep accounting of one step over the
+        # cells of the box (dispersive cells all lie inside it)
+        box = r["box"]
+        byt = step_bytes(box_cells(box, n), box_slab_cells(box, n, wl.npml), s, wl.tag_bytes, drude, lorentz)
         dom, dur = "leapfrog", r["h_ms"]
-        detail = {"box": r["box"], "box_nodes": box_nodes, "shell_avg_launch_us": round(r["e_ms"] * 1e3, 3)}
+        detail = {"box": box, "box_cells": box_cells(box, n), "shell_avg_launch_us": round(r["e_ms"] * 1e3, 3)}
     else:
         hb, eb = kernel_bytes(n, s, wl.npml, wl.tag_bytes, wl.ade_values)
         dom, dur, byt = ("update_e", r["e_ms"], eb) if r["e_ms"] >= r["h_ms"] else ("update_h", r["h_ms"], hb)
         detail = {"h_bytes_per_launch": hb, "h_avg_launch_us": round(r["h_ms"] * 1e3, 3),
                   "e_bytes_per_launch": eb, "e_avg_launch_us": round(r["e_ms"] * 1e3, 3)}
     achieved = byt / (dur * 1e-3) / 1e9
+    full = step_bytes(cells, slab_cells(n, wl.npml), s, wl.tag_bytes, drude, lorentz)
+    step_gbs = full / (r["wall"] / args.steps) / 1e9
     d64, d32 = res[capi.FDTD_FP64]["dft"], res[capi.FDTD_FP32]["dft"]
     peak = np.max(np.abs(d64), axis=1, keepdims=True)
     err = float(np.max(np.abs(d32 - d64) / peak))
@@ -322,6 +361,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": byt, "avg_launch_us": round(dur * 1e3, 3), **detail},
+        "step": {"bytes_per_step": full, "achieved_gbs": round(step_gbs, 2),
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "other_precision": {"dtype": "f32" if wl.headline == capi.FDTD_FP64 else "f64",
                             "value": round(cells * args.steps / other["wall"] / 1e6, 3)},
         "fp32_vs_fp64_probe_dft_err": err,

@@ -70,6 +70,42 @@ __device__ __forceinline__ double from_lower(double v) {
     return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
+// Buffer loads and stores: one resource per state block (the six field
+// arrays of a state, contiguous) with 32-bit byte offsets, so a lane keeps
+// its column offset in one register and the row, plane and component
+// offsets are scalar; no 64-bit address arithmetic per access.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, static_cast<int>(bytes), 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T bld(rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ float bld<float>(rsrc_t r, unsigned vo, unsigned so) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(vo), static_cast<int>(so), 0));
+}
+template <> __device__ __forceinline__ double bld<double>(rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(vo),
+                                                                           static_cast<int>(so), 0));
+}
+__device__ __forceinline__ void bst(float v, rsrc_t r, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, static_cast<int>(vo), static_cast<int>(so), 0);
+}
+__device__ __forceinline__ void bst(double v, rsrc_t r, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, static_cast<int>(vo),
+                                          static_cast<int>(so), 0);
+}
+template <typename TagT> __device__ __forceinline__ unsigned btag(rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ unsigned btag<uint8_t>(rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, static_cast<int>(vo), static_cast<int>(so), 0);
+}
+template <> __device__ __forceinline__ unsigned btag<uint16_t>(rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(vo), static_cast<int>(so), 0);
+}
+template <> __device__ __forceinline__ unsigned btag<uint32_t>(rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(vo), static_cast<int>(so), 0);
+}
+
 // The cold data of the step, read through a pointer the compiler cannot
 // see through, so its loads stay inside the slab / flagged-edge branches
 // that use them instead of being hoisted into registers for the whole loop.
@@ -408,18 +444,24 @@ k_leapfrog(rf_leap_t a) {
     }
     T *zpsi = static_cast<T *>(a.zpsi);
     const int zr = 2 * a.zpar, zw = 2 - 2 * a.zpar;   // H psi slots read / written
+    const rsrc_t RZ = make_rsrc(a.zpsi, ZC ? static_cast<unsigned>(a.zbytes) : 0u);
     // sources: does their bounding box meet this tile's rows and columns;
     // the components they touch before and after the E update
     const bool src_tile = a.n_srcs > 0 && a.src_box[2] < j0 + R && a.src_box[3] >= j0 &&
                           a.src_box[4] < k0 + KW && a.src_box[5] >= k0;
     const bool src_lane = kc >= a.src_box[4] && kc <= a.src_box[5];
 
-    const T *EA0 = static_cast<const T *>(a.ea[0]), *EA1 = static_cast<const T *>(a.ea[1]),
-            *EA2 = static_cast<const T *>(a.ea[2]);
-    const T *HA0 = static_cast<const T *>(a.ha[0]), *HA1 = static_cast<const T *>(a.ha[1]),
-            *HA2 = static_cast<const T *>(a.ha[2]);
-    T *EB0 = static_cast<T *>(a.eb[0]), *EB1 = static_cast<T *>(a.eb[1]), *EB2 = static_cast<T *>(a.eb[2]);
-    T *HB0 = static_cast<T *>(a.hb[0]), *HB1 = static_cast<T *>(a.hb[1]), *HB2 = static_cast<T *>(a.hb[2]);
+    // state blocks: component q at byte q * fsb (ex ey ez hx hy hz)
+    constexpr unsigned ES = sizeof(T);
+    const unsigned fsb = static_cast<unsigned>(a.fstride) * ES;
+    const rsrc_t RA = make_rsrc(a.blk_a, 6 * fsb), RB = make_rsrc(a.blk_b, 6 * fsb);
+    // tags (vacuum: an empty resource that is never read)
+    const rsrc_t RT = make_rsrc(a.tag, TAGS ? static_cast<unsigned>(a.fstride) * sizeof(TagT) : 0u);
+    const unsigned vk = static_cast<unsigned>(kc) * ES;          // this lane's column
+    const unsigned sxb = static_cast<unsigned>(g.sx) * ES, syb = static_cast<unsigned>(g.sy) * ES;
+    unsigned rowb[R + 2];                                          // row offsets
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) rowb[r] = static_cast<unsigned>(jr[r]) * syb;
     const T cb = static_cast<T>(a.cb), dx = static_cast<T>(a.dx);
     const T vac = static_cast<T>(a.vac_ca);
 
@@ -433,13 +475,12 @@ k_leapfrog(rf_leap_t a) {
     T nx[R + 2], ny[R + 2], nz[R + 2];   // E^n, plane i+1
     T py[R + 1], pz[R + 1];              // H^{n+1/2} hy, hz of plane i-1
     {
-        const long long base = (long long)i * g.sx;
+        const unsigned pb = static_cast<unsigned>(i) * sxb;
 #pragma unroll
         for (int r = 0; r < R + 2; r++) {
-            const int o = jr[r] * sy + kc;
-            cx[r] = EA0[base + o];
-            cy[r] = r <= R ? EA1[base + o] : T(0);
-            cz[r] = EA2[base + o];
+            cx[r] = bld<T>(RA, vk, pb + rowb[r]);
+            cy[r] = r <= R ? bld<T>(RA, vk, fsb + pb + rowb[r]) : T(0);
+            cz[r] = bld<T>(RA, vk, 2 * fsb + pb + rowb[r]);
         }
     }
 #pragma unroll
@@ -453,39 +494,44 @@ k_leapfrog(rf_leap_t a) {
         } else if (ip > g.nx) {
             ip = g.nx;   // plane nx updates only hx, which needs no i+1
         }
-        const long long base = (long long)i * g.sx;
-        const long long nbase = (long long)ip * g.sx;
+        const unsigned pb = static_cast<unsigned>(i) * sxb, npb = static_cast<unsigned>(ip) * sxb;
         // row j0+R feeds only hx, hz of row j0+R-1 (ez, ex at j+1)
 #pragma unroll
         for (int r = 0; r < R + 2; r++) {
-            const int o = jr[r] * sy + kc;
-            nx[r] = EA0[nbase + o];
-            ny[r] = r <= R ? EA1[nbase + o] : T(0);
-            nz[r] = EA2[nbase + o];
+            nx[r] = bld<T>(RA, vk, npb + rowb[r]);
+            ny[r] = r <= R ? bld<T>(RA, vk, fsb + npb + rowb[r]) : T(0);
+            nz[r] = bld<T>(RA, vk, 2 * fsb + npb + rowb[r]);
         }
         T hx[R + 1], hy[R + 1], hz[R + 1];
 #pragma unroll
         for (int r = 0; r <= R; r++) {
-            const int o = jr[r] * sy + kc;
-            hx[r] = HA0[base + o];
-            hy[r] = r > 0 ? HA1[base + o] : T(0);   // the halo row needs hx, hz only
-            hz[r] = HA2[base + o];
+            hx[r] = bld<T>(RA, vk, 3 * fsb + pb + rowb[r]);
+            hy[r] = r > 0 ? bld<T>(RA, vk, 4 * fsb + pb + rowb[r]) : T(0);   // the halo row needs hx, hz only
+            hz[r] = bld<T>(RA, vk, 5 * fsb + pb + rowb[r]);
         }
         // z-psi of the slab lanes, issued with the field loads: H of hx and
         // hy (read slots) on rows 0..R, E of ex and ey on rows 1..R
+        // (wavefront-uniform test, loads from every lane at a clamped
+        // node, zero outside the slab: no divergent branch around loads)
         T zh0[R + 1], zh1[R + 1], ze0[R + 1], ze1[R + 1];
 #pragma unroll
         for (int r = 0; r <= R; r++) zh0[r] = zh1[r] = ze0[r] = ze1[r] = T(0);
         if constexpr (ZC) {
-            if (zs && zp.cq >= 0) {
+            if (zs) {
+                const bool slab = zp.cq >= 0;
+                const unsigned zl = static_cast<unsigned>(slab ? zp.cq : 0) * 6 * ES;
 #pragma unroll
                 for (int r = 0; r <= R; r++) {
-                    const long long zo = (((long long)i * (g.ny + 1) + jr[r]) * a.zlen + zp.cq) * 6;
-                    zh0[r] = zpsi[zo + zr];
-                    zh1[r] = zpsi[zo + zr + 1];
-                    if (r >= 1 && !first) {
-                        ze0[r] = zpsi[zo + 4];
-                        ze1[r] = zpsi[zo + 5];
+                    const unsigned zb = ((static_cast<unsigned>(i) * static_cast<unsigned>(g.ny + 1) +
+                                          static_cast<unsigned>(jr[r])) *
+                                         static_cast<unsigned>(a.zlen)) * 6 * ES;
+                    const T h0 = bld<T>(RZ, zl, zb + zr * ES), h1 = bld<T>(RZ, zl, zb + (zr + 1) * ES);
+                    zh0[r] = slab ? h0 : T(0);
+                    zh1[r] = slab ? h1 : T(0);
+                    if (r >= 1) {
+                        const T e0 = bld<T>(RZ, zl, zb + 4 * ES), e1 = bld<T>(RZ, zl, zb + 5 * ES);
+                        ze0[r] = slab ? e0 : T(0);
+                        ze1[r] = slab ? e1 : T(0);
                     }
                 }
             }
@@ -496,7 +542,11 @@ k_leapfrog(rf_leap_t a) {
         for (int r = 0; r <= R; r++) {
             tg[r] = 0;
             if constexpr (TAGS)
-                if (r >= 1) tg[r] = load_tag<TagT>(a.tag, base + jr[r] * sy + kc);
+                if (r >= 1)
+                    tg[r] = btag<TagT>(RT, static_cast<unsigned>(kc) * sizeof(TagT),
+                                       (static_cast<unsigned>(i) * static_cast<unsigned>(g.sx) +
+                                        static_cast<unsigned>(jr[r]) * static_cast<unsigned>(sy)) *
+                                           sizeof(TagT));
         }
 
         // ---- H^{n+1/2}(i), rows j0-1 .. j0+R-1 (grid.c:70-89, cpml.c:185-215)
@@ -535,10 +585,9 @@ k_leapfrog(rf_leap_t a) {
             hy[r] = do_y ? nhy : hy[r];
             hz[r] = do_z ? nhz : hz[r];
             if (own) {
-                const int o = jc * sy + kc;
-                HB0[base + o] = hx[r];
-                HB1[base + o] = hy[r];
-                HB2[base + o] = hz[r];
+                bst(hx[r], RB, vk, 3 * fsb + pb + rowb[r]);
+                bst(hy[r], RB, vk, 4 * fsb + pb + rowb[r]);
+                bst(hz[r], RB, vk, 5 * fsb + pb + rowb[r]);
             }
         }
 
@@ -552,7 +601,6 @@ k_leapfrog(rf_leap_t a) {
                 const T hyk = from_lower(hy[r]);   // hy(i, j, k-1)
                 const int jc = jr[r];
                 if (j0 - 1 + r < jend && col_own) {
-                    const long long idx = base + jc * sy + kc;
                     const bool in = i < g.nx && jc < g.ny && kc < g.nz;
                     const bool j_ok = g.py || jc >= 1, k_ok = g.pz || kc >= 1;
                     const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
@@ -612,9 +660,9 @@ k_leapfrog(rf_leap_t a) {
                                            : apply_sources<T>(a.srcs, a.n_srcs, e[c], c, i, jc, kc, g.nx, g.ny,
                                                               g.nz, step, 1);
                     }
-                    EB0[idx] = e[0];
-                    EB1[idx] = e[1];
-                    EB2[idx] = e[2];
+                    bst(e[0], RB, vk, pb + rowb[r]);
+                    bst(e[1], RB, vk, fsb + pb + rowb[r]);
+                    bst(e[2], RB, vk, 2 * fsb + pb + rowb[r]);
                 }
             }
         }
@@ -929,14 +977,15 @@ template <typename T, typename TagT, int R>
 int capacity_tag(const rf_leap_t *a) {
     int nb = 0, cus = 0, dev = 0;
     const size_t lds = leap_lds<T>(a);
+    const int bw = WPB;
     hipError_t e = a->zc ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, true, R>), WL * WPB, lds)
+                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, true, R>), WL * bw, lds)
                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, false, R>), WL * WPB, lds);
+                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, false, R>), WL * bw, lds);
     if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nb <= 0)
         return 0;
-    return nb * cus * WPB;
+    return nb * cus * bw;
 }
 
 template <typename T, int R>

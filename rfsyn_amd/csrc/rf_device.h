@@ -193,8 +193,12 @@ typedef struct {
  * fields (rows, chunk: 0 = automatic). */
 typedef struct {
     rf_geom_t g;
-    const void *ea[3], *ha[3];
-    void *eb[3], *hb[3];
+    /* state blocks: the six arrays ex ey ez hx hy hz of state n (A, read)
+     * and n+1 (B, written), `fstride` elements apart, 6 * fstride *
+     * sizeof(T) < 2^32 bytes */
+    const void *blk_a;
+    void *blk_b;
+    long long fstride;
     double cb, dx;
     int ib0[3], ib1[3];         /* owned node box */
     int kbrk;                   /* z tiles restart here (ib0[2] <= kbrk <= ib1[2]) */
@@ -218,6 +222,7 @@ typedef struct {
     int zplo, zphi, zlo, zhi, zlen;
     const void *zk[6];
     void *zpsi;
+    long long zbytes;           /* size of the z-psi store (< 2^32) */
     int zpar;
     /* work queue: 2 x 8 zero-initialised device counters, the set
      * qpar used by this launch (alternate it from launch to launch) */

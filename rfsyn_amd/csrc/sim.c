@@ -66,7 +66,8 @@ struct fdtd_sim {
     rf_geom_t geom;
     size_t cells;
     rf_stream_t *st;
-    void *f[6];
+    void *f[6];           /* current state: six arrays of one block */
+    void *blk, *blk2;     /* the state blocks (f, and f2 of the fused path) */
     int has_cpml;
     rf_cpml_dev_t cp;
     int has_mat;
@@ -180,8 +181,8 @@ fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml, const mater
     if (s->K > 64) s->K = 64;
     s->st = rf_stream_create();
     if (!s->st) goto fail;
-    for (int q = 0; q < 6; q++)
-        if (dalloc(s, &s->f[q], s->cells * s->es)) goto fail;
+    if (dalloc(s, &s->blk, 6 * s->cells * s->es)) goto fail;
+    for (int q = 0; q < 6; q++) s->f[q] = (char *)s->blk + q * s->cells * s->es;
     if (upload_fields(s, g)) goto fail;
     if (pml) {
         if (rf_cpml_build(pml, s->prec, &s->cp) != 0) goto fail;
@@ -210,7 +211,8 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     if (s->st) rf_sync(s->st);
     rf_graph_destroy(s->graphs[0]);
     rf_graph_destroy(s->graphs[1]);
-    for (int q = 0; q < 6; q++) { rf_free(s->f[q]); rf_free(s->f2[q]); }
+    rf_free(s->blk);
+    rf_free(s->blk2);
     free(s->h_pts);
     if (s->has_cpml) rf_cpml_release(&s->cp);
     if (s->has_mat) rf_materials_release(&s->mat);
@@ -360,9 +362,12 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index, const doub
 /* The fused leapfrog kernel is the default step; RFSYN_FUSED=0 in the
  * environment selects the separate H and E kernels instead. */
 static int want_fused(const fdtd_sim_t *s) {
-    (void)s;
     const char *env = getenv("RFSYN_FUSED");
-    return !(env && env[0] == '0');
+    if (env && env[0] == '0') return 0;
+    /* the kernel addresses a state block and the z-psi store with 32-bit
+     * byte offsets */
+    const double zb = 6.0 * (double)(s->nx + 1) * (double)(s->ny + 1) * (s->nz + 1) * (double)s->es;
+    return 6.0 * (double)s->cells * (double)s->es < 4294967296.0 && zb < 4294967296.0;
 }
 
 /* Multiply-high division by d >= 1 for 32-bit n (round-up method of
@@ -439,10 +444,9 @@ static int finalize(fdtd_sim_t *s) {
     s->fused = want_fused(s);
     if (s->fused) {
         /* B starts as a copy of A so nodes no kernel writes agree */
-        for (int q = 0; q < 6; q++)
-            if (dalloc(s, &s->f2[q], s->cells * s->es) ||
-                rf_d2d(s->f2[q], s->f[q], s->cells * s->es, s->st))
-                return -1;
+        if (dalloc(s, &s->blk2, 6 * s->cells * s->es) || rf_d2d(s->blk2, s->blk, 6 * s->cells * s->es, s->st))
+            return -1;
+        for (int q = 0; q < 6; q++) s->f2[q] = (char *)s->blk2 + q * s->cells * s->es;
         if (rf_sync(s->st)) return -1;
         if (s->K & 1) s->K++;   /* a captured block must return to its buffer parity */
         s->post_in_kernel = !(s->has_mat && s->mat.n_tensor);
@@ -667,12 +671,9 @@ static int enqueue_fused(fdtd_sim_t *s) {
     rf_leap_t a;
     memset(&a, 0, sizeof a);
     a.g = s->geom;
-    for (int q = 0; q < 3; q++) {
-        a.ea[q] = s->f[q];
-        a.ha[q] = s->f[3 + q];
-        a.eb[q] = s->f2[q];
-        a.hb[q] = s->f2[3 + q];
-    }
+    a.blk_a = s->blk;
+    a.blk_b = s->blk2;
+    a.fstride = (long long)s->cells;
     a.cb = s->dt / (MU0 * s->dx);
     a.dx = s->dx;
     for (int q = 0; q < 3; q++) {
@@ -695,6 +696,7 @@ static int enqueue_fused(fdtd_sim_t *s) {
         a.zk[4] = c->b_m[2];
         a.zk[5] = c->a_m[2];
         a.zpsi = s->zpsi;
+        a.zbytes = (long long)((s->nx + 1) * (s->ny + 1) * (size_t)s->cp.d.len[2] * 6 * s->es);
         a.zpar = s->zpar;
     }
     const rf_mat_t *m = s->has_mat ? &s->mat.d : &s->vac;
@@ -736,6 +738,9 @@ static int enqueue_fused(fdtd_sim_t *s) {
         s->f[q] = s->f2[q];
         s->f2[q] = t;
     }
+    void *tb = s->blk;
+    s->blk = s->blk2;
+    s->blk2 = tb;
     s->parity ^= 1;
     s->zpar ^= 1;
     return 0;
