@@ -638,7 +638,10 @@ k_leapfrog(rf_leap_t a) {
                     } else {
                         const unsigned tag = tg[r];
                         const Coefs<T> kf = entry_coefs<T>(a.entries, s_ca, s_cb, s_fl, ncache, tag);
-                        if (kf.fl == 0) {
+                        // split ADE step: dispersive edges take the plain
+                        // part here and the ADE pass completes them
+                        const uint32_t fl = a.ade_split ? kf.fl & ~(RF_EDGE_ADE * 0x010101u) : kf.fl;
+                        if (fl == 0) {
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
                                 const T n = kf.ca[c] * e[c] + kf.cb[c] * (dA[c] - dB[c]);
@@ -647,7 +650,7 @@ k_leapfrog(rf_leap_t a) {
                         } else {
                             const uint32_t ai = static_cast<int>(tag) < ncache ? s_ai[tag]
                                                                                 : entry_ade_ids<T>(a.entries, tag);
-                            e_flagged3<T>(e, dA, dB, on, kf.ca, kf.cb, kf.fl, ai, s_ade, n_adec, a.n_slots, a.cold,
+                            e_flagged3<T>(e, dA, dB, on, kf.ca, kf.cb, fl, ai, s_ade, n_adec, a.n_slots, a.cold,
                                           i, jc, kc);
                         }
                     }
@@ -955,6 +958,170 @@ int shell_e_t(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void 
     return -1;
 }
 
+// ------------------------------------------------------------ ADE pass
+// With the split ADE step the fused kernel gives every dispersive edge of
+// its box the first two terms of the trapezoidal update,
+// t = Ca e^n + Cb (dA - dB), and stores t as E^{n+1}; this pass completes
+// it, E^{n+1} = t - Cu sum_u, and advances J and P.  These are the
+// operations of ade_edge (yee_device.h) in the same order and precision,
+// so E, J and P are bitwise those of the one-pass update.  One thread per
+// node of the part of the ADE state box that lies in the fused box, k
+// fastest; the state of all three components is loaded before any is
+// updated.
+template <typename T, int NS>
+__device__ __forceinline__ T ade_finish(const typename EntryOf<T>::ade &a, T t, T e_old, const T jv[NS],
+                                        const T pv[NS], T nj[NS], T np[NS]) {
+    T u[NS];
+    T sum_u = T(0);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (s < a.n_poles) {
+            u[s] = a.has_p[s] ? a.kj[s] * jv[s] - a.kp[s] * pv[s] : a.kj[s] * jv[s];
+            sum_u += u[s];
+        }
+    }
+    const T e_new = t - a.Cu * sum_u;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (s < a.n_poles) {
+            const T S = u[s] + a.v[s] * (e_new + e_old);
+            nj[s] = S - jv[s];
+            np[s] = a.has_p[s] ? pv[s] + a.hdt * S : pv[s];
+        }
+    }
+    return e_new;
+}
+
+template <typename T, typename TagT, int NS>
+__global__ void __launch_bounds__(256)
+k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, unsigned count,
+           const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
+           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, int n_adec) {
+    using Entry = typename EntryOf<T>::type;
+    using Ade = typename EntryOf<T>::ade;
+    // the first n_adec ADE sets and, for the first ncache entries of the
+    // material table, the flags and ADE set ids of the three components
+    // (one byte each), staged in LDS
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    Ade *s_ade = reinterpret_cast<Ade *>(s_raw);
+    uint32_t *s_ent = reinterpret_cast<uint32_t *>(s_raw + (size_t)n_adec * sizeof(Ade));
+    const Entry *ents = static_cast<const Entry *>(m.entries);
+    const Ade *ades = static_cast<const Ade *>(m.ade);
+    for (int e = threadIdx.x; e < ncache; e += blockDim.x) {
+        const Entry &en = ents[e];
+        s_ent[2 * e] = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+        s_ent[2 * e + 1] = (static_cast<uint32_t>(en.ade[0]) & 0xffu) | ((static_cast<uint32_t>(en.ade[1]) & 0xffu) << 8) |
+                           ((static_cast<uint32_t>(en.ade[2]) & 0xffu) << 16);
+    }
+    for (int e = threadIdx.x; e < n_adec; e += blockDim.x) s_ade[e] = ades[e];
+    __syncthreads();
+
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const Geo g = geo_of(gg);
+    const unsigned q = t / nk;
+    const int k = lo2 + static_cast<int>(t - q * nk);
+    const int j = lo1 + static_cast<int>(q % nj);
+    const int i = lo0 + static_cast<int>(q / nj);
+    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+    const unsigned tag = load_tag<TagT>(m.tag, idx);
+    uint32_t fl, ai;
+    if (static_cast<int>(tag) < ncache) {
+        fl = s_ent[2 * tag];
+        ai = s_ent[2 * tag + 1];
+    } else {
+        const Entry &en = ents[tag];
+        fl = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+        ai = (static_cast<uint32_t>(en.ade[0]) & 0xffu) | ((static_cast<uint32_t>(en.ade[1]) & 0xffu) << 8) |
+             ((static_cast<uint32_t>(en.ade[2]) & 0xffu) << 16);
+    }
+    // E ranges of the update (grid.c:95-117)
+    const bool in = i < g.nx && j < g.ny && k < g.nz;
+    const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
+    const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
+    bool act[3];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const uint32_t f = (fl >> (8 * c)) & 0xffu;
+        act[c] = on[c] && (f & RF_EDGE_ADE) && !(f & RF_EDGE_TENSOR);
+        any = any || act[c];
+    }
+    if (!any) return;
+    const T *ea[3] = {exa, eya, eza};
+    T *eb[3] = {exb, eyb, ezb};
+    T *__restrict__ J = static_cast<T *>(m.J);
+    T *__restrict__ P = static_cast<T *>(m.P);
+    const long long st = m.ade_stride;
+    const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
+    const Ade *ad[3];
+    T eo[3], et[3], jv[3][NS], pv[3][NS];
+    // every load of the node before any update
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int id = (ai >> (8 * c)) & 0xff;
+        ad[c] = id < n_adec ? s_ade + id : ades + id;
+        if (!act[c]) continue;
+        eo[c] = ea[c][idx];
+        et[c] = eb[c][idx];
+        const int np = ad[c]->n_poles;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const long long o = (long long)(3 * s + c) * st + cell;
+            jv[c][s] = s < np ? J[o] : T(0);
+            pv[c][s] = s < np && ad[c]->has_p[s] ? P[o] : T(0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (!act[c]) continue;
+        T nj[NS], np[NS];
+        eb[c][idx] = ade_finish<T, NS>(*ad[c], et[c], eo[c], jv[c], pv[c], nj, np);
+        const int npl = ad[c]->n_poles;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (s >= npl) continue;
+            const long long o = (long long)(3 * s + c) * st + cell;
+            J[o] = nj[s];
+            if (ad[c]->has_p[s]) P[o] = np[s];
+        }
+    }
+}
+
+template <typename T, typename TagT>
+int ade_pass_tag(const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3], void *const eb[3],
+                 const rf_mat_t *m, hipStream_t st) {
+    const unsigned nj = static_cast<unsigned>(hi[1] - lo[1]), nk = static_cast<unsigned>(hi[2] - lo[2]);
+    const unsigned count = static_cast<unsigned>(hi[0] - lo[0]) * nj * nk;
+    const dim3 grd((count + 255) / 256);
+    using Ade = typename EntryOf<T>::ade;
+    const int n_adec = m->n_ade < 16 ? m->n_ade : 16;
+    const int ncache = m->n_entries < 4096 ? m->n_entries : 4096;
+    const size_t lds = (size_t)n_adec * sizeof(Ade) + (size_t)ncache * 8;
+#define RF_ADE_PASS(NS)                                                                                        \
+    hipLaunchKernelGGL((k_ade_pass<T, TagT, NS>), grd, dim3(256), lds, st, *g, lo[0], lo[1], lo[2], nj, nk,    \
+                       count, static_cast<const T *>(ea[0]), static_cast<const T *>(ea[1]),                    \
+                       static_cast<const T *>(ea[2]), static_cast<T *>(eb[0]), static_cast<T *>(eb[1]),        \
+                       static_cast<T *>(eb[2]), *m, ncache, n_adec)
+    if (m->n_slots <= 1) RF_ADE_PASS(1);
+    else if (m->n_slots == 2) RF_ADE_PASS(2);
+    else if (m->n_slots <= 4) RF_ADE_PASS(4);
+    else RF_ADE_PASS(RF_MAX_SLOTS);
+#undef RF_ADE_PASS
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+int ade_pass_t(const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3], void *const eb[3],
+               const rf_mat_t *m, hipStream_t st) {
+    switch (m->tag_bytes) {
+    case 1: return ade_pass_tag<T, uint8_t>(g, lo, hi, ea, eb, m, st);
+    case 2: return ade_pass_tag<T, uint16_t>(g, lo, hi, ea, eb, m, st);
+    case 4: return ade_pass_tag<T, uint32_t>(g, lo, hi, ea, eb, m, st);
+    }
+    return -1;
+}
+
 template <typename T>
 size_t leap_lds(const rf_leap_t *a) {
     const size_t nls = a->n_srcs <= MAX_LSRC ? (size_t)a->n_srcs : 0;
@@ -1062,10 +1229,12 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
     }
     if (target_waves > 0) resident = target_waves;
     if (chunk <= 0) {
-        // about three items per resident wavefront: enough for the queue
-        // to even out unequal tiles, few enough that the warm-up plane of
-        // each item stays a small share
-        long long chunks = (3LL * resident + tiles - 1) / tiles;
+        // about twelve items per resident wavefront, at least 8 planes
+        // each: short items keep the wavefronts of an XCD on neighbouring
+        // tiles at nearby planes (their halo rows meet in its L2) and leave
+        // a short tail when the queue drains; each item recomputes one
+        // warm-up plane
+        long long chunks = (12LL * resident + tiles - 1) / tiles;
         if (chunks < 1) chunks = 1;
         chunk = (int)((planes + chunks - 1) / chunks);
         if (chunk < 8) chunk = 8;
@@ -1115,6 +1284,19 @@ extern "C" int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t 
     hipStream_t st = rf_raw_stream(s);
     return prec == 0 ? shell_h_t<double>(g, sh, fa, hb, cb, dx, cpml, z, st)
                      : shell_h_t<float>(g, sh, fa, hb, cb, dx, cpml, z, st);
+}
+
+extern "C" int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3],
+                                  void *const eb[3], const rf_mat_t *mat, rf_stream_t *s) {
+    long long count = 1;
+    for (int q = 0; q < 3; q++) {
+        if (hi[q] < lo[q] || lo[q] < 0) return -1;
+        count *= hi[q] - lo[q];
+    }
+    if (count == 0) return 0;
+    if (count >= (1LL << 31) || !mat->tag || !mat->J || mat->n_slots < 1 || mat->n_slots > RF_MAX_SLOTS) return -1;
+    hipStream_t st = rf_raw_stream(s);
+    return prec == 0 ? ade_pass_t<double>(g, lo, hi, ea, eb, mat, st) : ade_pass_t<float>(g, lo, hi, ea, eb, mat, st);
 }
 
 extern "C" int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3],

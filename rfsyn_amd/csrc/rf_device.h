@@ -207,6 +207,8 @@ typedef struct {
     const void *entries;
     const void *ade;            /* ADE coefficient sets (rf_ade32_t / rf_ade64_t) */
     int n_ade, n_slots;         /* sets; poles of the state arrays */
+    int ade_split;              /* 1: dispersive edges get the plain part of
+                                 * the update; rf_launch_ade_pass completes them */
     double vac_ca;
     const rf_leap_cold_t *cold; /* device */
     const rf_leap_src_t *srcs;  /* device table */
@@ -236,9 +238,10 @@ typedef struct {
 } rf_leap_t;
 
 /* Tiling of the fused step over a->ib0..ib1; rows: 2 or 4 (FP64), 4 or 8
- * (FP32), 0 = default; chunk 0 = about three work items per resident
- * wavefront (target_waves > 0 overrides the residency the device
- * reports).  0 on success, -1 for an unsupported request. */
+ * (FP32), 0 = default; chunk 0 = about twelve work items per resident
+ * wavefront, at least 8 planes each (target_waves > 0 overrides the
+ * residency the device reports).  0 on success, -1 for an unsupported
+ * request. */
 int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves);
 /* tag NULL = vacuum (vac_ca); an empty box launches nothing. */
 int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s);
@@ -274,6 +277,13 @@ int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *
                       void *const eb[3], double dx, const rf_cpml_t *cpml, const rf_zpsi_t *z,
                       const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
                       rf_stream_t *s);
+
+/* ADE pass of the split step: for the nodes of [lo, hi) (inside the ADE
+ * state box and the fused box), every dispersive edge whose E^{n+1} in eb
+ * holds the plain part Ca e^n + Cb (dA - dB) of its update is completed
+ * with E^n from ea, and J, P advance (ade_edge, yee_device.h). */
+int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3],
+                       void *const eb[3], const rf_mat_t *mat, rf_stream_t *s);
 
 /* ----------------------------------------------------------- launches
  * `prec` selects the field precision: 0 = double, 1 = float.  All

@@ -119,6 +119,7 @@ struct fdtd_sim {
     rf_shell_t shell;        /* the nodes around it (CPML slabs + margin) */
     int n_srcs, src_box[6];
     int post_in_kernel;      /* post-E plane sources inside the fused kernel */
+    int ade_split_off;       /* keep the ADE update inside the fused kernel */
     rf_point_src_t *h_pts;   /* host copy of the point sources (device wave tables) */
 
     /* kernel timing with HIP events (profiling mode launches directly) */
@@ -478,6 +479,7 @@ static int finalize(fdtd_sim_t *s) {
         s->leap_rows = (e = getenv("RFSYN_LEAP_ROWS")) ? atoi(e) : 0;
         s->leap_chunk = (e = getenv("RFSYN_LEAP_CHUNK")) ? atoi(e) : 0;
         s->leap_waves = (e = getenv("RFSYN_LEAP_WAVES")) ? atoi(e) : 0;
+        s->ade_split_off = (e = getenv("RFSYN_ADE_SPLIT")) && atoi(e) == 0;
     }
     int P = (int)s->n_probes;
     size_t cap = 0;
@@ -665,8 +667,26 @@ static int mark(fdtd_sim_t *s, int q) {
     return rf_event_record(s->ev[s->ev_used + (size_t)q], s->st);
 }
 
+/* Box of the split ADE pass: the ADE state box within the fused box.
+ * Empty (0) when the ADE update stays in the fused kernel: no dispersive
+ * material, or a source inside the fused kernel may touch a dispersive
+ * node (its value enters the update before or after the ADE terms). */
+static int ade_pass_box(const fdtd_sim_t *s, int lo[3], int hi[3]) {
+    if (!s->has_mat || s->ade_split_off || s->mat.d.n_ade <= 0 || !s->mat.d.J) return 0;
+    const rf_mat_t *m = &s->mat.d;
+    int meets = s->n_srcs > 0;
+    for (int a = 0; a < 3; a++) {
+        lo[a] = m->b0[a] > s->ib0[a] ? m->b0[a] : s->ib0[a];
+        hi[a] = m->b0[a] + m->bn[a] < s->ib1[a] ? m->b0[a] + m->bn[a] : s->ib1[a];
+        if (hi[a] <= lo[a]) return 0;
+        meets = meets && s->src_box[2 * a] < m->b0[a] + m->bn[a] && s->src_box[2 * a + 1] >= m->b0[a];
+    }
+    return !meets;
+}
+
 /* The fused step: the leapfrog kernel on the plain box (timed in event
- * slots 0-1) and the shell kernels around it (slots 2-3). */
+ * slots 0-1), then the split ADE pass and the shell kernels around the
+ * box (slots 2-3). */
 static int enqueue_fused(fdtd_sim_t *s) {
     rf_leap_t a;
     memset(&a, 0, sizeof a);
@@ -717,11 +737,14 @@ static int enqueue_fused(fdtd_sim_t *s) {
     a.n_srcs = s->n_srcs;
     memcpy(a.src_box, s->src_box, sizeof a.src_box);
     a.d_step = s->d_step;
+    int alo[3], ahi[3];
+    a.ade_split = ade_pass_box(s, alo, ahi);
     if (rf_leap_plan(s->prec, &a, s->leap_rows, s->leap_chunk, s->leap_waves)) {
         rf_set_error("fused step: unsupported tiling (rows %d, chunk %d)", s->leap_rows, s->leap_chunk);
         return -1;
     }
     if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
+    if (a.ade_split && rf_launch_ade_pass(s->prec, &s->geom, alo, ahi, s->f, s->f2, m, s->st)) return -1;
     if (s->shell.n) {
         const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
         rf_zpsi_t z = {s->zpsi, s->zpar, s->zc ? s->cp.d.len[2] : 0};

@@ -245,3 +245,74 @@ def test_plain_box_and_shell_fp64(gpu, oracle, case):
     assert same_values(a["td"], b["td"])
     for k in a["fields"]:
         assert same_values(a["fields"][k], b["fields"][k]), (k, np.max(np.abs(a["fields"][k] - b["fields"][k])))
+
+
+def _ade_region_setup(n):
+    """A 2-pole Lorentz block and a Drude block that both cross the CPML
+    slabs of a closed box, so the ADE state box is split between the fused
+    box (ADE pass) and the shell (in-kernel ADE)."""
+    def setup(lib, g, mat):
+        d = capi.MaterialDispersive()
+        d.eps_inf = 2.25
+        d.n_lorentz = 2
+        d.lorentz_deps[0], d.lorentz_deps[1] = 1.0, 0.5
+        d.lorentz_w0[0], d.lorentz_w0[1] = 2 * np.pi * 60e9, 2 * np.pi * 90e9
+        d.lorentz_delta[0], d.lorentz_delta[1] = 0.05 * 2 * np.pi * 60e9, 0.05 * 2 * np.pi * 90e9
+        lid = C.c_uint8()
+        assert lib.materials_register_dispersive(mat, C.byref(d), C.byref(lid)) == 0
+        dr = capi.MaterialDispersive()
+        dr.eps_inf = 1.0
+        dr.n_drude = 1
+        dr.drude_wp[0] = 1.37e16
+        dr.drude_gamma[0] = 1e14
+        did = C.c_uint8()
+        assert lib.materials_register_dispersive(mat, C.byref(dr), C.byref(did)) == 0
+        assert lib.materials_assign_box(mat, g, 2, n[0] - 3, 1, n[1] - 2, n[2] // 3, n[2] // 3 + 6, lid.value) == 0
+        assert lib.materials_assign_box(mat, g, 4, n[0] // 2, 6, n[1] - 1, n[2] // 2, n[2] - 2, did.value) == 0
+        assert lib.materials_set_box(mat, g, 1, n[0] - 1, 2, 9, 3, 7, 3.0, 0.0) == 0
+    return setup
+
+
+def _ade_region_run(lib, src_in_ade, steps=50):
+    n = (22, 20, 36)
+    run = NativeRun(lib, n, 0.5e-3, 0.99, cpml=capi.CpmlParams(5, 3.0, 1.0, 2.0, 0.05, 1e-8),
+                    material_setup=_ade_region_setup(n), init_fields=lambda g: random_fill(g, 5))
+    p = modgauss_params(40e9, 30e9)
+    run.add_point_source(capi.FIELD_EY, (11, 10, 4), "source_waveform_modulated_gaussian", p)
+    if src_in_ade:   # inside the Lorentz block: the ADE update stays in the fused kernel
+        run.add_point_source(capi.FIELD_EX, (10, 9, 14), "source_waveform_modulated_gaussian", p, amp=0.5)
+    pid = run.add_probe(capi.FIELD_EX, (9, 9, 13), steps, None)
+    run.run(steps)
+    out = {"td": run.td(pid, steps), "fields": run.fields()}
+    run.close()
+    return out
+
+
+@pytest.mark.parametrize("split", ["auto", "off"])
+@pytest.mark.parametrize("src_in_ade", [False, True])
+def test_ade_pass_box_and_shell_fp64(gpu, oracle, split, src_in_ade, monkeypatch):
+    """Dispersive blocks across the CPML shell, random initial fields: the
+    split ADE step (plain part in the fused kernel, ADE pass after it), the
+    in-kernel ADE update (forced, or chosen because a source lies in the
+    ADE box) and the shell's ADE update, bitwise against the oracle."""
+    if split == "off":
+        monkeypatch.setenv("RFSYN_ADE_SPLIT", "0")
+    a = _ade_region_run(gpu, src_in_ade)
+    b = _ade_region_run(oracle, src_in_ade)
+    assert np.max(np.abs(b["td"])) > 0
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), (k, np.max(np.abs(a["fields"][k] - b["fields"][k])))
+
+
+def test_ade_split_matches_inline_fp32(gpu, monkeypatch):
+    """FP32 periodic dispersive stack: the split ADE step and the in-kernel
+    update perform the same operations in the same order, so the fields
+    agree bit for bit."""
+    a = _run_periodic_stack(gpu, capi.FDTD_FP32, 120)
+    monkeypatch.setenv("RFSYN_ADE_SPLIT", "0")
+    b = _run_periodic_stack(gpu, capi.FDTD_FP32, 120)
+    assert np.max(np.abs(a["td"])) > 0
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
