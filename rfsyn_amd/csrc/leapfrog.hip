@@ -387,7 +387,8 @@ k_leapfrog(rf_leap_t a) {
     // in its L2; each wavefront takes the next item of its XCD's counter
     // until the range is exhausted, which balances tiles of unequal cost
     // (dispersive, source and CPML columns).  The counters of this launch
-    // are a.ctr[a.qpar][*]; the other set is zeroed here for the next one.
+    // are a.ctr[a.qpar][*] (a.ctr points at the pair of sets of its part);
+    // the other set is zeroed here for the next step.
     const int xcd = blockIdx.x % N_XCD;
     if (blockIdx.x == 0 && threadIdx.x < N_XCD) a.ctr[(1 - a.qpar) * N_XCD + threadIdx.x] = 0u;
     const int item_lo = (int)(((long long)a.n_items * xcd) / N_XCD);
@@ -399,9 +400,11 @@ k_leapfrog(rf_leap_t a) {
     if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
     const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
     if (item >= item_hi) break;
-    const int tk = item % a.ntk;
-    const int tj = (item / a.ntk) % a.ntj;
-    const int tc = item / (a.ntk * a.ntj);
+    const int ntl = a.tkr[1] + a.tkr[3];   // z tiles of this part
+    const int tl = item % ntl;
+    const int tk = tl < a.tkr[1] ? a.tkr[0] + tl : a.tkr[2] + (tl - a.tkr[1]);
+    const int tj = (item / ntl) % a.ntj;
+    const int tc = item / (ntl * a.ntj);
 
     // the box of nodes this launch owns, [ib0, ib1) per axis; along z the
     // tiles restart at kbrk (the start of the upper CPML slab)
@@ -442,9 +445,14 @@ k_leapfrog(rf_leap_t a) {
         zp.am = zp.on ? static_cast<const T *>(a.zk[5])[kc] : T(0);
         zs = __ballot(zp.on) != 0;
     }
-    T *zpsi = static_cast<T *>(a.zpsi);
+    // z-psi store: per (i, j) node row six slot rows of zlen values,
+    // slot-major, so the slab lanes of a wavefront access one contiguous run
     const int zr = 2 * a.zpar, zw = 2 - 2 * a.zpar;   // H psi slots read / written
     const rsrc_t RZ = make_rsrc(a.zpsi, ZC ? static_cast<unsigned>(a.zbytes) : 0u);
+    const unsigned zsb = static_cast<unsigned>(a.zlen) * sizeof(T);   // bytes per slot row
+    auto zrow = [&](int ii, int jj) {
+        return (static_cast<unsigned>(ii) * static_cast<unsigned>(g.ny + 1) + static_cast<unsigned>(jj)) * 6 * zsb;
+    };
     // sources: does their bounding box meet this tile's rows and columns;
     // the components they touch before and after the E update
     const bool src_tile = a.n_srcs > 0 && a.src_box[2] < j0 + R && a.src_box[3] >= j0 &&
@@ -519,17 +527,15 @@ k_leapfrog(rf_leap_t a) {
         if constexpr (ZC) {
             if (zs) {
                 const bool slab = zp.cq >= 0;
-                const unsigned zl = static_cast<unsigned>(slab ? zp.cq : 0) * 6 * ES;
+                const unsigned zl = static_cast<unsigned>(slab ? zp.cq : 0) * ES;
 #pragma unroll
                 for (int r = 0; r <= R; r++) {
-                    const unsigned zb = ((static_cast<unsigned>(i) * static_cast<unsigned>(g.ny + 1) +
-                                          static_cast<unsigned>(jr[r])) *
-                                         static_cast<unsigned>(a.zlen)) * 6 * ES;
-                    const T h0 = bld<T>(RZ, zl, zb + zr * ES), h1 = bld<T>(RZ, zl, zb + (zr + 1) * ES);
+                    const unsigned zb = zrow(i, jr[r]);
+                    const T h0 = bld<T>(RZ, zl, zb + zr * zsb), h1 = bld<T>(RZ, zl, zb + (zr + 1) * zsb);
                     zh0[r] = slab ? h0 : T(0);
                     zh1[r] = slab ? h1 : T(0);
                     if (r >= 1) {
-                        const T e0 = bld<T>(RZ, zl, zb + 4 * ES), e1 = bld<T>(RZ, zl, zb + 5 * ES);
+                        const T e0 = bld<T>(RZ, zl, zb + 4 * zsb), e1 = bld<T>(RZ, zl, zb + 5 * zsb);
                         ze0[r] = slab ? e0 : T(0);
                         ze1[r] = slab ? e1 : T(0);
                     }
@@ -572,9 +578,9 @@ k_leapfrog(rf_leap_t a) {
                     if (do_x) dBx = zterm<T>(dBx, zp.kh, zp.bh, zp.ah, slab, zh0[r], px_, dx);
                     if (do_y) dAy = zterm<T>(dAy, zp.kh, zp.bh, zp.ah, slab, zh1[r], py_, dx);
                     if (slab && own) {
-                        const long long zo = (((long long)i * (g.ny + 1) + jc) * a.zlen + zp.cq) * 6;
-                        if (do_x) zpsi[zo + zw] = px_;
-                        if (do_y) zpsi[zo + zw + 1] = py_;
+                        const unsigned zl = static_cast<unsigned>(zp.cq) * ES, zb = zrow(i, jc);
+                        if (do_x) bst(px_, RZ, zl, zb + zw * zsb);
+                        if (do_y) bst(py_, RZ, zl, zb + (zw + 1) * zsb);
                     }
                 }
             }
@@ -623,9 +629,9 @@ k_leapfrog(rf_leap_t a) {
                             if (on[0]) dB[0] = zterm<T>(dB[0], zp.km, zp.bm, zp.am, slab, ze0[r], px_, dx);
                             if (on[1]) dA[1] = zterm<T>(dA[1], zp.km, zp.bm, zp.am, slab, ze1[r], py_, dx);
                             if (slab) {
-                                const long long zo = (((long long)i * (g.ny + 1) + jc) * a.zlen + zp.cq) * 6;
-                                if (on[0]) zpsi[zo + 4] = px_;
-                                if (on[1]) zpsi[zo + 5] = py_;
+                                const unsigned zl = static_cast<unsigned>(zp.cq) * ES, zb = zrow(i, jc);
+                                if (on[0]) bst(px_, RZ, zl, zb + 4 * zsb);
+                                if (on[1]) bst(py_, RZ, zl, zb + 5 * zsb);
                             }
                         }
                     }
@@ -745,9 +751,9 @@ __device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, 
     T psi = T(0);
     if (cq >= 0) {
         T *zp = static_cast<T *>(z.zpsi);
-        const long long zo = (((long long)i * (g.ny + 1) + j) * z.zlen + cq) * 6;
-        psi = bb[k] * zp[zo + rd] + aa[k] * diff;
-        zp[zo + wr] = psi;
+        const long long zo = ((long long)i * (g.ny + 1) + j) * 6 * z.zlen + cq;
+        psi = bb[k] * zp[zo + (long long)rd * z.zlen] + aa[k] * diff;
+        zp[zo + (long long)wr * z.zlen] = psi;
     }
     return diff * kinv[k] + dx * psi;
 }
@@ -964,10 +970,16 @@ int shell_e_t(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void 
 // t = Ca e^n + Cb (dA - dB), and stores t as E^{n+1}; this pass completes
 // it, E^{n+1} = t - Cu sum_u, and advances J and P.  These are the
 // operations of ade_edge (yee_device.h) in the same order and precision,
-// so E, J and P are bitwise those of the one-pass update.  One thread per
-// node of the part of the ADE state box that lies in the fused box, k
-// fastest; the state of all three components is loaded before any is
-// updated.
+// so E, J and P are bitwise those of the one-pass update.
+//
+// One thread per node of the part of the ADE state box that lies in the
+// fused box, k fastest.  The pass streams the box: every thread loads E^n,
+// t and the state of all slots of its node without waiting for its tag
+// (the partly dispersive cache lines of the box are fetched whole either
+// way), so one memory round trip covers the node; only the dispersive
+// edges compute and store.
+constexpr int ADE_PASS_MAX_SETS = 64;   // ADE sets staged in LDS
+
 template <typename T, int NS>
 __device__ __forceinline__ T ade_finish(const typename EntryOf<T>::ade &a, T t, T e_old, const T jv[NS],
                                         const T pv[NS], T nj[NS], T np[NS]) {
@@ -996,24 +1008,25 @@ template <typename T, typename TagT, int NS>
 __global__ void __launch_bounds__(256)
 k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, unsigned count,
            const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
-           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, int n_adec) {
+           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache) {
     using Entry = typename EntryOf<T>::type;
     using Ade = typename EntryOf<T>::ade;
-    // the first n_adec ADE sets and, for the first ncache entries of the
-    // material table, the flags and ADE set ids of the three components
-    // (one byte each), staged in LDS
+    // every ADE set and, for the first ncache entries of the material
+    // table, the flags and ADE set ids of the three components (one byte
+    // each), staged in LDS
     extern __shared__ __align__(16) unsigned char s_raw[];
     Ade *s_ade = reinterpret_cast<Ade *>(s_raw);
-    uint32_t *s_ent = reinterpret_cast<uint32_t *>(s_raw + (size_t)n_adec * sizeof(Ade));
+    uint32_t *s_ent = reinterpret_cast<uint32_t *>(s_raw + (size_t)m.n_ade * sizeof(Ade));
     const Entry *ents = static_cast<const Entry *>(m.entries);
-    const Ade *ades = static_cast<const Ade *>(m.ade);
     for (int e = threadIdx.x; e < ncache; e += blockDim.x) {
         const Entry &en = ents[e];
         s_ent[2 * e] = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
-        s_ent[2 * e + 1] = (static_cast<uint32_t>(en.ade[0]) & 0xffu) | ((static_cast<uint32_t>(en.ade[1]) & 0xffu) << 8) |
+        s_ent[2 * e + 1] = (static_cast<uint32_t>(en.ade[0]) & 0xffu) |
+                           ((static_cast<uint32_t>(en.ade[1]) & 0xffu) << 8) |
                            ((static_cast<uint32_t>(en.ade[2]) & 0xffu) << 16);
     }
-    for (int e = threadIdx.x; e < n_adec; e += blockDim.x) s_ade[e] = ades[e];
+    const Ade *ades = static_cast<const Ade *>(m.ade);
+    for (int e = threadIdx.x; e < m.n_ade; e += blockDim.x) s_ade[e] = ades[e];
     __syncthreads();
 
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1024,7 +1037,24 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
     const int j = lo1 + static_cast<int>(q % nj);
     const int i = lo0 + static_cast<int>(q / nj);
     const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+    const long long st = m.ade_stride;
+    const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
+    const T *__restrict__ J = static_cast<const T *>(m.J);
+    const T *__restrict__ P = static_cast<const T *>(m.P);
+    const int ns = m.n_slots;
     const unsigned tag = load_tag<TagT>(m.tag, idx);
+    const T eo[3] = {exa[idx], eya[idx], eza[idx]};
+    const T et[3] = {exb[idx], eyb[idx], ezb[idx]};
+    T jv[3][NS], pv[3][NS];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const long long o = (long long)(3 * s + c) * st + cell;
+            jv[c][s] = (NS <= 2 || s < ns) ? J[o] : T(0);
+            pv[c][s] = (NS <= 2 || s < ns) ? P[o] : T(0);
+        }
+
     uint32_t fl, ai;
     if (static_cast<int>(tag) < ncache) {
         fl = s_ent[2 * tag];
@@ -1039,51 +1069,23 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
     const bool in = i < g.nx && j < g.ny && k < g.nz;
     const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
     const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
-    bool act[3];
-    bool any = false;
+    T *eb[3] = {exb, eyb, ezb};
+    T *__restrict__ Jw = static_cast<T *>(m.J);
+    T *__restrict__ Pw = static_cast<T *>(m.P);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const uint32_t f = (fl >> (8 * c)) & 0xffu;
-        act[c] = on[c] && (f & RF_EDGE_ADE) && !(f & RF_EDGE_TENSOR);
-        any = any || act[c];
-    }
-    if (!any) return;
-    const T *ea[3] = {exa, eya, eza};
-    T *eb[3] = {exb, eyb, ezb};
-    T *__restrict__ J = static_cast<T *>(m.J);
-    T *__restrict__ P = static_cast<T *>(m.P);
-    const long long st = m.ade_stride;
-    const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
-    const Ade *ad[3];
-    T eo[3], et[3], jv[3][NS], pv[3][NS];
-    // every load of the node before any update
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int id = (ai >> (8 * c)) & 0xff;
-        ad[c] = id < n_adec ? s_ade + id : ades + id;
-        if (!act[c]) continue;
-        eo[c] = ea[c][idx];
-        et[c] = eb[c][idx];
-        const int np = ad[c]->n_poles;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const long long o = (long long)(3 * s + c) * st + cell;
-            jv[c][s] = s < np ? J[o] : T(0);
-            pv[c][s] = s < np && ad[c]->has_p[s] ? P[o] : T(0);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        if (!act[c]) continue;
+        if (!on[c] || !(f & RF_EDGE_ADE) || (f & RF_EDGE_TENSOR)) continue;
+        const Ade &ad = s_ade[(ai >> (8 * c)) & 0xff];
         T nj[NS], np[NS];
-        eb[c][idx] = ade_finish<T, NS>(*ad[c], et[c], eo[c], jv[c], pv[c], nj, np);
-        const int npl = ad[c]->n_poles;
+        eb[c][idx] = ade_finish<T, NS>(ad, et[c], eo[c], jv[c], pv[c], nj, np);
+        const int npl = ad.n_poles;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             if (s >= npl) continue;
             const long long o = (long long)(3 * s + c) * st + cell;
-            J[o] = nj[s];
-            if (ad[c]->has_p[s]) P[o] = np[s];
+            Jw[o] = nj[s];
+            if (ad.has_p[s]) Pw[o] = np[s];
         }
     }
 }
@@ -1095,14 +1097,13 @@ int ade_pass_tag(const rf_geom_t *g, const int lo[3], const int hi[3], void *con
     const unsigned count = static_cast<unsigned>(hi[0] - lo[0]) * nj * nk;
     const dim3 grd((count + 255) / 256);
     using Ade = typename EntryOf<T>::ade;
-    const int n_adec = m->n_ade < 16 ? m->n_ade : 16;
     const int ncache = m->n_entries < 4096 ? m->n_entries : 4096;
-    const size_t lds = (size_t)n_adec * sizeof(Ade) + (size_t)ncache * 8;
+    const size_t lds = (size_t)m->n_ade * sizeof(Ade) + (size_t)ncache * 8;
 #define RF_ADE_PASS(NS)                                                                                        \
     hipLaunchKernelGGL((k_ade_pass<T, TagT, NS>), grd, dim3(256), lds, st, *g, lo[0], lo[1], lo[2], nj, nk,    \
                        count, static_cast<const T *>(ea[0]), static_cast<const T *>(ea[1]),                    \
                        static_cast<const T *>(ea[2]), static_cast<T *>(eb[0]), static_cast<T *>(eb[1]),        \
-                       static_cast<T *>(eb[2]), *m, ncache, n_adec)
+                       static_cast<T *>(eb[2]), *m, ncache)
     if (m->n_slots <= 1) RF_ADE_PASS(1);
     else if (m->n_slots == 2) RF_ADE_PASS(2);
     else if (m->n_slots <= 4) RF_ADE_PASS(4);
@@ -1203,38 +1204,79 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
     const int cap = (int)((40 * 1024 - a->n_adec * ade_sz) / (prec == 0 ? coef_bytes<double>() : coef_bytes<float>()));
     a->ncache = a->tag ? (a->n_entries < cap ? a->n_entries : cap) : 0;
     if (planes <= 0 || jn <= 0 || kn <= 0) {   /* empty box: nothing to launch */
-        a->ntk = a->ntk0 = a->ntj = a->n_items = a->blocks = 0;
+        a->ntk = a->ntk0 = a->ntj = a->n_parts = a->n_items = 0;
         a->chunk = 1;
         return 0;
     }
     a->ntk0 = (a->kbrk - a->ib0[2] + KW - 1) / KW;
     a->ntk = a->ntk0 + (a->ib1[2] - a->kbrk + KW - 1) / KW;
     a->ntj = (int)((jn + rows - 1) / rows);
+    // z tiles whose halo lane or owned lanes lie in the z CPML (outside the
+    // plain range) need the z CPML code; with the slabs at both ends of the
+    // z range they are a prefix [0, zt0) and a suffix [zt1, ntk)
+    int zt0 = 0, zt1 = a->ntk;
+    if (a->zc && !a->zsplit) zt0 = zt1 = 0;   // one part with the z CPML code
+    if (a->zc && a->zsplit) {
+        const int nz = a->g.n[2];
+        auto slab_tile = [&](int tk) {
+            const int k0 = tk < a->ntk0 ? a->ib0[2] + tk * KW : a->kbrk + (tk - a->ntk0) * KW;
+            const int kend = tk < a->ntk0 ? (k0 + KW < a->kbrk ? k0 + KW : a->kbrk) : a->ib1[2];
+            for (int kt = k0 - 1; kt < kend; kt++) {
+                const int kc = kt < 0 ? 0 : (kt > nz ? nz : kt);
+                if (kc < a->zplo || kc >= a->zphi) return true;
+            }
+            return false;
+        };
+        while (zt0 < a->ntk && slab_tile(zt0)) zt0++;
+        while (zt1 > zt0 && slab_tile(zt1 - 1)) zt1--;
+        for (int tk = zt0; tk < zt1; tk++)
+            if (slab_tile(tk)) zt0 = zt1 = 0;   // not of that shape: one part with the z CPML code
+    }
     const long long tiles = (long long)a->ntk * a->ntj;
-    // resident wavefronts of the variant this launch selects
+    int np = 0;
+    if (zt1 > zt0) {   // tiles without z CPML lanes
+        a->ptk[np][0] = zt0;
+        a->ptk[np][1] = zt1 - zt0;
+        a->ptk[np][2] = a->ptk[np][3] = 0;
+        a->pzc[np++] = 0;
+    }
+    if (zt0 > 0 || zt1 < a->ntk) {
+        a->ptk[np][0] = 0;
+        a->ptk[np][1] = zt0;
+        a->ptk[np][2] = zt1;
+        a->ptk[np][3] = a->ntk - zt1;
+        a->pzc[np++] = 1;
+    }
+    a->n_parts = np;
+    // resident wavefronts of each part's variant
     static long long cache_key[8];
     static int cache_val[8], cache_n = 0;
     const size_t lds = prec == 0 ? leap_lds<double>(a) : leap_lds<float>(a);
-    const long long key = ((long long)lds << 16) | (prec << 12) | (a->tag_bytes << 8) | (a->zc << 7) | (rows << 3);
-    int resident = 0;
-    for (int q = 0; q < cache_n; q++)
-        if (cache_key[q] == key) resident = cache_val[q];
-    if (!resident) {
-        resident = leap_capacity(prec, a);
-        if (resident <= 0) resident = 4096;
-        if (cache_n < 8) {
-            cache_key[cache_n] = key;
-            cache_val[cache_n++] = resident;
+    int resident[2] = {0, 0};
+    for (int p = 0; p < np; p++) {
+        const long long key =
+            ((long long)lds << 16) | (prec << 12) | (a->tag_bytes << 8) | (a->pzc[p] << 7) | (rows << 3);
+        for (int q = 0; q < cache_n; q++)
+            if (cache_key[q] == key) resident[p] = cache_val[q];
+        if (!resident[p]) {
+            rf_leap_t v = *a;
+            v.zc = a->pzc[p];
+            resident[p] = leap_capacity(prec, &v);
+            if (resident[p] <= 0) resident[p] = 4096;
+            if (cache_n < 8) {
+                cache_key[cache_n] = key;
+                cache_val[cache_n++] = resident[p];
+            }
         }
+        if (target_waves > 0) resident[p] = target_waves;
     }
-    if (target_waves > 0) resident = target_waves;
     if (chunk <= 0) {
         // about twelve items per resident wavefront, at least 8 planes
         // each: short items keep the wavefronts of an XCD on neighbouring
         // tiles at nearby planes (their halo rows meet in its L2) and leave
         // a short tail when the queue drains; each item recomputes one
         // warm-up plane
-        long long chunks = (12LL * resident + tiles - 1) / tiles;
+        long long chunks = (12LL * resident[0] + tiles - 1) / tiles;
         if (chunks < 1) chunks = 1;
         chunk = (int)((planes + chunks - 1) / chunks);
         if (chunk < 8) chunk = 8;
@@ -1242,37 +1284,51 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
     if (chunk > planes) chunk = (int)planes;
     a->chunk = chunk;
     const long long nch = (planes + chunk - 1) / chunk;
-    const long long items = tiles * nch;
-    if (items > (1LL << 30)) return -1;
-    a->n_items = (int)items;
-    long long blocks = (resident + WPB - 1) / WPB;
-    const long long need = (items + WPB - 1) / WPB;
-    if (blocks > need) blocks = need;
-    blocks = (blocks + N_XCD - 1) / N_XCD * N_XCD;
-    a->blocks = (int)blocks;
+    for (int p = 0; p < np; p++) {
+        const long long items = (long long)(a->ptk[p][1] + a->ptk[p][3]) * a->ntj * nch;
+        if (items > (1LL << 30)) return -1;
+        a->pitems[p] = (int)items;
+        long long blocks = (resident[p] + WPB - 1) / WPB;
+        const long long need = (items + WPB - 1) / WPB;
+        if (blocks > need) blocks = need;
+        a->pblocks[p] = (int)((blocks + N_XCD - 1) / N_XCD * N_XCD);
+    }
     return 0;
 }
 
 extern "C" int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s) {
-    if (a->n_items == 0) return 0;
-    if (a->n_items < 0 || a->ntk <= 0 || a->ntj <= 0 || a->chunk <= 0 || a->blocks <= 0 || !a->ctr) return -1;
+    if (a->n_parts == 0) return 0;
+    if (a->n_parts < 0 || a->n_parts > 2 || a->ntk <= 0 || a->ntj <= 0 || a->chunk <= 0 || !a->ctr) return -1;
     if (a->n_srcs > 0 && !a->srcs) return -1;
     if (a->tag && !a->cold) return -1;
     if (a->zc && (!a->zpsi || a->zlen <= 0)) return -1;
     hipStream_t st = rf_raw_stream(s);
-    const unsigned blocks = static_cast<unsigned>(a->blocks);
-    if (prec == 0) {
-        switch (a->rows) {
-        case 2: return launch_rows<double, 2>(a, blocks, st);
-        case 4: return launch_rows<double, 4>(a, blocks, st);
+    for (int p = 0; p < a->n_parts; p++) {
+        const int *r = a->ptk[p];
+        if (a->pitems[p] <= 0 || a->pblocks[p] <= 0 || r[1] < 0 || r[3] < 0 || r[1] + r[3] <= 0 || r[0] < 0 ||
+            r[2] < 0 || r[0] + r[1] > a->ntk || r[2] + r[3] > a->ntk || (a->pzc[p] && !a->zc))
+            return -1;
+        rf_leap_t b = *a;
+        for (int q = 0; q < 4; q++) b.tkr[q] = r[q];
+        b.zc = a->pzc[p];
+        b.n_items = a->pitems[p];
+        b.ctr = a->ctr + 16 * p;
+        const unsigned blocks = static_cast<unsigned>(a->pblocks[p]);
+        int rc = -1;
+        if (prec == 0) {
+            switch (a->rows) {
+            case 2: rc = launch_rows<double, 2>(&b, blocks, st); break;
+            case 4: rc = launch_rows<double, 4>(&b, blocks, st); break;
+            }
+        } else {
+            switch (a->rows) {
+            case 4: rc = launch_rows<float, 4>(&b, blocks, st); break;
+            case 8: rc = launch_rows<float, 8>(&b, blocks, st); break;
+            }
         }
-        return -1;
+        if (rc) return rc;
     }
-    switch (a->rows) {
-    case 4: return launch_rows<float, 4>(a, blocks, st);
-    case 8: return launch_rows<float, 8>(a, blocks, st);
-    }
-    return -1;
+    return 0;
 }
 
 extern "C" int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const fa[6],
@@ -1294,7 +1350,9 @@ extern "C" int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3],
         count *= hi[q] - lo[q];
     }
     if (count == 0) return 0;
-    if (count >= (1LL << 31) || !mat->tag || !mat->J || mat->n_slots < 1 || mat->n_slots > RF_MAX_SLOTS) return -1;
+    if (count >= (1LL << 31) || !mat->tag || !mat->J || mat->n_slots < 1 || mat->n_slots > RF_MAX_SLOTS ||
+        mat->n_ade < 1 || mat->n_ade > ADE_PASS_MAX_SETS)
+        return -1;
     hipStream_t st = rf_raw_stream(s);
     return prec == 0 ? ade_pass_t<double>(g, lo, hi, ea, eb, mat, st) : ade_pass_t<float>(g, lo, hi, ea, eb, mat, st);
 }

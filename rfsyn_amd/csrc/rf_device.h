@@ -218,23 +218,36 @@ typedef struct {
     const long long *d_step;
     /* CPML along z inside the box: plain range, psi slab [0,zlo) u [zhi,nz],
      * the z profiles of rf_cpml_t (kinv_h b_h a_h kinv_m b_m a_m) and the
-     * z-psi store: 6 values per (i, j, compressed k) node, H of hx and hy
-     * in slots 2*zpar (read) and 2-2*zpar (written), then E of ex and ey */
+     * z-psi store: for each (i, j) six slot rows of zlen values (compressed
+     * k fastest), value (i, j, s, q) at ((i (ny+1) + j) 6 + s) zlen + q; H of
+     * hx and hy in slots 2*zpar (read) and 2-2*zpar (written), then E of ex
+     * and ey in slots 4 and 5 */
     int zc;
     int zplo, zphi, zlo, zhi, zlen;
     const void *zk[6];
     void *zpsi;
     long long zbytes;           /* size of the z-psi store (< 2^32) */
     int zpar;
-    /* work queue: 2 x 8 zero-initialised device counters, the set
-     * qpar used by this launch (alternate it from launch to launch) */
+    /* work queue: 4 x 8 zero-initialised device counters; launch part p
+     * uses the set 2 p + qpar (alternate qpar from step to step) */
     unsigned *ctr;
     int qpar;
     /* tiling, filled by rf_leap_plan */
     int rows, chunk;            /* rows per item, x planes per item */
     int ntk, ntk0, ntj;         /* tiles along z (ntk0 below kbrk) and y */
-    int n_items, blocks;        /* work items; blocks launched (resident) */
     int ncache, n_adec;         /* material entries and ADE sets staged in LDS */
+    /* the step runs in n_parts launches over disjoint sets of z tiles:
+     * part p takes the tiles ptk[p][0] + [0, ptk[p][1]) and
+     * ptk[p][2] + [0, ptk[p][3]), with the z CPML code if pzc[p], in
+     * pitems[p] work items on pblocks[p] blocks; with zsplit set (input)
+     * the tiles with no lane in the z CPML form a part of their own that
+     * runs the variant without it */
+    int zsplit;
+    int n_parts;
+    int ptk[2][4], pzc[2], pitems[2], pblocks[2];
+    /* the part a launch runs (set by rf_launch_leapfrog) */
+    int tkr[4];
+    int n_items;
 } rf_leap_t;
 
 /* Tiling of the fused step over a->ib0..ib1; rows: 2 or 4 (FP64), 4 or 8
@@ -281,7 +294,8 @@ int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *
 /* ADE pass of the split step: for the nodes of [lo, hi) (inside the ADE
  * state box and the fused box), every dispersive edge whose E^{n+1} in eb
  * holds the plain part Ca e^n + Cb (dA - dB) of its update is completed
- * with E^n from ea, and J, P advance (ade_edge, yee_device.h). */
+ * with E^n from ea, and J, P advance (ade_edge, yee_device.h).  At most
+ * 64 ADE sets and 1 <= n_slots. */
 int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3],
                        void *const eb[3], const rf_mat_t *mat, rf_stream_t *s);
 

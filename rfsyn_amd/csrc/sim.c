@@ -108,6 +108,7 @@ struct fdtd_sim {
     int parity;
     void *f2[6];
     int leap_rows, leap_chunk, leap_waves;   /* tiling requests (0: automatic) */
+    int leap_zsplit;         /* z CPML tiles in a launch of their own */
     rf_leap_src_t *d_srcs;   /* device source table of the fused step */
     rf_leap_cold_t *d_cold;  /* its cold data (ADE) */
     int ib0[3], ib1[3];      /* node box of the fused kernel */
@@ -474,11 +475,12 @@ static int finalize(fdtd_sim_t *s) {
         }
         if (s->zc && dalloc(s, &s->zpsi, (s->nx + 1) * (s->ny + 1) * (size_t)s->cp.d.len[2] * 6 * s->es))
             return -1;
-        if (dalloc(s, (void **)&s->d_ctr, 16 * sizeof(unsigned))) return -1;
+        if (dalloc(s, (void **)&s->d_ctr, 32 * sizeof(unsigned))) return -1;
         const char *e;
         s->leap_rows = (e = getenv("RFSYN_LEAP_ROWS")) ? atoi(e) : 0;
         s->leap_chunk = (e = getenv("RFSYN_LEAP_CHUNK")) ? atoi(e) : 0;
         s->leap_waves = (e = getenv("RFSYN_LEAP_WAVES")) ? atoi(e) : 0;
+        s->leap_zsplit = (e = getenv("RFSYN_LEAP_ZSPLIT")) ? atoi(e) != 0 : 0;
         s->ade_split_off = (e = getenv("RFSYN_ADE_SPLIT")) && atoi(e) == 0;
     }
     int P = (int)s->n_probes;
@@ -672,7 +674,9 @@ static int mark(fdtd_sim_t *s, int q) {
  * material, or a source inside the fused kernel may touch a dispersive
  * node (its value enters the update before or after the ADE terms). */
 static int ade_pass_box(const fdtd_sim_t *s, int lo[3], int hi[3]) {
-    if (!s->has_mat || s->ade_split_off || s->mat.d.n_ade <= 0 || !s->mat.d.J) return 0;
+    if (!s->has_mat || s->ade_split_off || s->mat.d.n_ade <= 0 || s->mat.d.n_ade > 64 || !s->mat.d.J ||
+        s->mat.d.n_slots < 1)
+        return 0;
     const rf_mat_t *m = &s->mat.d;
     int meets = s->n_srcs > 0;
     for (int a = 0; a < 3; a++) {
@@ -737,6 +741,7 @@ static int enqueue_fused(fdtd_sim_t *s) {
     a.n_srcs = s->n_srcs;
     memcpy(a.src_box, s->src_box, sizeof a.src_box);
     a.d_step = s->d_step;
+    a.zsplit = s->leap_zsplit;
     int alo[3], ahi[3];
     a.ade_split = ade_pass_box(s, alo, ahi);
     if (rf_leap_plan(s->prec, &a, s->leap_rows, s->leap_chunk, s->leap_waves)) {
