@@ -17,8 +17,14 @@ average event-timed duration, against the 8 TB/s HBM3E peak.  The bytes
 follow the two-sweep accounting of a Yee step (18 values per cell, a
 material tag, 8 psi values per cell and CPML axis, 6 per Drude cell, 9 per
 Lorentz pole and cell, 6 per tensor cell) over the cells the launch
-updates; "step" gives the same accounting for the whole grid against the
-wall time per step.
+updates; on the fused step the dominant kernel is the leapfrog launch over
+its box, whose dispersive state moves in the split ADE pass (reported
+beside it with its own bytes).  "step" gives the survey accounting for the
+whole grid against the wall time per step.  "traffic" is the HBM bytes per
+leapfrog launch measured with rocprofv3 counters (FETCH_SIZE, WRITE_SIZE)
+and scaled by what the same counters read for a copy of known size; it is
+read from the committed profile summary of the same workload and dtype
+(profiles/), or null when there is none.
 
 CPU baseline: c1 runs the rfsyn reference compiled from its own sources
 (oracle/_ref, kind "reference") on one pinned core over the same
@@ -190,9 +196,21 @@ class C3:
         # J and P per pole per component read+write (24 values)
         self.ade_values = 6 * drude_cells + 24 * lorentz_cells
         self.drude_cells, self.lorentz_pole_cells = drude_cells, 2 * lorentz_cells
+        self.lorentz_cells = lorentz_cells
+        # ADE pass box: the ADE state box (grating to the end of the Lorentz
+        # layer, all x and y)
+        self.ade_box_cells = nxy * (lay["lorentz"][1] - lay["grating"][0])
         self.config = {"workload": "c3: 512x512x256 periodic Drude/Lorentz stack, 16-cell z CPML, "
                                    "2 plane monitors x 128 freqs, 6000 steps",
                        "grid": list(self.n), "dx_m": 0.5e-3, "courant": 0.99, "layers": lay}
+
+    def ade_pass_bytes(self, s):
+        """Algorithmic bytes of the split ADE pass: a tag per node of its
+        box; per dispersive node E^n and the plain part read and E^{n+1}
+        written (3 components); J read and written (Drude, 3 components);
+        J and P read and written per Lorentz pole (3 components)."""
+        return (self.tag_bytes * self.ade_box_cells + s * 9 * (self.drude_cells + self.lorentz_cells)
+                + s * 6 * self.drude_cells + s * 12 * self.lorentz_pole_cells)
 
     def build(self, lib, prec, td_cap):
         run, pids, mons, _ = build_c3(lib, prec, td_cap, self.n)
@@ -272,6 +290,35 @@ def cpu_baseline(wl, budget_s=15.0):
                       f"(host has {os.cpu_count()} cpus)"}
 
 
+def copy_calibration():
+    import torch
+    x = torch.ones((256 << 20) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copied 256 MiB x 5", file=sys.stderr)
+
+
+def profile_traffic(workload, dtype):
+    """HBM bytes per leapfrog launch from the committed profile summary of
+    this workload and dtype (lines 'workload: ...', 'dtype: ...' and
+    'traffic_bytes_per_launch: N'), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*.md"))):
+        kv = {}
+        for line in open(path):
+            if ":" in line and not line.startswith((" ", "\t", "|", "`")):
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        if kv.get("workload") == workload and kv.get("dtype") == dtype and "traffic_bytes_per_launch" in kv:
+            try:
+                return int(kv["traffic_bytes_per_launch"])
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,7 +328,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device 5 times (the known-size kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is designed but not built", file=sys.stderr)
@@ -300,7 +353,7 @@ def main():
         t0 = time.perf_counter()
         run.run(args.steps)
         wall = time.perf_counter() - t0
-        h_ms, e_ms, nk = run.kernel_times()
+        (h_ms, e_ms, sh_ms), nk = run.kernel_times3()
         fused = lib.fdtd_sim_fused(run.sim) == 1
         box = None
         if fused:
@@ -309,12 +362,13 @@ def main():
             box = [(lo[q], hi[q]) for q in range(3)]
         run.set_profiling(False)
         print(f"[bench] {args.workload} {tag}: {wall / args.steps * 1e3:.4f} ms/step, "
-              f"H {h_ms / nk * 1e3:.1f} us, E {e_ms / nk * 1e3:.1f} us", file=sys.stderr, flush=True)
+              f"H/fused {h_ms / nk * 1e3:.1f} us, E/ADE pass {e_ms / nk * 1e3:.1f} us, "
+              f"shell {sh_ms / nk * 1e3:.1f} us", file=sys.stderr, flush=True)
         if total > timed_steps:
             run.run(total - timed_steps)
         dft = np.stack([run.dft(p) for p in pids])
         run.close()
-        res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, dft=dft, fused=fused, box=box)
+        res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, sh_ms=sh_ms / nk, dft=dft, fused=fused, box=box)
 
     n = wl.n
     cells = n[0] * n[1] * n[2]
@@ -324,14 +378,20 @@ def main():
     drude = getattr(wl, "drude_cells", 0)
     lorentz = getattr(wl, "lorentz_pole_cells", 0)
     if r["fused"]:
-        # the fused launch on its box (H slot of the events); the shell
-        # kernels around it (x/y CPML slabs) are timed in the E slot.  Its
-        # algorithmic bytes: the two-sweep accounting of one step over the
-        # cells of the box (dispersive cells all lie inside it)
+        # the fused launch on its box (first event slot): the two-sweep
+        # accounting of one step over the cells of the box without the ADE
+        # state, which the split ADE pass (second slot) moves; the shell
+        # kernels around the box are the third slot
         box = r["box"]
-        byt = step_bytes(box_cells(box, n), box_slab_cells(box, n, wl.npml), s, wl.tag_bytes, drude, lorentz)
+        ade = r["e_ms"] > 0 and (drude or lorentz)
+        byt = step_bytes(box_cells(box, n), box_slab_cells(box, n, wl.npml), s, wl.tag_bytes,
+                         0 if ade else drude, 0 if ade else lorentz)
         dom, dur = "leapfrog", r["h_ms"]
-        detail = {"box": box, "box_cells": box_cells(box, n), "shell_avg_launch_us": round(r["e_ms"] * 1e3, 3)}
+        detail = {"box": box, "box_cells": box_cells(box, n), "shell_avg_launch_us": round(r["sh_ms"] * 1e3, 3)}
+        if ade and hasattr(wl, "ade_pass_bytes"):
+            ab = wl.ade_pass_bytes(s)
+            detail["ade_pass"] = {"bytes_per_launch": ab, "avg_launch_us": round(r["e_ms"] * 1e3, 3),
+                                  "achieved": round(ab / (r["e_ms"] * 1e-3) / 1e9, 2), "unit": "GB/s"}
     else:
         hb, eb = kernel_bytes(n, s, wl.npml, wl.tag_bytes, wl.ade_values)
         dom, dur, byt = ("update_e", r["e_ms"], eb) if r["e_ms"] >= r["h_ms"] else ("update_h", r["h_ms"], hb)
@@ -359,7 +419,9 @@ def main():
         "data": "synthetic",
         "config": wl.config,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profile_traffic(args.workload, "f64" if wl.headline == capi.FDTD_FP64 else "f32")
+                     if r["fused"] else None,
                      "bytes_per_launch": byt, "avg_launch_us": round(dur * 1e3, 3), **detail},
         "step": {"bytes_per_step": full, "achieved_gbs": round(step_gbs, 2),
                  "frac": round(step_gbs / HBM_PEAK_GBS, 4)},

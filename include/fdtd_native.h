@@ -101,16 +101,21 @@ int fdtd_sim_upload(fdtd_sim_t *s, const fdtd_grid_t *g);
 int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic);
 
 /* Profiling mode: on != 0 launches every kernel directly (no graph
- * replay) and times the H and E kernels with HIP events on the handle's
- * stream.  fdtd_sim_kernel_times returns the accumulated milliseconds and
- * the number of timed steps since the previous call, then resets them. */
+ * replay) and times the step's kernels with HIP events on the handle's
+ * stream in three slots: the H kernel (fused step: the leapfrog
+ * launches), the E kernel (fused step: the split ADE pass) and, on the
+ * fused step, the shell kernels.  fdtd_sim_kernel_times returns the
+ * accumulated milliseconds of slot 0 and of slots 1 + 2 and the number of
+ * timed steps since the previous call, then resets them;
+ * fdtd_sim_kernel_times3 returns the three slots. */
 int fdtd_sim_set_profiling(fdtd_sim_t *s, int on);
 int fdtd_sim_kernel_times(fdtd_sim_t *s, double *h_ms, double *e_ms, long *steps);
+int fdtd_sim_kernel_times3(fdtd_sim_t *s, double ms[3], long *steps);
 
 /* 1 when the run uses the fused leapfrog step (H, sources and E in one
- * pass over double-buffered fields on the CPML-free box, timed as h_ms,
- * and node-wise shell kernels around it, timed as e_ms), 0 for separate
- * H and E kernels; decided at the first run. */
+ * pass over double-buffered fields on the box inside the x/y CPML, the
+ * split ADE pass after it and node-wise shell kernels around the box),
+ * 0 for separate H and E kernels; decided at the first run. */
 int fdtd_sim_fused(const fdtd_sim_t *s);
 /* The node box [lo, hi) per axis that the fused kernel owns (the CPML
  * plain region with a one-node margin); the nodes around it are updated

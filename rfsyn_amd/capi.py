@@ -212,6 +212,7 @@ EXTENSION_API = {
     "fdtd_sim_device_bytes": (_sz, [_vp]),
     "fdtd_sim_set_profiling": (C.c_int, [_vp, C.c_int]),
     "fdtd_sim_kernel_times": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_long)]),
+    "fdtd_sim_kernel_times3": (C.c_int, [_vp, _dp, C.POINTER(C.c_long)]),
     "fdtd_sim_fused": (C.c_int, [_vp]),
     "fdtd_sim_fused_box": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fdtd_last_error": (C.c_char_p, []),

@@ -127,7 +127,7 @@ struct fdtd_sim {
     int profiling;
     rf_event_t **ev;
     size_t n_ev, ev_used;
-    double prof_h_ms, prof_e_ms;
+    double prof_ms[3];       /* event slots: H / fused, E / ADE pass, shell */
     long prof_launches;
 };
 
@@ -689,8 +689,8 @@ static int ade_pass_box(const fdtd_sim_t *s, int lo[3], int hi[3]) {
 }
 
 /* The fused step: the leapfrog kernel on the plain box (timed in event
- * slots 0-1), then the split ADE pass and the shell kernels around the
- * box (slots 2-3). */
+ * slots 0-1), then the split ADE pass (slots 2-3) and the shell kernels
+ * around the box (slots 4-5). */
 static int enqueue_fused(fdtd_sim_t *s) {
     rf_leap_t a;
     memset(&a, 0, sizeof a);
@@ -750,6 +750,7 @@ static int enqueue_fused(fdtd_sim_t *s) {
     }
     if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
     if (a.ade_split && rf_launch_ade_pass(s->prec, &s->geom, alo, ahi, s->f, s->f2, m, s->st)) return -1;
+    if (mark(s, 3) || mark(s, 4)) return -1;
     if (s->shell.n) {
         const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
         rf_zpsi_t z = {s->zpsi, s->zpar, s->zc ? s->cp.d.len[2] : 0};
@@ -759,7 +760,7 @@ static int enqueue_fused(fdtd_sim_t *s) {
                               s->n_srcs, s->d_step, s->st))
             return -1;
     }
-    if (mark(s, 3)) return -1;
+    if (mark(s, 5)) return -1;
     /* B now holds state n+1 */
     for (int q = 0; q < 6; q++) {
         void *t = s->f[q];
@@ -781,7 +782,7 @@ static int enqueue_step(fdtd_sim_t *s) {
     double cb = s->dt / (MU0 * s->dx);
     if (s->fused) {
         if (enqueue_fused(s)) return -1;
-        if (s->profiling) s->ev_used += 4;
+        if (s->profiling) s->ev_used += 6;
         if (s->has_mat && s->mat.n_tensor &&
             rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
             return -1;
@@ -802,8 +803,8 @@ static int enqueue_step(fdtd_sim_t *s) {
                 return -1;
         if (mark(s, 2)) return -1;
         if (rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) return -1;
-        if (mark(s, 3)) return -1;
-        if (s->profiling) s->ev_used += 4;
+        if (mark(s, 3) || mark(s, 4) || mark(s, 5)) return -1;
+        if (s->profiling) s->ev_used += 6;
         if (s->has_mat && s->mat.n_tensor &&
             rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
             return -1;
@@ -858,12 +859,12 @@ static int capture_block(fdtd_sim_t *s) {
 /* Folds the recorded event pairs into the totals (synchronises). */
 static int prof_collect(fdtd_sim_t *s) {
     if (rf_sync(s->st)) return -1;
-    for (size_t q = 0; q + 4 <= s->ev_used; q += 4) {
-        float h = 0.f, e = 0.f;
-        if (rf_event_elapsed_ms(s->ev[q], s->ev[q + 1], &h) || rf_event_elapsed_ms(s->ev[q + 2], s->ev[q + 3], &e))
-            return -1;
-        s->prof_h_ms += h;
-        s->prof_e_ms += e;
+    for (size_t q = 0; q + 6 <= s->ev_used; q += 6) {
+        for (int w = 0; w < 3; w++) {
+            float ms = 0.f;
+            if (rf_event_elapsed_ms(s->ev[q + 2 * w], s->ev[q + 2 * w + 1], &ms)) return -1;
+            s->prof_ms[w] += ms;
+        }
         s->prof_launches++;
     }
     s->ev_used = 0;
@@ -873,7 +874,7 @@ static int prof_collect(fdtd_sim_t *s) {
 int fdtd_sim_set_profiling(fdtd_sim_t *s, int on) {
     if (!s) return -1;
     if (on && !s->ev) {
-        s->n_ev = 4 * 256;
+        s->n_ev = 6 * 256;
         s->ev = calloc(s->n_ev, sizeof *s->ev);
         if (!s->ev) return -1;
         for (size_t q = 0; q < s->n_ev; q++)
@@ -886,10 +887,19 @@ int fdtd_sim_set_profiling(fdtd_sim_t *s, int on) {
 
 int fdtd_sim_kernel_times(fdtd_sim_t *s, double *h_ms, double *e_ms, long *launches) {
     if (!s || !h_ms || !e_ms || !launches) return -1;
-    *h_ms = s->prof_h_ms;
-    *e_ms = s->prof_e_ms;
+    *h_ms = s->prof_ms[0];
+    *e_ms = s->prof_ms[1] + s->prof_ms[2];
     *launches = s->prof_launches;
-    s->prof_h_ms = s->prof_e_ms = 0.0;
+    memset(s->prof_ms, 0, sizeof s->prof_ms);
+    s->prof_launches = 0;
+    return 0;
+}
+
+int fdtd_sim_kernel_times3(fdtd_sim_t *s, double ms[3], long *launches) {
+    if (!s || !ms || !launches) return -1;
+    for (int w = 0; w < 3; w++) ms[w] = s->prof_ms[w];
+    *launches = s->prof_launches;
+    memset(s->prof_ms, 0, sizeof s->prof_ms);
     s->prof_launches = 0;
     return 0;
 }
@@ -906,7 +916,7 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
     }
     size_t left = n_steps;
     while (left) {
-        if (s->profiling && s->ev_used + 4 > s->n_ev && prof_collect(s) != 0) return -1;
+        if (s->profiling && s->ev_used + 6 > s->n_ev && prof_collect(s) != 0) return -1;
         if (!s->profiling && s->pending == 0 && left >= (size_t)s->K) {
             if (!s->graphs[s->parity] && capture_block(s) != 0) {
                 rf_set_error("graph capture: %s", rf_error_string());

@@ -204,6 +204,14 @@ class NativeRun:
         assert self.lib.fdtd_sim_kernel_times(self.sim, C.byref(h), C.byref(e), C.byref(n)) == 0
         return h.value, e.value, n.value
 
+    def kernel_times3(self):
+        """(ms per event slot: H / fused, E / ADE pass, shell; timed steps)
+        since the previous call."""
+        ms = (C.c_double * 3)()
+        n = C.c_long()
+        assert self.lib.fdtd_sim_kernel_times3(self.sim, ms, C.byref(n)) == 0
+        return list(ms), n.value
+
     def energy(self):
         e = C.c_double()
         h = C.c_double()
