@@ -100,6 +100,30 @@ int fdtd_sim_upload(fdtd_sim_t *s, const fdtd_grid_t *g);
  * reduction order (eps from the scalar coefficient of each edge). */
 int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic);
 
+/* Energy audit of step n = steps done - 1 (audit_periodic_energy.c:43-71,
+ * extension of the reference's host-side audit): from E^n, H^{n-1/2} and
+ * H^{n+1/2}, out[0] = 0.5 sum eps (E^n)^2 dV, out[1] = 0.5 sum mu0
+ * (H^{n-1/2})^2 dV, out[2] = 0.5 sum mu0 (H^{n+1/2})^2 dV, out[3] = 0.5 sum
+ * mu0 ((H^{n-1/2} + H^{n+1/2})/2)^2 dV, out[4] = 0.5 sum mu0 H^{n-1/2}
+ * H^{n+1/2} dV, over the cells fdtd_sim_energy sums.  out[0] + out[4] is
+ * the modified energy that a closed lossless grid conserves to rounding.
+ * Needs the fused step and at least one completed step; *step (may be
+ * NULL) receives n. */
+int fdtd_sim_energy_audit(fdtd_sim_t *s, double out[5], size_t *step);
+
+/* Poynting flux plane (extension; the reference integrates E x H at single
+ * probe points in validation_common.h): the plane of E nodes at `index`
+ * (< n) along `axis`.  Each step n -> n+1 adds dt dx^2 times the sum over
+ * the plane's cells of (0.5 (E^n + E^{n+1}))_t x (H^{n+1/2})_t along +axis,
+ * with the tangential E at the nodes of the plane and the tangential H at
+ * the same array index (half a cell above it): with this pairing the
+ * change of the modified energy between two planes equals the difference
+ * of their fluxes (the discrete Poynting theorem).  Add before the first
+ * run; needs the fused step.  Returns the plane id or -1.
+ * fdtd_sim_flux returns the time integral so far in joules. */
+int fdtd_sim_add_flux_plane(fdtd_sim_t *s, int axis, size_t index);
+int fdtd_sim_flux(fdtd_sim_t *s, int plane, double *energy);
+
 /* Profiling mode: on != 0 launches every kernel directly (no graph
  * replay) and times the step's kernels with HIP events on the handle's
  * stream in three slots: the H kernel (fused step: the leapfrog

@@ -113,6 +113,12 @@ struct fdtd_sim {
     oplane_t *planes; size_t n_planes;
     oprobe_t *probes; size_t n_probes;
     omon_t *mons; size_t n_mons;
+    /* state n kept beside state n+1 for the energy audit and flux planes:
+     * E^n, H^{n-1/2} copied at the start of every step */
+    int track;
+    double *prev[6];
+    int (*flux)[2]; size_t n_flux;
+    double *flux_acc;
 };
 
 static cpml_t *clone_cpml(const cpml_t *p, const fdtd_grid_t *g) {
@@ -178,6 +184,8 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
         for (int w = 0; w < 2; w++) { free(s->mons[p].re[w]); free(s->mons[p].im[w]); }
     }
     free(s->probes); free(s->mons); free(s->pts); free(s->planes);
+    for (int q = 0; q < 6; q++) free(s->prev[q]);
+    free(s->flux); free(s->flux_acc);
     materials_destroy(s->mat);
     cpml_destroy(s->pml);
     fdtd_grid_destroy(s->g);
@@ -284,11 +292,73 @@ static void dft_add(kahan_t *re, kahan_t *im, double v, double w, double t, doub
     else { re->s += tr; im->s -= ti; }
 }
 
+/* Flux along +axis through the plane of E nodes at index p between state
+ * n (prev) and n+1 (grid): sum over the plane's cells of
+ * 0.5 (E_b^n + E_b^{n+1}) H_c^{n+1/2} - 0.5 (E_c^n + E_c^{n+1}) H_b^{n+1/2}
+ * with (a, b, c) cyclic; unscaled by dx^2 dt. */
+static double plane_flux(const fdtd_sim_t *s, int a, size_t p) {
+    const fdtd_grid_t *g = s->g;
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const size_t n[3] = {g->nx, g->ny, g->nz};
+    const double *en[3] = {s->prev[0], s->prev[1], s->prev[2]};
+    const double *ep[3] = {g->ex, g->ey, g->ez};
+    const double *hp[3] = {g->hx, g->hy, g->hz};
+    double sum = 0.0;
+    size_t x[3];
+    x[a] = p;
+    for (x[b] = 0; x[b] < n[b]; x[b]++)
+        for (x[c] = 0; x[c] < n[c]; x[c]++) {
+            const size_t idx = fdtd_index(g, x[0], x[1], x[2]);
+            const double eb = 0.5 * (en[b][idx] + ep[b][idx]), ec = 0.5 * (en[c][idx] + ep[c][idx]);
+            sum += eb * hp[c][idx] - ec * hp[b][idx];
+        }
+    return sum;
+}
+
+int fdtd_sim_add_flux_plane(fdtd_sim_t *s, int axis, size_t index) {
+    if (!s || axis < 0 || axis > 2 || s->steps) return -1;
+    const size_t n[3] = {s->g->nx, s->g->ny, s->g->nz};
+    if (index >= n[axis]) return -1;
+    int (*f)[2] = realloc(s->flux, (s->n_flux + 1) * sizeof *f);
+    double *acc = realloc(s->flux_acc, (s->n_flux + 1) * sizeof *acc);
+    if (f) s->flux = f;
+    if (acc) s->flux_acc = acc;
+    if (!f || !acc) return -1;
+    s->flux[s->n_flux][0] = axis;
+    s->flux[s->n_flux][1] = (int)index;
+    s->flux_acc[s->n_flux] = 0.0;
+    return (int)s->n_flux++;
+}
+
+int fdtd_sim_flux(fdtd_sim_t *s, int plane, double *energy) {
+    if (!s || !energy || plane < 0 || (size_t)plane >= s->n_flux) return -1;
+    *energy = s->flux_acc[plane] * s->g->dx * s->g->dx * s->g->dt;
+    return 0;
+}
+
+/* The state is tracked when flux planes exist or the grid is small (the
+ * audit of large grids is left to the device library). */
+static int start_tracking(fdtd_sim_t *s) {
+    const fdtd_grid_t *g = s->g;
+    const size_t total = (g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+    if (s->track || !(s->n_flux || g->nx * g->ny * g->nz <= ((size_t)1 << 22))) return 0;
+    for (int q = 0; q < 6; q++)
+        if (!(s->prev[q] = malloc(total * sizeof(double)))) return -1;
+    s->track = 1;
+    return 0;
+}
+
 int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
     if (!s) return -1;
     fdtd_grid_t *g = s->g;
+    if (start_tracking(s)) return -1;
+    const size_t bytes = (g->nx + 1) * (g->ny + 1) * (g->nz + 1) * sizeof(double);
     for (size_t it = 0; it < n_steps; it++) {
         size_t n = s->steps;
+        if (s->track) {
+            double *cur[6] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz};
+            for (int q = 0; q < 6; q++) memcpy(s->prev[q], cur[q], bytes);
+        }
         double t = (double)n * g->dt;
         double tp = (double)(n + 1) * g->dt;
         if (s->pml) fdtd_update_h_cpml(g, s->pml); else fdtd_update_h(g);
@@ -323,6 +393,7 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
                         }
                 }
         }
+        for (size_t q = 0; q < s->n_flux; q++) s->flux_acc[q] += plane_flux(s, s->flux[q][0], (size_t)s->flux[q][1]);
         s->steps++;
     }
     return 0;
@@ -402,6 +473,36 @@ int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic) {
     double vol = g->dx * g->dx * g->dx;
     *electric = ue * vol;
     *magnetic = uh * vol;
+    return 0;
+}
+
+/* Energy audit of step n = steps - 1 from the tracked state n and the
+ * current H^{n+1/2} (audit_periodic_energy.c:43-71), same sums and cells
+ * as fdtd_sim_energy. */
+int fdtd_sim_energy_audit(fdtd_sim_t *s, double out[5], size_t *step) {
+    if (!s || !out || !s->track || s->steps == 0) return -1;
+    fdtd_grid_t *g = s->g;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (size_t i = 0; i < g->nx; i++)
+        for (size_t j = 0; j < g->ny; j++)
+            for (size_t k = 0; k < g->nz; k++) {
+                size_t idx = fdtd_index(g, i, j, k);
+                for (int c = 0; c < 3; c++) {
+                    material_coeff_t co;
+                    if (materials_get_coefficients(s->mat, g, (field_component_t)c, i, j, k, &co) != 0) return -1;
+                    const double eps = g->dt / (g->dx * co.Cb);
+                    const double e = s->prev[c][idx], hm = s->prev[3 + c][idx], hp = h_of(g, c)[idx];
+                    const double avg = 0.5 * (hm + hp);
+                    acc[0] += 0.5 * eps * e * e;
+                    acc[1] += 0.5 * MU0 * hm * hm;
+                    acc[2] += 0.5 * MU0 * hp * hp;
+                    acc[3] += 0.5 * MU0 * avg * avg;
+                    acc[4] += 0.5 * MU0 * hm * hp;
+                }
+            }
+    const double vol = g->dx * g->dx * g->dx;
+    for (int q = 0; q < 5; q++) out[q] = acc[q] * vol;
+    if (step) *step = s->steps - 1;
     return 0;
 }
 

@@ -354,6 +354,18 @@ int rf_launch_plane_dft(const void *buf, int buf_prec, long long cells, int ring
 int rf_launch_energy(int prec, const rf_geom_t *g, void *const f[6], const void *tag, int tag_bytes,
                      const double *eps_tab, double vac_eps, double mu0, double *partials,
                      int n_partials, double *out, rf_stream_t *s);
+/* Energy audit of step n: E^n and H^{n-1/2} from fn (state n), H^{n+1/2}
+ * from fp (state n+1); out[0..5) = sums over the cells of 0.5 eps E^2,
+ * 0.5 mu0 (H-)^2, 0.5 mu0 (H+)^2, 0.5 mu0 ((H- + H+)/2)^2, 0.5 mu0 H- H+
+ * (unscaled by dx^3; partials holds 5 * n_partials values). */
+int rf_launch_audit(int prec, const rf_geom_t *g, void *const fn[6], void *const fp[6], const void *tag,
+                    int tag_bytes, const double *eps_tab, double vac_eps, double mu0, double *partials,
+                    int n_partials, double *out, rf_stream_t *s);
+/* Adds to *acc the Poynting flux along +axis through the plane of E nodes
+ * at `index` at time n+1/2 (unscaled by dx^2 dt): fn holds state n, fp
+ * state n+1. */
+int rf_launch_flux(int prec, const rf_geom_t *g, int axis, int index, void *const fn[6], void *const fp[6],
+                   double *partials, int n_partials, double *acc, rf_stream_t *s);
 
 /* Precision conversion between the double host layout and device fields. */
 int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,

@@ -99,6 +99,10 @@ struct fdtd_sim {
 
     double *d_energy;     /* partials + 2 results */
     int n_partials;
+    double *d_aux;        /* 5 * n_partials partials + 5 results: audit and flux reductions */
+    int (*flux)[2];       /* Poynting flux planes: axis, index */
+    int n_flux;
+    double *d_flux;       /* their time integrals (unscaled) */
 
     rf_graph_t *graphs[2];   /* captured block per buffer parity */
     size_t bytes;
@@ -247,6 +251,9 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     rf_free(s->d_tw);
     rf_free(s->d_step);
     rf_free(s->d_energy);
+    rf_free(s->d_aux);
+    rf_free(s->d_flux);
+    free(s->flux);
     for (size_t q = 0; q < s->n_ev; q++) rf_event_destroy(s->ev[q]);
     free(s->ev);
     rf_stream_destroy(s->st);
@@ -359,6 +366,19 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index, const doub
     memcpy(m->freqs, freqs_hz, n_freqs * sizeof(double));
     s->n_mons++;
     return (int)s->n_mons - 1;
+}
+
+int fdtd_sim_add_flux_plane(fdtd_sim_t *s, int axis, size_t index) {
+    if (!s || axis < 0 || axis > 2) { rf_set_error("bad flux plane"); return -1; }
+    if (s->started) { rf_set_error("flux planes must be added before the first run"); return -1; }
+    const size_t n[3] = {s->nx, s->ny, s->nz};
+    if (index >= n[axis]) { rf_set_error("flux plane outside the grid"); return -1; }
+    int (*nf)[2] = realloc(s->flux, (size_t)(s->n_flux + 1) * sizeof *nf);
+    if (!nf) return -1;
+    s->flux = nf;
+    s->flux[s->n_flux][0] = axis;
+    s->flux[s->n_flux][1] = (int)index;
+    return s->n_flux++;
 }
 
 /* The fused leapfrog kernel is the default step; RFSYN_FUSED=0 in the
@@ -483,6 +503,16 @@ static int finalize(fdtd_sim_t *s) {
         s->leap_zsplit = (e = getenv("RFSYN_LEAP_ZSPLIT")) ? atoi(e) != 0 : 0;
         s->ade_split_off = (e = getenv("RFSYN_ADE_SPLIT")) && atoi(e) == 0;
     }
+    if (s->n_flux) {
+        if (!s->fused) {
+            rf_set_error("flux planes need the fused step (state n is kept beside state n+1 there)");
+            return -1;
+        }
+        if (dalloc(s, (void **)&s->d_flux, (size_t)s->n_flux * sizeof(double))) return -1;
+    }
+    if ((s->n_flux || s->fused) && !s->d_aux &&
+        dalloc(s, (void **)&s->d_aux, (5 * (size_t)s->n_partials + 5) * sizeof(double)))
+        return -1;
     int P = (int)s->n_probes;
     size_t cap = 0;
     for (size_t p = 0; p < s->n_probes; p++) if (s->probes[p].td_cap > cap) cap = s->probes[p].td_cap;
@@ -791,6 +821,17 @@ static int enqueue_step(fdtd_sim_t *s) {
             if (s->planes[p].after_e &&
                 rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
                 return -1;
+        /* flux at n+1/2 from state n (f2) and state n+1 (f) */
+        for (int q = 0; q < s->n_flux; q++) {
+            const size_t n[3] = {s->nx, s->ny, s->nz};
+            const int a = s->flux[q][0];
+            const size_t cells = n[(a + 1) % 3] * n[(a + 2) % 3];
+            const int blocks = (int)((cells + 255) / 256 < (size_t)s->n_partials ? (cells + 255) / 256
+                                                                                 : (size_t)s->n_partials);
+            if (rf_launch_flux(s->prec, &s->geom, a, s->flux[q][1], s->f2, s->f, s->d_aux, blocks, s->d_flux + q,
+                               s->st))
+                return -1;
+        }
     } else {
         if (mark(s, 0)) return -1;
         if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
@@ -1050,6 +1091,41 @@ int fdtd_sim_energy(fdtd_sim_t *s, double *electric, double *magnetic) {
     double vol = s->dx * s->dx * s->dx;
     *electric = out[0] * vol;
     *magnetic = out[1] * vol;
+    return 0;
+}
+
+int fdtd_sim_energy_audit(fdtd_sim_t *s, double out[5], size_t *step) {
+    if (!s || !out) return -1;
+    if (!s->fused || s->steps == 0 || !s->d_aux) {
+        rf_set_error("the energy audit needs the fused step and at least one completed step");
+        return -1;
+    }
+    double vac_eps = s->dt / (s->dx * s->vac.vac_ca);
+    const void *tag = s->has_mat ? s->mat.d.tag : NULL;
+    int tb = s->has_mat ? s->mat.d.tag_bytes : 0;
+    const double *eps = s->has_mat ? s->mat.eps_tab : NULL;
+    double *d_out = s->d_aux + 5 * (size_t)s->n_partials;
+    double h[5];
+    /* state n = f2 (E^n, H^{n-1/2}), state n+1 = f (H^{n+1/2}) */
+    if (rf_launch_audit(s->prec, &s->geom, s->f2, s->f, tag, tb, eps, vac_eps, MU0, s->d_aux, s->n_partials, d_out,
+                        s->st) ||
+        rf_d2h(h, d_out, sizeof h, s->st))
+        return -1;
+    const double vol = s->dx * s->dx * s->dx;
+    for (int q = 0; q < 5; q++) out[q] = h[q] * vol;
+    if (step) *step = s->steps - 1;
+    return 0;
+}
+
+int fdtd_sim_flux(fdtd_sim_t *s, int plane, double *energy) {
+    if (!s || !energy || plane < 0 || plane >= s->n_flux) return -1;
+    if (!s->d_flux) {   /* nothing run yet */
+        *energy = 0.0;
+        return 0;
+    }
+    double v = 0.0;
+    if (rf_d2h(&v, s->d_flux + plane, sizeof v, s->st)) return -1;
+    *energy = v * s->dx * s->dx * s->dt;
     return 0;
 }
 

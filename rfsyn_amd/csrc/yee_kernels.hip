@@ -491,6 +491,105 @@ __global__ void k_energy_final(const double *partials, int n, double *out) {
     out[1] = h;
 }
 
+// Energy audit of step n (audit_periodic_energy.c:43-71): from E^n and
+// H^{n-1/2} (state n) and H^{n+1/2} (state n+1), per block the sums over
+// i<nx, j<ny, k<nz of 0.5 eps E^2, 0.5 mu0 (H-)^2, 0.5 mu0 (H+)^2,
+// 0.5 mu0 ((H- + H+)/2)^2 and 0.5 mu0 H- H+ (the last one makes the
+// modified energy E-part + it, invariant in a closed lossless grid).
+template <typename T, typename TagT>
+__global__ void __launch_bounds__(256)
+k_audit(rf_geom_t gg, const T *ex, const T *ey, const T *ez, const T *hxm, const T *hym, const T *hzm,
+        const T *hxp, const T *hyp, const T *hzp, const void *tag, const double *eps_tab, double vac_eps,
+        double mu0, double *partials) {
+    const Geo g = geo_of(gg);
+    const long long cells = (long long)g.nx * g.ny * g.nz;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long long r = t / g.nz;
+        const int j = (int)(r % g.ny), i = (int)(r / g.ny);
+        const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+        double eps[3] = {vac_eps, vac_eps, vac_eps};
+        if constexpr (!std::is_same<TagT, NoTag>::value) {
+            const unsigned tg = load_tag<TagT>(tag, idx);
+            eps[0] = eps_tab[3 * tg]; eps[1] = eps_tab[3 * tg + 1]; eps[2] = eps_tab[3 * tg + 2];
+        }
+        const double e[3] = {(double)ex[idx], (double)ey[idx], (double)ez[idx]};
+        const double hm[3] = {(double)hxm[idx], (double)hym[idx], (double)hzm[idx]};
+        const double hp[3] = {(double)hxp[idx], (double)hyp[idx], (double)hzp[idx]};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double avg = 0.5 * (hm[c] + hp[c]);
+            acc[0] += 0.5 * eps[c] * e[c] * e[c];
+            acc[1] += 0.5 * mu0 * hm[c] * hm[c];
+            acc[2] += 0.5 * mu0 * hp[c] * hp[c];
+            acc[3] += 0.5 * mu0 * avg * avg;
+            acc[4] += 0.5 * mu0 * hm[c] * hp[c];
+        }
+    }
+    __shared__ double sm[5][256];
+#pragma unroll
+    for (int q = 0; q < 5; q++) sm[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 5; q++) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 5; q++) partials[5 * blockIdx.x + q] = sm[q][0];
+}
+
+// Sum of n_partials blocks of `width` partial sums, in block order, into
+// out[0..width) (add = 0) or added to it (add = 1).
+__global__ void k_sum_partials(const double *partials, int n, int width, double *out, int add) {
+    if (blockIdx.x != 0 || threadIdx.x >= width) return;
+    const int q = threadIdx.x;
+    double s = 0.0;
+    for (int b = 0; b < n; b++) s += partials[width * b + q];
+    out[q] = add ? out[q] + s : s;
+}
+
+// Poynting flux through the plane of E nodes at index p along axis a,
+// at time n+1/2: the sum over the plane's cells of
+// (0.5 (E^n + E^{n+1})) x H^{n+1/2} along +a, with E tangential at node p
+// and H tangential at array index p (half a cell above the plane).  With
+// this pairing the change of the modified energy between two such planes
+// is exactly dt times the difference of their fluxes (the discrete
+// Poynting theorem of the Yee scheme).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_flux(rf_geom_t gg, int a, int p, const T *e0n, const T *e1n, const T *e0p, const T *e1p, const T *h0,
+       const T *h1, double *partials) {
+    const Geo g = geo_of(gg);
+    // the two tangential axes b, c (in cyclic order after a)
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const long long st[3] = {g.sx, g.sy, 1};
+    const long long cells = (long long)n[b] * n[c];
+    double s = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long ib = t / n[c], ic = t % n[c];
+        const long long idx = (long long)p * st[a] + ib * st[b] + ic * st[c];
+        // E_b, E_c time-averaged; H_b, H_c at n+1/2: S_a = E_b H_c - E_c H_b
+        const double eb = 0.5 * ((double)e0n[idx] + (double)e0p[idx]);
+        const double ec = 0.5 * ((double)e1n[idx] + (double)e1p[idx]);
+        s += eb * (double)h1[idx] - ec * (double)h0[idx];
+    }
+    __shared__ double sm[256];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sm[0];
+}
+
 template <typename D, typename S>
 __global__ void k_convert(D *dst, const S *src, long long n) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -562,6 +661,41 @@ int launch_energy_t(const rf_geom_t *g, T *const F[6], const void *tag, const do
 }  // namespace
 
 static hipStream_t hs(rf_stream_t *s) { return rf_raw_stream(s); }
+
+template <typename T, typename TagT>
+int launch_audit_t(const rf_geom_t *g, void *const fn[6], void *const fp[6], const void *tag, const double *eps_tab,
+                   double vac_eps, double mu0, double *partials, int n_partials, double *out, hipStream_t st) {
+    auto P = [](void *v) { return static_cast<const T *>(v); };
+    hipLaunchKernelGGL((k_audit<T, TagT>), dim3(n_partials), dim3(256), 0, st, g[0], P(fn[0]), P(fn[1]), P(fn[2]),
+                       P(fn[3]), P(fn[4]), P(fn[5]), P(fp[3]), P(fp[4]), P(fp[5]), tag, eps_tab, vac_eps, mu0,
+                       partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, n_partials, 5, out, 0);
+    return launched();
+}
+
+template <typename T>
+int launch_audit_p(const rf_geom_t *g, void *const fn[6], void *const fp[6], const void *tag, int tag_bytes,
+                   const double *eps_tab, double vac_eps, double mu0, double *partials, int n_partials,
+                   double *out, hipStream_t st) {
+    if (!tag) return launch_audit_t<T, NoTag>(g, fn, fp, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, st);
+    switch (tag_bytes) {
+    case 1: return launch_audit_t<T, uint8_t>(g, fn, fp, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, st);
+    case 2: return launch_audit_t<T, uint16_t>(g, fn, fp, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, st);
+    case 4: return launch_audit_t<T, uint32_t>(g, fn, fp, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, st);
+    }
+    return -1;
+}
+
+template <typename T>
+int launch_flux_t(const rf_geom_t *g, int a, int p, void *const fn[6], void *const fp[6], double *partials,
+                  int n_partials, double *acc, hipStream_t st) {
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    auto P = [](void *v) { return static_cast<const T *>(v); };
+    hipLaunchKernelGGL((k_flux<T>), dim3(n_partials), dim3(256), 0, st, g[0], a, p, P(fn[b]), P(fn[c]), P(fp[b]),
+                       P(fp[c]), P(fp[3 + b]), P(fp[3 + c]), partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, n_partials, 1, acc, 1);
+    return launched();
+}
 
 extern "C" {
 
@@ -719,6 +853,23 @@ int rf_launch_energy(int prec, const rf_geom_t *g, void *const f[6], const void 
     if (tag_bytes == 1) return launch_energy_t<float, uint8_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
     if (tag_bytes == 2) return launch_energy_t<float, uint16_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
     return launch_energy_t<float, uint32_t>(g, F, tag, eps_tab, vac_eps, mu0, partials, n_partials, out, hs(s));
+}
+
+int rf_launch_audit(int prec, const rf_geom_t *g, void *const fn[6], void *const fp[6], const void *tag,
+                    int tag_bytes, const double *eps_tab, double vac_eps, double mu0, double *partials,
+                    int n_partials, double *out, rf_stream_t *s) {
+    if (n_partials < 1) return -1;
+    return prec == 0 ? launch_audit_p<double>(g, fn, fp, tag, tag_bytes, eps_tab, vac_eps, mu0, partials, n_partials,
+                                              out, hs(s))
+                     : launch_audit_p<float>(g, fn, fp, tag, tag_bytes, eps_tab, vac_eps, mu0, partials, n_partials,
+                                             out, hs(s));
+}
+
+int rf_launch_flux(int prec, const rf_geom_t *g, int axis, int index, void *const fn[6], void *const fp[6],
+                   double *partials, int n_partials, double *acc, rf_stream_t *s) {
+    if (axis < 0 || axis > 2 || index < 0 || index >= g->n[axis] || n_partials < 1) return -1;
+    return prec == 0 ? launch_flux_t<double>(g, axis, index, fn, fp, partials, n_partials, acc, hs(s))
+                     : launch_flux_t<float>(g, axis, index, fn, fp, partials, n_partials, acc, hs(s));
 }
 
 int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,

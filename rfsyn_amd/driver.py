@@ -218,6 +218,23 @@ class NativeRun:
         assert self.lib.fdtd_sim_energy(self.sim, C.byref(e), C.byref(h)) == 0
         return e.value, h.value
 
+    def energy_audit(self):
+        """(the five audit sums of fdtd_sim_energy_audit, audited step)."""
+        out = (C.c_double * 5)()
+        step = C.c_size_t()
+        assert self.lib.fdtd_sim_energy_audit(self.sim, out, C.byref(step)) == 0, self.lib.fdtd_last_error()
+        return list(out), step.value
+
+    def add_flux_plane(self, axis, index):
+        fid = self.lib.fdtd_sim_add_flux_plane(self.sim, axis, index)
+        assert fid >= 0
+        return fid
+
+    def flux(self, fid):
+        w = C.c_double()
+        assert self.lib.fdtd_sim_flux(self.sim, fid, C.byref(w)) == 0
+        return w.value
+
     def close(self):
         self.lib.fdtd_sim_destroy(self.sim)
         self.lib.materials_destroy(self.mat)
