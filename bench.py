@@ -261,7 +261,46 @@ class C2:
         return el, self.n[0] * self.n[1] * self.n[2], "port"
 
 
-WORKLOADS = {"c1": C1, "c2": C2, "c3": C3}
+WORKLOADS = {"c1": C1, "c2": C2, "c3": C3, "c4": None}
+
+
+def bench_c4(args, lib):
+    """96 seeded scenes (sizes U{48..192} per axis, 10-cell CPML, 1-12
+    lossy boxes, one dipole, 4 probes x 64-frequency DFT), FP32, advanced
+    together with fdtd_sim_run_many; Mcell-updates/s over all scenes."""
+    from rfsyn_amd.driver import build_c4_scene, c4_scene, run_many
+    specs = [c4_scene(q) for q in range(96)]
+    cells = sum(sp["n"][0] * sp["n"][1] * sp["n"][2] for sp in specs)
+    runs = [build_c4_scene(lib, sp, capi.FDTD_FP32, 0) for sp in specs]
+    handles = [r for r, _ in runs]
+    run_many(lib, handles, args.warmup)
+    t0 = time.perf_counter()
+    run_many(lib, handles, args.steps)
+    wall = time.perf_counter() - t0
+    one = time.perf_counter()
+    for r in handles[:8]:   # the same 8 scenes one after another, for the batching gain
+        r.run(args.steps)
+    one = time.perf_counter() - one
+    cells8 = sum(sp["n"][0] * sp["n"][1] * sp["n"][2] for sp in specs[:8])
+    slab = sum(slab_cells(sp["n"], (10, 10, 10)) for sp in specs)
+    byt = step_bytes(cells, slab, 4, 1)
+    for r in handles:
+        r.close()
+    value = cells * args.steps / wall / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c4: 96 seeded scenes, n U{48..192} per axis, 10-cell CPML, 1-12 lossy boxes, "
+                               "dipole, 4 probes x 64-freq DFT, all scenes stepped together",
+                   "total_cells": cells},
+        "roofline": {"bound": "hbm", "kernel": "step (all scenes)", "achieved": round(byt / (wall / args.steps) / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(byt / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": byt},
+        "sequential_8_scenes_mcell_s": round(cells8 * args.steps / one / 1e6, 3),
+    }
+    print(json.dumps(out))
 
 
 def cpu_baseline(wl, budget_s=15.0):
@@ -341,6 +380,9 @@ def main():
         sys.exit(2)
 
     lib = rfsyn_amd.load_library()
+    if args.workload == "c4":
+        bench_c4(args, lib)
+        return
     wl = WORKLOADS[args.workload]()
     timed_steps = args.warmup + args.steps
     total = max(timed_steps, 0 if args.no_accuracy else wl.steps_total)

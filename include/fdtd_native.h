@@ -80,6 +80,14 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index,
 
 /* Advance n_steps full steps. */
 int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps);
+
+/* Batch of independent scenes (extension: the reference runs one scene
+ * per process): advances every handle n_steps, enqueueing one block of
+ * steps of each handle in turn on its own stream without waiting, so the
+ * scenes run concurrently on the GPU; returns when all are done.  Each
+ * handle ends in the state fdtd_sim_run would give it.  A handle may
+ * appear once. */
+int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 
 /* Copies min(n, recorded) time-domain samples; returns the count or -1. */

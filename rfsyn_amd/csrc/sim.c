@@ -945,8 +945,9 @@ int fdtd_sim_kernel_times3(fdtd_sim_t *s, double ms[3], long *launches) {
     return 0;
 }
 
-int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
-    if (!s) return -1;
+/* Set-up before n_steps more steps: first-run allocation and the
+ * waveform tables. */
+static int run_prepare(fdtd_sim_t *s, size_t n_steps) {
     if (!s->started && finalize(s) != 0) {
         rf_set_error("fdtd_sim_run set-up: %s", rf_error_string());
         return -1;
@@ -955,27 +956,41 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
         rf_set_error("waveform tables: %s", rf_error_string());
         return -1;
     }
-    size_t left = n_steps;
-    while (left) {
+    return 0;
+}
+
+/* Enqueues at most `max` of the *left steps on the handle's stream (one
+ * replayed graph block, or single steps when a block does not fit or
+ * profiling is on) without waiting for them. */
+static int run_enqueue(fdtd_sim_t *s, size_t *left, size_t max) {
+    size_t budget = max;
+    while (*left && budget) {
         if (s->profiling && s->ev_used + 6 > s->n_ev && prof_collect(s) != 0) return -1;
-        if (!s->profiling && s->pending == 0 && left >= (size_t)s->K) {
+        if (!s->profiling && s->pending == 0 && *left >= (size_t)s->K && budget >= (size_t)s->K) {
             if (!s->graphs[s->parity] && capture_block(s) != 0) {
                 rf_set_error("graph capture: %s", rf_error_string());
                 return -1;
             }
             if (rf_graph_launch(s->graphs[s->parity], s->st)) return -1;
             s->steps += (size_t)s->K;
-            left -= (size_t)s->K;
+            *left -= (size_t)s->K;
+            budget -= (size_t)s->K;
             continue;
         }
         if (enqueue_step(s)) return -1;
         s->steps++;
-        left--;
+        (*left)--;
+        budget--;
         if (++s->pending == s->K) {
             if (enqueue_flush(s, s->K)) return -1;
             s->pending = 0;
         }
     }
+    return 0;
+}
+
+/* Flushes the partial DFT block and waits for the handle's stream. */
+static int run_finish(fdtd_sim_t *s) {
     if (s->pending) {
         if (enqueue_flush(s, s->pending)) return -1;
         s->pending = 0;
@@ -985,6 +1000,48 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
         return -1;
     }
     return s->profiling ? prof_collect(s) : 0;
+}
+
+int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
+    if (!s) return -1;
+    if (run_prepare(s, n_steps)) return -1;
+    size_t left = n_steps;
+    if (run_enqueue(s, &left, n_steps)) return -1;
+    return run_finish(s);
+}
+
+int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
+    if (!sims && n_sims) return -1;
+    for (size_t q = 0; q < n_sims; q++) {
+        if (!sims[q]) return -1;
+        for (size_t r = 0; r < q; r++)
+            if (sims[r] == sims[q]) {
+                rf_set_error("fdtd_sim_run_many: a handle appears twice");
+                return -1;
+            }
+    }
+    for (size_t q = 0; q < n_sims; q++)
+        if (run_prepare(sims[q], n_steps)) return -1;
+    /* one block of each handle in turn: each handle's stream runs on its
+     * own, so small scenes fill the GPU together */
+    size_t *left = malloc((n_sims ? n_sims : 1) * sizeof *left);
+    if (!left) return -1;
+    for (size_t q = 0; q < n_sims; q++) left[q] = n_steps;
+    int rc = 0, busy = 1;
+    while (!rc && busy) {
+        busy = 0;
+        for (size_t q = 0; q < n_sims && !rc; q++) {
+            if (!left[q]) continue;
+            rc = run_enqueue(sims[q], &left[q], (size_t)sims[q]->K);
+            busy |= left[q] != 0;
+        }
+    }
+    for (size_t q = 0; q < n_sims; q++) {
+        const int r = run_finish(sims[q]);
+        if (!rc) rc = r;
+    }
+    free(left);
+    return rc;
 }
 
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }

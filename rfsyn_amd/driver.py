@@ -399,3 +399,55 @@ def build_c2(lib, precision, td_cap, n=256, npml=12, n_spheres=40, r_lo=4.0, r_h
     probes = c1_probes(n, (c, c, c), n_probes, seed=seed, lo=npml + 3, hi=n - npml - 3)
     pids = [run.add_probe(capi.FIELD_EZ, p, td_cap, freqs) for p in probes]
     return run, pids
+
+
+def c4_scene(index, seed=4, npml=10, n_lo=48, n_hi=192):
+    """Scene `index` of the seeded batch: sizes U{n_lo..n_hi} per axis;
+    1-12 lossy boxes (eps U[1,10], sigma U[0,0.05] S/m) inside the CPML;
+    one dipole on a random axis at a random interior node; 4 probes on the
+    dipole's component at least 8 cells from it.  PCG32 stream = index."""
+    rng = Pcg32(seed, index)
+    n = tuple(n_lo + rng.next_u32() % (n_hi - n_lo + 1) for _ in range(3))
+    lo_i = npml + 2
+    boxes = []
+    for _ in range(1 + rng.next_u32() % 12):
+        b = []
+        for a in range(3):
+            span = n[a] - 2 * lo_i
+            size = 4 + rng.next_u32() % max(1, span // 3)
+            start = lo_i + rng.next_u32() % max(1, span - size)
+            b.append((start, start + size))
+        eps = 1.0 + 9.0 * rng.uniform()
+        sigma = 0.05 * rng.uniform()
+        boxes.append((b, eps, sigma))
+    comp = rng.next_u32() % 3
+    src = tuple(lo_i + rng.next_u32() % (n[a] - 2 * lo_i) for a in range(3))
+    probes = []
+    while len(probes) < 4:
+        p = tuple(lo_i + rng.next_u32() % (n[a] - 2 * lo_i) for a in range(3))
+        if sum((p[a] - src[a]) ** 2 for a in range(3)) >= 64:
+            probes.append(p)
+    return {"n": n, "boxes": boxes, "comp": comp, "src": src, "probes": probes}
+
+
+def build_c4_scene(lib, spec, precision, td_cap, npml=10):
+    """A handle for one c4 scene: dx = 1 mm, S = 0.99, CPML npml cells on
+    every face (C1 grading), modulated-Gaussian dipole (30 GHz, 20 GHz -20 dB
+    bandwidth), 4 probes with 64-frequency DFTs over 20-40 GHz."""
+    def mats(lb, g, mat):
+        for b, eps, sigma in spec["boxes"]:
+            assert lb.materials_set_box(mat, g, b[0][0], b[0][1], b[1][0], b[1][1], b[2][0], b[2][1],
+                                        eps, sigma) == 0
+    run = NativeRun(lib, spec["n"], 1e-3, 0.99, cpml=c1_cpml_params(npml), precision=precision,
+                    compensated=1 if precision == capi.FDTD_FP32 else 0, material_setup=mats)
+    run.add_point_source(spec["comp"], spec["src"], "source_waveform_modulated_gaussian",
+                         modgauss_params(30e9, 20e9))
+    freqs = c1_freqs()
+    pids = [run.add_probe(spec["comp"], p, td_cap, freqs) for p in spec["probes"]]
+    return run, pids
+
+
+def run_many(lib, runs, steps):
+    """fdtd_sim_run_many over NativeRun handles."""
+    arr = (C.c_void_p * len(runs))(*[r.sim for r in runs])
+    assert lib.fdtd_sim_run_many(arr, len(runs), steps) == 0, lib.fdtd_last_error()
