@@ -1,11 +1,14 @@
 """Benchmark of the rfsyn FDTD core on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4]
 
 One step is one full Yee step (H update, sources, E update with CPML,
 materials and dispersive ADE, probes and plane monitors) over the whole
 grid.  The default workload is the 512x512x256 periodic dispersive stack
-(c3).  W untimed steps run first; then K steps are timed with the wall
+(c3); c4 steps 96 seeded scenes together (fdtd_sim_run_many).  Each
+handle first runs the tiling autotuner (fdtd_sim_autotune, before the
+first step; --no-autotune keeps the default tiling).  W untimed steps run
+first; then K steps are timed with the wall
 clock around fdtd_sim_run (which synchronises its stream), with the step's
 kernels bracketed by HIP events on the simulation stream.  Inputs are
 resident in HBM before the timed region starts.  The run then continues to
@@ -367,6 +370,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tiling of the fused step instead of timing the candidates first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -390,6 +395,9 @@ def main():
     for prec in (wl.headline, capi.FDTD_FP32 if wl.headline == capi.FDTD_FP64 else capi.FDTD_FP64):
         tag = "fp32" if prec == capi.FDTD_FP32 else "fp64"
         run, pids, mons = wl.build(lib, prec, total)
+        chosen = (C.c_int * 3)(0, 0, 0)
+        if not args.no_autotune:
+            assert lib.fdtd_sim_autotune(run.sim, 3, chosen) == 0, lib.fdtd_last_error()
         run.run(args.warmup)
         run.set_profiling(True)
         t0 = time.perf_counter()
@@ -410,7 +418,9 @@ def main():
             run.run(total - timed_steps)
         dft = np.stack([run.dft(p) for p in pids])
         run.close()
-        res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, sh_ms=sh_ms / nk, dft=dft, fused=fused, box=box)
+        res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, sh_ms=sh_ms / nk, dft=dft, fused=fused, box=box,
+                         tiling=None if args.no_autotune or not fused else
+                         {"rows": chosen[0], "planes_per_item": chosen[1], "z_split": chosen[2]})
 
     n = wl.n
     cells = n[0] * n[1] * n[2]
@@ -470,6 +480,7 @@ def main():
         "other_precision": {"dtype": "f32" if wl.headline == capi.FDTD_FP64 else "f64",
                             "value": round(cells * args.steps / other["wall"] / 1e6, 3)},
         "fp32_vs_fp64_probe_dft_err": err,
+        "tiling": r["tiling"],
         "accuracy_steps": total,
     }
     if not args.no_cpu_baseline:

@@ -215,6 +215,7 @@ EXTENSION_API = {
     "fdtd_sim_kernel_times3": (C.c_int, [_vp, _dp, C.POINTER(C.c_long)]),
     "fdtd_sim_energy_audit": (C.c_int, [_vp, _dp, C.POINTER(_sz)]),
     "fdtd_sim_run_many": (C.c_int, [C.POINTER(_vp), _sz, _sz]),
+    "fdtd_sim_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "fdtd_sim_add_flux_plane": (C.c_int, [_vp, C.c_int, _sz]),
     "fdtd_sim_flux": (C.c_int, [_vp, C.c_int, _dp]),
     "fdtd_sim_fused": (C.c_int, [_vp]),

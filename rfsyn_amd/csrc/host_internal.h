@@ -75,6 +75,7 @@ typedef struct {
     long long n_tensor;
     double *eps_tab;          /* device, 3 per entry, FP64 */
     size_t n_entries;
+    size_t ade_bytes;         /* bytes of each of d.J and d.P (0: none) */
     void *owned[16];
     int n_owned;
     size_t bytes;

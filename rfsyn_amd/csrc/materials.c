@@ -664,6 +664,7 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
         d->d.ade_stride = (long long)stride;
         size_t sb = (size_t)(slots ? slots : 1) * 3 * stride * es;
         if (dev_upload(d, NULL, sb, &d->d.J) || dev_upload(d, NULL, sb, &d->d.P)) goto out;
+        d->ade_bytes = sb;
     }
 
     /* tags */

@@ -721,7 +721,9 @@ static int ade_pass_box(const fdtd_sim_t *s, int lo[3], int hi[3]) {
 /* The fused step: the leapfrog kernel on the plain box (timed in event
  * slots 0-1), then the split ADE pass (slots 2-3) and the shell kernels
  * around the box (slots 4-5). */
-static int enqueue_fused(fdtd_sim_t *s) {
+/* The fused kernel's arguments for the current buffers and the handle's
+ * tiling requests; *m receives the materials it reads. */
+static int fill_leap(fdtd_sim_t *s, rf_leap_t *out, const rf_mat_t **mout, int alo[3], int ahi[3]) {
     rf_leap_t a;
     memset(&a, 0, sizeof a);
     a.g = s->geom;
@@ -772,12 +774,21 @@ static int enqueue_fused(fdtd_sim_t *s) {
     memcpy(a.src_box, s->src_box, sizeof a.src_box);
     a.d_step = s->d_step;
     a.zsplit = s->leap_zsplit;
-    int alo[3], ahi[3];
     a.ade_split = ade_pass_box(s, alo, ahi);
     if (rf_leap_plan(s->prec, &a, s->leap_rows, s->leap_chunk, s->leap_waves)) {
         rf_set_error("fused step: unsupported tiling (rows %d, chunk %d)", s->leap_rows, s->leap_chunk);
         return -1;
     }
+    *out = a;
+    *mout = m;
+    return 0;
+}
+
+static int enqueue_fused(fdtd_sim_t *s) {
+    rf_leap_t a;
+    const rf_mat_t *m;
+    int alo[3], ahi[3];
+    if (fill_leap(s, &a, &m, alo, ahi)) return -1;
     if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
     if (a.ade_split && rf_launch_ade_pass(s->prec, &s->geom, alo, ahi, s->f, s->f2, m, s->st)) return -1;
     if (mark(s, 3) || mark(s, 4)) return -1;
@@ -1008,6 +1019,80 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
     size_t left = n_steps;
     if (run_enqueue(s, &left, n_steps)) return -1;
     return run_finish(s);
+}
+
+/* Tiling autotuner: before the first step, times the fused kernel alone
+ * (A -> B, `launches` launches per candidate) for each rows / chunk / z
+ * split candidate and keeps the fastest.  The launches write only B, the
+ * z-psi store, the in-kernel ADE state and the queue counters, which all
+ * hold their initial values before the first step (B = A, the rest zero)
+ * and are restored afterwards.  The tiling changes which wavefront updates
+ * a node, never the arithmetic, so results do not depend on it. */
+int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[3]) {
+    if (!s || launches < 1) return -1;
+    if (s->steps) {
+        rf_set_error("fdtd_sim_autotune must run before the first step");
+        return -1;
+    }
+    if (run_prepare(s, 1)) return -1;
+    if (!s->fused) return 0;   /* nothing to tune on the two-kernel path */
+    rf_event_t *e0 = rf_event_create(), *e1 = rf_event_create();
+    int rc = (e0 && e1) ? 0 : -1;
+    const int rows32[2] = {4, 8}, rows64[2] = {2, 4};
+    const int chunks[5] = {0, 6, 10, 16, 32};
+    const int *rows = s->prec == 0 ? rows64 : rows32;
+    const int r0 = s->leap_rows, c0 = s->leap_chunk, z0 = s->leap_zsplit;
+    int best[3] = {r0, c0, z0};
+    float best_ms = -1.f;
+    for (int ri = 0; ri < 2 && !rc; ri++)
+        for (int ci = 0; ci < 5 && !rc; ci++)
+            for (int zi = 0; zi < (s->zc ? 2 : 1) && !rc; zi++) {
+                s->leap_rows = rows[ri];
+                s->leap_chunk = chunks[ci];
+                s->leap_zsplit = zi;
+                rf_leap_t a;
+                const rf_mat_t *m;
+                int alo[3], ahi[3];
+                for (int w = 0; w <= launches && !rc; w++) {   /* one warm-up launch */
+                    if (w == 1 && rf_event_record(e0, s->st)) rc = -1;
+                    if (!rc && fill_leap(s, &a, &m, alo, ahi)) rc = -1;
+                    if (!rc && rf_launch_leapfrog(s->prec, &a, s->st)) rc = -1;
+                    s->parity ^= 1;   /* alternate the counter sets as the step does */
+                    s->zpar ^= 1;
+                }
+                float ms = 0.f;
+                if (!rc && (rf_event_record(e1, s->st) || rf_sync(s->st) || rf_event_elapsed_ms(e0, e1, &ms)))
+                    rc = -1;
+                if (!rc && (best_ms < 0.f || ms < best_ms)) {
+                    best_ms = ms;
+                    best[0] = rows[ri];
+                    best[1] = chunks[ci];
+                    best[2] = zi;
+                }
+            }
+    rf_event_destroy(e0);
+    rf_event_destroy(e1);
+    /* restore the state before the first step */
+    s->parity = 0;
+    s->zpar = 0;
+    if (!rc && (rf_memset(s->d_ctr, 0, 32 * sizeof(unsigned), s->st) ||
+                rf_d2d(s->blk2, s->blk, 6 * s->cells * s->es, s->st) ||
+                (s->zpsi && rf_memset(s->zpsi, 0, (s->nx + 1) * (s->ny + 1) * (size_t)s->cp.d.len[2] * 6 * s->es,
+                                      s->st)) ||
+                (s->has_mat && s->mat.d.J &&
+                 (rf_memset(s->mat.d.J, 0, s->mat.ade_bytes, s->st) ||
+                  rf_memset(s->mat.d.P, 0, s->mat.ade_bytes, s->st))) ||
+                rf_sync(s->st)))
+        rc = -1;
+    s->leap_rows = rc ? r0 : best[0];
+    s->leap_chunk = rc ? c0 : best[1];
+    s->leap_zsplit = rc ? z0 : best[2];
+    if (chosen) {
+        chosen[0] = s->leap_rows;
+        chosen[1] = s->leap_chunk;
+        chosen[2] = s->leap_zsplit;
+    }
+    return rc;
 }
 
 int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
