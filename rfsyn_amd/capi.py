@@ -141,6 +141,13 @@ class SimOptions(C.Structure):
                 ("dft_block", C.c_size_t)]
 
 
+class SurConfig(C.Structure):
+    """fdtd_sur_config_t (include/fdtd_surrogate.h)."""
+    _fields_ = [("grid", C.c_int), ("in_channels", C.c_int), ("channels", C.c_int * 4), ("coords", C.c_int),
+                ("width", C.c_int), ("outputs", C.c_int), ("queries", C.c_int), ("max_batch", C.c_int)]
+
+
+_fp = C.POINTER(C.c_float)
 _sz = C.c_size_t
 _vp = C.c_void_p
 _gp = C.POINTER(Grid)
@@ -216,6 +223,14 @@ EXTENSION_API = {
     "fdtd_sim_energy_audit": (C.c_int, [_vp, _dp, C.POINTER(_sz)]),
     "fdtd_sim_run_many": (C.c_int, [C.POINTER(_vp), _sz, _sz]),
     "fdtd_sim_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+    "fdtd_sur_create": (_vp, [C.POINTER(SurConfig), C.c_uint64]),
+    "fdtd_sur_destroy": (None, [_vp]),
+    "fdtd_sur_n_params": (_sz, [_vp]),
+    "fdtd_sur_get_params": (C.c_int, [_vp, _fp]),
+    "fdtd_sur_set_params": (C.c_int, [_vp, _fp]),
+    "fdtd_sur_predict": (C.c_int, [_vp, _fp, _fp, C.c_int, _fp]),
+    "fdtd_sur_gradient": (C.c_int, [_vp, _fp, _fp, _fp, C.c_int, _fp, _dp]),
+    "fdtd_sur_train_step": (C.c_int, [_vp, _fp, _fp, _fp, C.c_int, C.c_float, _dp]),
     "fdtd_sim_add_flux_plane": (C.c_int, [_vp, C.c_int, _sz]),
     "fdtd_sim_flux": (C.c_int, [_vp, C.c_int, _dp]),
     "fdtd_sim_fused": (C.c_int, [_vp]),

@@ -371,6 +371,29 @@ int rf_launch_flux(int prec, const rf_geom_t *g, int axis, int index, void *cons
 int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,
                       rf_stream_t *s);
 
+/* ------------------------------------------------ surrogate kernels
+ * (sur_kernels.hip): BF16 operands as uint16_t bit patterns. */
+#include <stdint.h>
+/* C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), A(m, k) = A[m sam + k sak],
+ * B(k, n) = B[k sbk + n sbn], FP32 accumulation; splits > 1 partitions K
+ * and sums the partials (part: splits M N floats) in split order. */
+int rf_sur_gemm(int M, int N, int K, const uint16_t *A, long long sam, long long sak, const uint16_t *B,
+                long long sbk, long long sbn, float *C, long long ldc, const float *bias, int splits, float *part,
+                rf_stream_t *s);
+int rf_sur_to_bf16(const float *x, uint16_t *y, long long n, rf_stream_t *s);
+int rf_sur_gelu(const float *z, float *a, uint16_t *a16, long long n, rf_stream_t *s);
+int rf_sur_gelu_bwd(const float *z, const float *da, float *dz, uint16_t *dz16, long long n, rf_stream_t *s);
+int rf_sur_im2col(const uint16_t *in, uint16_t *col, int B, int S, int C, rf_stream_t *s);
+int rf_sur_col2im(const float *dcol, float *din, int B, int S, int C, rf_stream_t *s);
+int rf_sur_expand_y(const float *maps, uint16_t *vox, int B, int S, int C, rf_stream_t *s);
+int rf_sur_add_scene(const float *U, const float *V, const float *bias, float *h, long long rows, int P, int N,
+                     rf_stream_t *s);
+int rf_sur_sum_queries(const float *dh, float *dU, uint16_t *dU16, long long scenes, int P, int N, rf_stream_t *s);
+int rf_sur_mse(const float *y, const float *label, float *grad, uint16_t *grad16, long long n, double *partials,
+               int n_partials, rf_stream_t *s);
+int rf_sur_adam(float *w, uint16_t *w16, const float *g, float *m, float *v, long long n, float lr, float b1,
+                float b2, float eps, float c1, float c2, rf_stream_t *s);
+
 #ifdef __cplusplus
 }
 #endif

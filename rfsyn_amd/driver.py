@@ -451,3 +451,61 @@ def run_many(lib, runs, steps):
     """fdtd_sim_run_many over NativeRun handles."""
     arr = (C.c_void_p * len(runs))(*[r.sim for r in runs])
     assert lib.fdtd_sim_run_many(arr, len(runs), steps) == 0, lib.fdtd_last_error()
+
+
+class Surrogate:
+    """ctypes wrapper of an fdtd_sur_t (include/fdtd_surrogate.h)."""
+
+    def __init__(self, lib, grid, in_channels, channels, coords, width, outputs, queries, max_batch, seed=1):
+        self.lib = lib
+        self.cfg = capi.SurConfig(grid, in_channels, (C.c_int * 4)(*channels), coords, width, outputs, queries,
+                                  max_batch)
+        self.h = lib.fdtd_sur_create(C.byref(self.cfg), seed)
+        assert self.h, lib.fdtd_last_error()
+        self.n_params = lib.fdtd_sur_n_params(self.h)
+
+    @staticmethod
+    def _f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        assert self.lib.fdtd_sur_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))) == 0
+        return out
+
+    def set_params(self, p):
+        a, pa = self._f(p)
+        assert a.size == self.n_params
+        assert self.lib.fdtd_sur_set_params(self.h, pa) == 0
+
+    def predict(self, maps, coords):
+        n = maps.shape[0]
+        m, pm = self._f(maps)
+        c, pc = self._f(coords)
+        out = np.zeros((n, self.cfg.queries, self.cfg.outputs), dtype=np.float32)
+        assert self.lib.fdtd_sur_predict(self.h, pm, pc, n, out.ctypes.data_as(C.POINTER(C.c_float))) == 0
+        return out
+
+    def gradient(self, maps, coords, labels):
+        n = maps.shape[0]
+        m, pm = self._f(maps)
+        c, pc = self._f(coords)
+        lb, pl = self._f(labels)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        assert self.lib.fdtd_sur_gradient(self.h, pm, pc, pl, n, g.ctypes.data_as(C.POINTER(C.c_float)),
+                                          C.byref(loss)) == 0
+        return loss.value, g
+
+    def train_step(self, maps, coords, labels, lr):
+        n = maps.shape[0]
+        m, pm = self._f(maps)
+        c, pc = self._f(coords)
+        lb, pl = self._f(labels)
+        loss = C.c_double()
+        assert self.lib.fdtd_sur_train_step(self.h, pm, pc, pl, n, lr, C.byref(loss)) == 0
+        return loss.value
+
+    def close(self):
+        self.lib.fdtd_sur_destroy(self.h)
