@@ -264,7 +264,60 @@ class C2:
         return el, self.n[0] * self.n[1] * self.n[2], "port"
 
 
-WORKLOADS = {"c1": C1, "c2": C2, "c3": C3, "c4": None}
+WORKLOADS = {"c1": C1, "c2": C2, "c3": C3, "c4": None, "c5": None}
+
+
+def bench_c5(args, lib):
+    """Learned surrogate plus FDTD judge on 64^3 scenes of the periodic
+    dispersive stack distribution: label train and test scenes with the
+    batched GPU FDTD, train the BF16 network, then infer the test scenes at
+    batch 1024 with a 5% FDTD spot check and FDTD fallback above 2e-2
+    relative L2.  Reports scenes/s of both paths and the accuracy."""
+    from rfsyn_amd import surrogate_pipeline as sp
+    steps = args.c5_fdtd_steps
+    n_tr, n_te = args.c5_train, args.c5_test
+    log = lambda m: print(f"[c5] {m}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    mtr, ctr, ltr, fdtd_tr = sp.label_scenes(lib, list(range(n_tr)), steps)
+    mte, cte, lte, fdtd_te = sp.label_scenes(lib, list(range(n_tr, n_tr + n_te)), steps)
+    label_wall = time.perf_counter() - t0
+    log(f"labelled {n_tr + n_te} scenes: {label_wall:.1f} s wall, {fdtd_tr + fdtd_te:.1f} s of FDTD")
+    scale = float(np.std(ltr))
+    net = sp.make_net(lib, batch=1024)
+    t1 = time.perf_counter()
+    losses = sp.train(net, mtr, ctr, ltr / scale, args.c5_epochs, lr=args.c5_lr, batch=min(256, n_tr), log=log)
+    train_s = time.perf_counter() - t1
+    # accuracy on the held-out scenes, and the predictor without the network
+    pred = sp.predict(net, mte, cte, 1024) * scale
+    err = sp.rel_l2(pred, lte)
+    base = sp.rel_l2(np.broadcast_to(ltr.mean(axis=0), lte.shape), lte)
+    # inference with the FDTD judge
+    out, inf_t, fallbacks, spot = sp.spot_checked_inference(lib, net, list(range(n_tr, n_tr + n_te)), mte, cte,
+                                                            steps, 1024, scale=scale)
+    net.close()
+    # GPU time of each path (the FDTD's host-side scene set-up is left out
+    # of both, as in the labelling rate)
+    fdtd_rate = (n_tr + n_te) / (fdtd_tr + fdtd_te)
+    sur_rate = n_te / (inf_t["network"] + inf_t["fdtd"])
+    res = {
+        "metric": "scenes/s (surrogate with FDTD spot checks vs batched FDTD), relative L2 vs FDTD",
+        "value": round(sur_rate, 3), "unit": "scenes/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"c5: 64^3 periodic dispersive stacks, {n_tr} train / {n_te} held-out scenes, "
+                               f"8 probes x 64-freq transfer functions, FDTD {steps} steps",
+                   "epochs": args.c5_epochs, "lr": args.c5_lr, "batch_train": min(256, n_tr), "batch_infer": 1024},
+        "fdtd_scenes_per_s": round(fdtd_rate, 3),
+        "speedup_vs_fdtd": round(sur_rate / fdtd_rate, 3),
+        "network_only_scenes_per_s": round(n_te / max(inf_t["network"], 1e-9), 1),
+        "held_out_rel_l2_median": float(np.median(err)), "held_out_rel_l2_p90": float(np.percentile(err, 90)),
+        "mean_predictor_rel_l2_median": float(np.median(base)),
+        "spot_checks": len(spot), "spot_check_rel_l2_max": float(np.max(spot)) if spot else None,
+        "fallback_batches": fallbacks, "train_seconds": round(train_s, 2), "train_loss_last": losses[-1],
+        "inference_seconds": {k: round(v, 3) for k, v in inf_t.items()},
+        "label_wall_seconds": round(label_wall, 2),
+    }
+    print(json.dumps(res))
 
 
 def bench_c4(args, lib):
@@ -370,6 +423,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")
+    ap.add_argument("--c5-train", type=int, default=2048, help="c5: training scenes")
+    ap.add_argument("--c5-test", type=int, default=1024, help="c5: held-out scenes")
+    ap.add_argument("--c5-epochs", type=int, default=30, help="c5: training epochs")
+    ap.add_argument("--c5-lr", type=float, default=3e-4, help="c5: Adam learning rate")
+    ap.add_argument("--c5-fdtd-steps", type=int, default=1024, help="c5: FDTD steps per label")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiling of the fused step instead of timing the candidates first")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -387,6 +445,9 @@ def main():
     lib = rfsyn_amd.load_library()
     if args.workload == "c4":
         bench_c4(args, lib)
+        return
+    if args.workload == "c5":
+        bench_c5(args, lib)
         return
     wl = WORKLOADS[args.workload]()
     timed_steps = args.warmup + args.steps
