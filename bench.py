@@ -456,7 +456,7 @@ def main():
     for prec in (wl.headline, capi.FDTD_FP32 if wl.headline == capi.FDTD_FP64 else capi.FDTD_FP64):
         tag = "fp32" if prec == capi.FDTD_FP32 else "fp64"
         run, pids, mons = wl.build(lib, prec, total)
-        chosen = (C.c_int * 3)(0, 0, 0)
+        chosen = (C.c_int * 4)(0, 0, 0, 0)
         if not args.no_autotune:
             assert lib.fdtd_sim_autotune(run.sim, 3, chosen) == 0, lib.fdtd_last_error()
         run.run(args.warmup)
@@ -481,7 +481,7 @@ def main():
         run.close()
         res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, sh_ms=sh_ms / nk, dft=dft, fused=fused, box=box,
                          tiling=None if args.no_autotune or not fused else
-                         {"rows": chosen[0], "planes_per_item": chosen[1], "z_split": chosen[2]})
+                         {"rows": chosen[0], "planes_per_item": chosen[1], "z_split": chosen[2], "coop": chosen[3]})
 
     n = wl.n
     cells = n[0] * n[1] * n[2]

@@ -91,12 +91,16 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps);
 
 /* Tiling autotuner (extension): before the first step, times the fused
  * step's kernel for each candidate tiling (rows per tile, planes per work
- * item, a separate launch for the tiles without z CPML lanes), `launches`
+ * item, a separate launch for the tiles without z CPML lanes, independent
+ * wavefronts or cooperative blocks of neighbouring tiles), `launches`
  * launches each, and keeps the fastest for this handle.  The tiling never
  * changes the results.  chosen (may be NULL) receives rows, planes per
- * item (0: automatic) and the z split.  0 on success (also on the
- * two-kernel path, where there is nothing to tune). */
-int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[3]);
+ * item (0: automatic), the z split and the block mode (0: independent
+ * wavefronts; else cooperative blocks of all z tiles by (mode & 15) row
+ * tiles, whose wavefronts stay within mode >> 4 planes of each other, 0 =
+ * lockstep).  0 on success (also on the two-kernel path, where there is
+ * nothing to tune). */
+int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 
 /* Copies min(n, recorded) time-domain samples; returns the count or -1. */

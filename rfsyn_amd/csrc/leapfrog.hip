@@ -46,7 +46,8 @@ using namespace rfk;
 
 constexpr int WL = 64;       // lanes per wavefront
 constexpr int KW = WL - 2;   // columns owned per wavefront
-constexpr int WPB = 4;       // wavefronts per block
+constexpr int WPB = 4;       // wavefronts per block (independent mode)
+constexpr int WPB_MAX = 4;   // wavefronts per block (cooperative mode)
 constexpr int N_XCD = 8;     // blocks are dealt round-robin to the XCDs
 
 // DPP wavefront shifts: lane l receives lane l+1 (from_upper) or lane l-1
@@ -325,7 +326,7 @@ __device__ __forceinline__ T zterm(T diff, T kinv, T b, T aa, bool slab, T old, 
     return diff * kinv + dx * psi;
 }
 
-template <typename T, typename TagT, bool ZC, int R>
+template <typename T, typename TagT, bool ZC, int R, bool COOP>
 __global__ void __launch_bounds__(WL *WPB)
 k_leapfrog(rf_leap_t a) {
     constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
@@ -381,30 +382,75 @@ k_leapfrog(rf_leap_t a) {
     }
 
     const int lane = threadIdx.x & (WL - 1);
-    // Work queue: the items (tile, x chunk) are split in N_XCD contiguous
-    // ranges, one per XCD (blocks are dealt round-robin to the XCDs), so
-    // the tiles an XCD runs at a time are neighbours that share halo rows
-    // in its L2; each wavefront takes the next item of its XCD's counter
-    // until the range is exhausted, which balances tiles of unequal cost
-    // (dispersive, source and CPML columns).  The counters of this launch
-    // are a.ctr[a.qpar][*] (a.ctr points at the pair of sets of its part);
-    // the other set is zeroed here for the next step.
+    // Work queue: the items are split in N_XCD contiguous ranges, one per
+    // XCD (blocks are dealt round-robin to the XCDs), so the tiles an XCD
+    // runs at a time are neighbours that share halo rows in its L2; each
+    // taker gets the next item of its XCD's counter until the range is
+    // exhausted, which balances tiles of unequal cost (dispersive, source
+    // and CPML columns).  The counters of this launch are a.ctr[a.qpar][*]
+    // (a.ctr points at the pair of sets of its part); the other set is
+    // zeroed here for the next step.
+    //
+    // Independent mode (a.gz == 0): an item is one (tile, x chunk) and each
+    // wavefront takes its own.  Cooperative mode: an item is a group of
+    // gz x gy tiles (gz neighbouring z tiles by gy neighbouring row tiles)
+    // of one x chunk, taken by a block of gz * gy wavefronts that march the
+    // planes together (a block barrier per plane, or at most gk planes
+    // apart), so the lines that neighbouring tiles share (the halo rows)
+    // are fetched once and hit in L1/L2 for the other wavefront instead of
+    // being fetched again planes later.  The plan uses gz = 1: tiles of
+    // one z tile cost the same per plane (z CPML columns), so no wavefront
+    // of a block idles waiting for a costlier neighbour.
     const int xcd = blockIdx.x % N_XCD;
     if (blockIdx.x == 0 && threadIdx.x < N_XCD) a.ctr[(1 - a.qpar) * N_XCD + threadIdx.x] = 0u;
     const int item_lo = (int)(((long long)a.n_items * xcd) / N_XCD);
     const int item_hi = (int)(((long long)a.n_items * (xcd + 1)) / N_XCD);
     unsigned *my_ctr = a.ctr + a.qpar * N_XCD + xcd;
     const Geo g = geo_of(a.g);
+    const int wid = static_cast<int>(threadIdx.x) / WL;
+    // cooperative mode: the block's ticket and, with a plane slack
+    // (a.gk > 0), the planes each wavefront of the group has completed
+    __shared__ unsigned s_ticket;
+    __shared__ int s_prog[WPB_MAX];
     for (;;) {
-    unsigned ticket = 0;
-    if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
-    const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
-    if (item >= item_hi) break;
     const int ntl = a.tkr[1] + a.tkr[3];   // z tiles of this part
-    const int tl = item % ntl;
+    int tl, tj, tc;
+    bool valid = true;
+    bool lane_wave = false;   // lane < group size and that wavefront has a tile
+    if constexpr (COOP) {
+        __syncthreads();   // every wavefront has read the previous ticket
+        if (threadIdx.x == 0) s_ticket = atomicAdd(my_ctr, 1u);
+        if (lane == 0) s_prog[wid] = 0;
+        __syncthreads();
+        const int grp = item_lo + static_cast<int>(s_ticket);
+        if (grp >= item_hi) break;   // block-uniform
+        const int nzg = (ntl + a.gz - 1) / a.gz, nyg = (a.ntj + a.gy - 1) / a.gy;
+        const int tl0 = (grp % nzg) * a.gz, tj0 = ((grp / nzg) % nyg) * a.gy;
+        tl = tl0 + wid % a.gz;
+        tj = tj0 + wid / a.gz;
+        tc = grp / (nzg * nyg);
+        valid = tl < ntl && tj < a.ntj;
+        lane_wave = lane < a.gz * a.gy && tl0 + lane % a.gz < ntl && tj0 + lane / a.gz < a.ntj;
+    } else {
+        unsigned ticket = 0;
+        if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
+        const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
+        if (item >= item_hi) break;
+        tl = item % ntl;
+        tj = (item / ntl) % a.ntj;
+        tc = item / (ntl * a.ntj);
+    }
+    if (!valid) {
+        // a wavefront of the block without a tile: only the barriers of
+        // the group's planes (the count depends on the chunk alone)
+        if (a.gk == 0) {
+            const int ib = a.ib0[0] + tc * a.chunk, ie = min(ib + a.chunk, a.ib1[0]);
+            const int n_it = (ie - ib) + ((ib > 0 || g.px) ? 1 : 0);
+            for (int it = 0; it < n_it; it++) __builtin_amdgcn_s_barrier();
+        }
+        continue;
+    }
     const int tk = tl < a.tkr[1] ? a.tkr[0] + tl : a.tkr[2] + (tl - a.tkr[1]);
-    const int tj = (item / ntl) % a.ntj;
-    const int tc = item / (ntl * a.ntj);
 
     // the box of nodes this launch owns, [ib0, ib1) per axis; along z the
     // tiles restart at kbrk (the start of the upper CPML slab)
@@ -495,6 +541,20 @@ k_leapfrog(rf_leap_t a) {
     for (int r = 0; r <= R; r++) py[r] = pz[r] = T(0);
 
     for (int it = 0; it < iters; it++) {
+        if constexpr (COOP) {
+            // plane slack: start plane `it` once every wavefront of the
+            // group has completed plane it - gk.  Only a hint for the
+            // caches (no wavefront reads what another writes in this
+            // launch), so the wait is bounded and then proceeds anyway.
+            if (a.gk > 0 && it > a.gk) {
+                const int need = it - a.gk;
+                for (int spin = 0; spin < (1 << 14); spin++) {
+                    const int p = lane_wave ? *reinterpret_cast<volatile int *>(&s_prog[lane]) : need;
+                    if (__ballot(p < need) == 0) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
         const bool first = warm && it == 0;
         int ip = i + 1;
         if (g.px) {
@@ -688,6 +748,15 @@ k_leapfrog(rf_leap_t a) {
             cz[r] = nz[r];
         }
         i = first ? i_begin : i + 1;
+        // the block's tiles stay on the same plane: a barrier without a
+        // memory fence (no wavefront reads what another writes in this
+        // launch), so no wavefront waits for its stores to complete
+        if constexpr (COOP) {
+            if (a.gk == 0)
+                __builtin_amdgcn_s_barrier();
+            else if (lane == 0)
+                *reinterpret_cast<volatile int *>(&s_prog[wid]) = it + 1;
+        }
     }
     }
 }
@@ -973,11 +1042,12 @@ int shell_e_t(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void 
 // so E, J and P are bitwise those of the one-pass update.
 //
 // One thread per node of the part of the ADE state box that lies in the
-// fused box, k fastest.  The pass streams the box: every thread loads E^n,
-// t and the state of all slots of its node without waiting for its tag
-// (the partly dispersive cache lines of the box are fetched whole either
-// way), so one memory round trip covers the node; only the dispersive
-// edges compute and store.
+// fused box, k fastest.  A thread reads its node's tag first and then only
+// what its dispersive edges use: E^n, t and, per pole of the edge's ADE
+// set, J and (Lorentz poles) P.  The box holds non-dispersive nodes too
+// (gaps between layers, the dielectric stripes of a grating), whose lines
+// are then not fetched at all; the second round trip is hidden by the
+// other wavefronts.
 constexpr int ADE_PASS_MAX_SETS = 64;   // ADE sets staged in LDS
 
 template <typename T, int NS>
@@ -1041,19 +1111,7 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
     const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
     const T *__restrict__ J = static_cast<const T *>(m.J);
     const T *__restrict__ P = static_cast<const T *>(m.P);
-    const int ns = m.n_slots;
     const unsigned tag = load_tag<TagT>(m.tag, idx);
-    const T eo[3] = {exa[idx], eya[idx], eza[idx]};
-    const T et[3] = {exb[idx], eyb[idx], ezb[idx]};
-    T jv[3][NS], pv[3][NS];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const long long o = (long long)(3 * s + c) * st + cell;
-            jv[c][s] = (NS <= 2 || s < ns) ? J[o] : T(0);
-            pv[c][s] = (NS <= 2 || s < ns) ? P[o] : T(0);
-        }
 
     uint32_t fl, ai;
     if (static_cast<int>(tag) < ncache) {
@@ -1069,13 +1127,39 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
     const bool in = i < g.nx && j < g.ny && k < g.nz;
     const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
     const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
+    bool need[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const uint32_t f = (fl >> (8 * c)) & 0xffu;
+        need[c] = on[c] && (f & RF_EDGE_ADE) && !(f & RF_EDGE_TENSOR);
+    }
+    if (!(need[0] || need[1] || need[2])) return;
+    // the loads of all dispersive edges of the node, then the updates
+    const T *ea_[3] = {exa, eya, eza};
     T *eb[3] = {exb, eyb, ezb};
+    T eo[3], et[3], jv[3][NS], pv[3][NS];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        eo[c] = et[c] = T(0);
+#pragma unroll
+        for (int s = 0; s < NS; s++) jv[c][s] = pv[c][s] = T(0);
+        if (!need[c]) continue;
+        const Ade &ad = s_ade[(ai >> (8 * c)) & 0xff];
+        eo[c] = ea_[c][idx];
+        et[c] = eb[c][idx];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (s >= ad.n_poles) continue;
+            const long long o = (long long)(3 * s + c) * st + cell;
+            jv[c][s] = J[o];
+            if (ad.has_p[s]) pv[c][s] = P[o];
+        }
+    }
     T *__restrict__ Jw = static_cast<T *>(m.J);
     T *__restrict__ Pw = static_cast<T *>(m.P);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const uint32_t f = (fl >> (8 * c)) & 0xffu;
-        if (!on[c] || !(f & RF_EDGE_ADE) || (f & RF_EDGE_TENSOR)) continue;
+        if (!need[c]) continue;
         const Ade &ad = s_ade[(ai >> (8 * c)) & 0xff];
         T nj[NS], np[NS];
         eb[c][idx] = ade_finish<T, NS>(ad, et[c], eo[c], jv[c], pv[c], nj, np);
@@ -1130,13 +1214,24 @@ size_t leap_lds(const rf_leap_t *a) {
            (size_t)a->ncache * coef_bytes<T>();
 }
 
+// Wavefronts per block: WPB in independent mode, gz * gy cooperatively.
+inline int block_waves(const rf_leap_t *a) { return a->gz > 0 ? a->gz * a->gy : WPB; }
+
 template <typename T, typename TagT, int R>
 int launch_tag(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
     const size_t lds = leap_lds<T>(a);
-    if (a->zc)
-        hipLaunchKernelGGL((k_leapfrog<T, TagT, true, R>), dim3(blocks), dim3(WL * WPB), lds, st, *a);
-    else
-        hipLaunchKernelGGL((k_leapfrog<T, TagT, false, R>), dim3(blocks), dim3(WL * WPB), lds, st, *a);
+    const unsigned threads = static_cast<unsigned>(WL * block_waves(a));
+    if (a->gz > 0) {
+        if (a->zc)
+            hipLaunchKernelGGL((k_leapfrog<T, TagT, true, R, true>), dim3(blocks), dim3(threads), lds, st, *a);
+        else
+            hipLaunchKernelGGL((k_leapfrog<T, TagT, false, R, true>), dim3(blocks), dim3(threads), lds, st, *a);
+    } else {
+        if (a->zc)
+            hipLaunchKernelGGL((k_leapfrog<T, TagT, true, R, false>), dim3(blocks), dim3(threads), lds, st, *a);
+        else
+            hipLaunchKernelGGL((k_leapfrog<T, TagT, false, R, false>), dim3(blocks), dim3(threads), lds, st, *a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1145,11 +1240,12 @@ template <typename T, typename TagT, int R>
 int capacity_tag(const rf_leap_t *a) {
     int nb = 0, cus = 0, dev = 0;
     const size_t lds = leap_lds<T>(a);
-    const int bw = WPB;
-    hipError_t e = a->zc ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, true, R>), WL * bw, lds)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                               &nb, reinterpret_cast<const void *>(k_leapfrog<T, TagT, false, R>), WL * bw, lds);
+    const int bw = block_waves(a);
+    const void *fn = a->gz > 0 ? (a->zc ? reinterpret_cast<const void *>(k_leapfrog<T, TagT, true, R, true>)
+                                        : reinterpret_cast<const void *>(k_leapfrog<T, TagT, false, R, true>))
+                               : (a->zc ? reinterpret_cast<const void *>(k_leapfrog<T, TagT, true, R, false>)
+                                        : reinterpret_cast<const void *>(k_leapfrog<T, TagT, false, R, false>));
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, WL * bw, lds);
     if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nb <= 0)
         return 0;
@@ -1248,19 +1344,39 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
         a->pzc[np++] = 1;
     }
     a->n_parts = np;
-    // resident wavefronts of each part's variant
+    // block shape of each part: cooperative blocks take coop & 15 (at most
+    // WPB_MAX) neighbouring row tiles of one z tile
+    if (a->coop < 0) return -1;
+    a->gk = a->coop >> 4;
+    for (int p = 0; p < np; p++) {
+        if (a->coop == 0) {
+            a->pgz[p] = a->pgy[p] = 0;
+            continue;
+        }
+        const int gz = 1;
+        int gy = a->coop & 15;
+        if (gy < 1) gy = 1;
+        if (gy > WPB_MAX / gz) gy = WPB_MAX / gz;
+        if (gy > a->ntj) gy = a->ntj;
+        a->pgz[p] = gz;
+        a->pgy[p] = gy;
+    }
+    // resident wavefronts of each part's variant and block shape
     static long long cache_key[8];
     static int cache_val[8], cache_n = 0;
     const size_t lds = prec == 0 ? leap_lds<double>(a) : leap_lds<float>(a);
     int resident[2] = {0, 0};
     for (int p = 0; p < np; p++) {
-        const long long key =
-            ((long long)lds << 16) | (prec << 12) | (a->tag_bytes << 8) | (a->pzc[p] << 7) | (rows << 3);
+        const int bw = a->pgz[p] > 0 ? a->pgz[p] * a->pgy[p] : 0;
+        const long long key = ((long long)lds << 20) | (bw << 13) | (prec << 12) | (a->tag_bytes << 8) |
+                              (a->pzc[p] << 7) | (rows << 3);
         for (int q = 0; q < cache_n; q++)
             if (cache_key[q] == key) resident[p] = cache_val[q];
         if (!resident[p]) {
             rf_leap_t v = *a;
             v.zc = a->pzc[p];
+            v.gz = a->pgz[p];
+            v.gy = a->pgy[p];
             resident[p] = leap_capacity(prec, &v);
             if (resident[p] <= 0) resident[p] = 4096;
             if (cache_n < 8) {
@@ -1285,11 +1401,19 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
     a->chunk = chunk;
     const long long nch = (planes + chunk - 1) / chunk;
     for (int p = 0; p < np; p++) {
-        const long long items = (long long)(a->ptk[p][1] + a->ptk[p][3]) * a->ntj * nch;
+        const long long ntl = a->ptk[p][1] + a->ptk[p][3];
+        long long items, bw;
+        if (a->pgz[p] > 0) {   // groups of gz x gy tiles
+            items = (ntl + a->pgz[p] - 1) / a->pgz[p] * ((a->ntj + a->pgy[p] - 1) / a->pgy[p]) * nch;
+            bw = a->pgz[p] * a->pgy[p];
+        } else {
+            items = ntl * a->ntj * nch;
+            bw = WPB;
+        }
         if (items > (1LL << 30)) return -1;
         a->pitems[p] = (int)items;
-        long long blocks = (resident[p] + WPB - 1) / WPB;
-        const long long need = (items + WPB - 1) / WPB;
+        long long blocks = (resident[p] + bw - 1) / bw;
+        const long long need = a->pgz[p] > 0 ? items : (items + bw - 1) / bw;
         if (blocks > need) blocks = need;
         a->pblocks[p] = (int)((blocks + N_XCD - 1) / N_XCD * N_XCD);
     }
@@ -1312,6 +1436,9 @@ extern "C" int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s) 
         for (int q = 0; q < 4; q++) b.tkr[q] = r[q];
         b.zc = a->pzc[p];
         b.n_items = a->pitems[p];
+        b.gz = a->pgz[p];
+        b.gy = a->pgy[p];
+        if (b.gz < 0 || b.gy < 0 || b.gz * b.gy > WPB_MAX || (b.gz > 0 && b.gy < 1)) return -1;
         b.ctr = a->ctr + 16 * p;
         const unsigned blocks = static_cast<unsigned>(a->pblocks[p]);
         int rc = -1;

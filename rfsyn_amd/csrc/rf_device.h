@@ -243,11 +243,21 @@ typedef struct {
      * the tiles with no lane in the z CPML form a part of their own that
      * runs the variant without it */
     int zsplit;
+    /* coop (input): 0 = every wavefront takes its own work item; else
+     * cooperative blocks: a block of gz x gy wavefronts takes a group of
+     * gz = 1 z tile by gy = max(1, coop & 15) neighbouring row tiles of one
+     * x chunk (gy <= 4; reduced to fit) and marches its planes together:
+     * in lockstep (a barrier per plane) when gk = coop >> 4 is 0, else no
+     * wavefront starts plane p before every one of the group has completed
+     * plane p - gk */
+    int coop;
     int n_parts;
-    int ptk[2][4], pzc[2], pitems[2], pblocks[2];
-    /* the part a launch runs (set by rf_launch_leapfrog) */
+    int ptk[2][4], pzc[2], pitems[2], pblocks[2], pgz[2], pgy[2];
+    /* the part a launch runs (set by rf_launch_leapfrog); gz = 0:
+     * independent mode */
     int tkr[4];
     int n_items;
+    int gz, gy, gk;
 } rf_leap_t;
 
 /* Tiling of the fused step over a->ib0..ib1; rows: 2 or 4 (FP64), 4 or 8

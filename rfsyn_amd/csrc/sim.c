@@ -113,6 +113,7 @@ struct fdtd_sim {
     void *f2[6];
     int leap_rows, leap_chunk, leap_waves;   /* tiling requests (0: automatic) */
     int leap_zsplit;         /* z CPML tiles in a launch of their own */
+    int leap_coop;           /* cooperative blocks (rf_leap_t.coop) */
     rf_leap_src_t *d_srcs;   /* device source table of the fused step */
     rf_leap_cold_t *d_cold;  /* its cold data (ADE) */
     int ib0[3], ib1[3];      /* node box of the fused kernel */
@@ -501,6 +502,8 @@ static int finalize(fdtd_sim_t *s) {
         s->leap_chunk = (e = getenv("RFSYN_LEAP_CHUNK")) ? atoi(e) : 0;
         s->leap_waves = (e = getenv("RFSYN_LEAP_WAVES")) ? atoi(e) : 0;
         s->leap_zsplit = (e = getenv("RFSYN_LEAP_ZSPLIT")) ? atoi(e) != 0 : 0;
+        s->leap_coop = (e = getenv("RFSYN_LEAP_COOP")) ? atoi(e) : 0;
+        if (s->leap_coop < 0 || s->leap_coop > 255) s->leap_coop = 0;
         s->ade_split_off = (e = getenv("RFSYN_ADE_SPLIT")) && atoi(e) == 0;
     }
     if (s->n_flux) {
@@ -774,6 +777,7 @@ static int fill_leap(fdtd_sim_t *s, rf_leap_t *out, const rf_mat_t **mout, int a
     memcpy(a.src_box, s->src_box, sizeof a.src_box);
     a.d_step = s->d_step;
     a.zsplit = s->leap_zsplit;
+    a.coop = s->leap_coop;
     a.ade_split = ade_pass_box(s, alo, ahi);
     if (rf_leap_plan(s->prec, &a, s->leap_rows, s->leap_chunk, s->leap_waves)) {
         rf_set_error("fused step: unsupported tiling (rows %d, chunk %d)", s->leap_rows, s->leap_chunk);
@@ -1028,7 +1032,7 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
  * hold their initial values before the first step (B = A, the rest zero)
  * and are restored afterwards.  The tiling changes which wavefront updates
  * a node, never the arithmetic, so results do not depend on it. */
-int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[3]) {
+int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]) {
     if (!s || launches < 1) return -1;
     if (s->steps) {
         rf_set_error("fdtd_sim_autotune must run before the first step");
@@ -1041,15 +1045,20 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[3]) {
     const int rows32[2] = {4, 8}, rows64[2] = {2, 4};
     const int chunks[5] = {0, 6, 10, 16, 32};
     const int *rows = s->prec == 0 ? rows64 : rows32;
-    const int r0 = s->leap_rows, c0 = s->leap_chunk, z0 = s->leap_zsplit;
-    int best[3] = {r0, c0, z0};
+    /* independent; cooperative blocks of 2 or 4 row tiles, in lockstep or
+     * with a slack of 1 plane */
+    const int coops[5] = {0, 2, 4, 16 + 2, 16 + 4};
+    const int r0 = s->leap_rows, c0 = s->leap_chunk, z0 = s->leap_zsplit, g0 = s->leap_coop;
+    int best[4] = {r0, c0, z0, g0};
     float best_ms = -1.f;
     for (int ri = 0; ri < 2 && !rc; ri++)
         for (int ci = 0; ci < 5 && !rc; ci++)
-            for (int zi = 0; zi < (s->zc ? 2 : 1) && !rc; zi++) {
+            for (int zi = 0; zi < (s->zc ? 2 : 1) && !rc; zi++)
+            for (int gi = 0; gi < 5 && !rc; gi++) {
                 s->leap_rows = rows[ri];
                 s->leap_chunk = chunks[ci];
                 s->leap_zsplit = zi;
+                s->leap_coop = coops[gi];
                 rf_leap_t a;
                 const rf_mat_t *m;
                 int alo[3], ahi[3];
@@ -1068,6 +1077,7 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[3]) {
                     best[0] = rows[ri];
                     best[1] = chunks[ci];
                     best[2] = zi;
+                    best[3] = coops[gi];
                 }
             }
     rf_event_destroy(e0);
@@ -1087,10 +1097,12 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[3]) {
     s->leap_rows = rc ? r0 : best[0];
     s->leap_chunk = rc ? c0 : best[1];
     s->leap_zsplit = rc ? z0 : best[2];
+    s->leap_coop = rc ? g0 : best[3];
     if (chosen) {
         chosen[0] = s->leap_rows;
         chosen[1] = s->leap_chunk;
         chosen[2] = s->leap_zsplit;
+        chosen[3] = s->leap_coop;
     }
     return rc;
 }

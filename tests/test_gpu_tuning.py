@@ -8,7 +8,7 @@ import pytest
 
 from rfsyn_amd import capi
 from rfsyn_amd.driver import NativeRun, modgauss_params, random_fill, same_values
-from test_gpu_parity import _ade_region_setup, _dispersive_setup
+from test_gpu_parity import _ade_region_run, _ade_region_setup, _dispersive_setup, _region_run
 
 pytestmark = pytest.mark.gpu
 
@@ -18,7 +18,7 @@ def _stack(lib, precision, tune):
                     precision=precision, material_setup=_dispersive_setup, init_fields=lambda g: random_fill(g, 2))
     run.add_plane_source(capi.FIELD_EX, 2, 10, "source_waveform_modulated_gaussian", modgauss_params(40e9, 30e9))
     pid = run.add_probe(capi.FIELD_EX, (3, 5, 24), 90, None)
-    chosen = (C.c_int * 3)(-1, -1, -1)
+    chosen = (C.c_int * 4)(-1, -1, -1, -1)
     if tune:
         assert run.lib.fdtd_sim_autotune(run.sim, 2, chosen) == 0, run.lib.fdtd_last_error()
     run.run(90)
@@ -59,3 +59,42 @@ def test_autotune_after_first_step_is_refused(gpu):
     run.run(1)
     assert gpu.fdtd_sim_autotune(run.sim, 1, None) == -1
     run.close()
+
+
+@pytest.mark.parametrize("coop", [1, 3, 4, 20, 35])
+@pytest.mark.parametrize("case", ["wide_z", "x_periodic", "z_periodic", "all_shell"])
+def test_cooperative_blocks_fp64(gpu, oracle, case, coop, monkeypatch):
+    """Cooperative blocks (coop & 15 neighbouring row tiles of one z tile,
+    planes in lockstep or within coop >> 4 planes), with short chunks so
+    that the last chunk is shorter and some wavefronts of a block have no
+    tile: bitwise against the oracle."""
+    monkeypatch.setenv("RFSYN_LEAP_COOP", str(coop))
+    monkeypatch.setenv("RFSYN_LEAP_CHUNK", "3")
+    a = _region_run(gpu, case)
+    b = _region_run(oracle, case)
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
+
+
+@pytest.mark.parametrize("coop", [4, 18])
+def test_cooperative_blocks_ade_fp64(gpu, oracle, coop, monkeypatch):
+    """Dispersive blocks across the CPML shell with cooperative blocks."""
+    monkeypatch.setenv("RFSYN_LEAP_COOP", str(coop))
+    a = _ade_region_run(gpu, False)
+    b = _ade_region_run(oracle, False)
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
+
+
+@pytest.mark.parametrize("coop", [2, 4, 20])
+def test_cooperative_blocks_fp32_bitwise_independent(gpu, coop, monkeypatch):
+    """FP32 periodic dispersive stack: cooperative blocks give the bits of
+    independent wavefronts."""
+    a, _ = _stack(gpu, capi.FDTD_FP32, False)
+    monkeypatch.setenv("RFSYN_LEAP_COOP", str(coop))
+    b, _ = _stack(gpu, capi.FDTD_FP32, False)
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
