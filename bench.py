@@ -285,7 +285,9 @@ def bench_c5(args, lib):
     scale = float(np.std(ltr))
     net = sp.make_net(lib, batch=1024)
     t1 = time.perf_counter()
-    losses = sp.train(net, mtr, ctr, ltr / scale, args.c5_epochs, lr=args.c5_lr, batch=min(256, n_tr), log=log)
+    bt = min(args.c5_batch, n_tr)
+    losses = sp.train(net, mtr, ctr, ltr / scale, args.c5_epochs, lr=args.c5_lr, batch=bt,
+                      log=log, schedule="cos")
     train_s = time.perf_counter() - t1
     # accuracy on the held-out scenes, and the predictor without the network
     pred = sp.predict(net, mte, cte, 1024) * scale
@@ -306,7 +308,8 @@ def bench_c5(args, lib):
         "data": "synthetic",
         "config": {"workload": f"c5: 64^3 periodic dispersive stacks, {n_tr} train / {n_te} held-out scenes, "
                                f"8 probes x 64-freq transfer functions, FDTD {steps} steps",
-                   "epochs": args.c5_epochs, "lr": args.c5_lr, "batch_train": min(256, n_tr), "batch_infer": 1024},
+                   "epochs": args.c5_epochs, "lr": args.c5_lr, "lr_schedule": "cosine", "batch_train": bt,
+                   "batch_infer": 1024},
         "fdtd_scenes_per_s": round(fdtd_rate, 3),
         "speedup_vs_fdtd": round(sur_rate / fdtd_rate, 3),
         "network_only_scenes_per_s": round(n_te / max(inf_t["network"], 1e-9), 1),
@@ -423,10 +426,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")
-    ap.add_argument("--c5-train", type=int, default=2048, help="c5: training scenes")
+    ap.add_argument("--c5-train", type=int, default=4096, help="c5: training scenes")
     ap.add_argument("--c5-test", type=int, default=1024, help="c5: held-out scenes")
-    ap.add_argument("--c5-epochs", type=int, default=30, help="c5: training epochs")
-    ap.add_argument("--c5-lr", type=float, default=3e-4, help="c5: Adam learning rate")
+    ap.add_argument("--c5-epochs", type=int, default=240, help="c5: training epochs")
+    ap.add_argument("--c5-lr", type=float, default=1e-3, help="c5: Adam learning rate (cosine-annealed)")
+    ap.add_argument("--c5-batch", type=int, default=64, help="c5: training batch")
     ap.add_argument("--c5-fdtd-steps", type=int, default=1024, help="c5: FDTD steps per label")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiling of the fused step instead of timing the candidates first")

@@ -251,21 +251,29 @@ __device__ __forceinline__ void e_flagged3(T e[3], const T dA[3], const T dB[3],
     if (!any) return;
     const rf_leap_cold_t &m = *cold_view(cold);
     T *J = static_cast<T *>(m.J), *P = static_cast<T *>(m.P);
-    const long long st = m.ade_stride;
-    const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
     if (n_slots <= 2) {
+        // the state of every dispersive edge of the node, loaded before any
+        // edge is updated (each edge in the box of its own ADE set)
         T jv[3][2], pv[3][2];
+        long long off[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+        for (int c = 0; c < 3; c++) {
+            off[c] = 0;
+#pragma unroll
+            for (int s = 0; s < 2; s++) jv[c][s] = pv[c][s] = T(0);
+            if (!ade_c[c]) continue;
+            const int id = (ai >> (8 * c)) & 0xff;
+            const Ade &ad = id < n_adec ? s_ade[id] : static_cast<const Ade *>(m.ade)[id];
+            off[c] = ade_off(ad, c, i, j, k);
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                jv[c][s] = pv[c][s] = T(0);
-                if (ade_c[c] && s < n_slots) {
-                    const long long o = s * 3 * st + c * st + cell;
+                if (s < ad.n_poles) {
+                    const long long o = s * 3 * ad.st + off[c];
                     jv[c][s] = J[o];
-                    pv[c][s] = P[o];
+                    if (ad.has_p[s]) pv[c][s] = P[o];
                 }
             }
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             if (!ade_c[c]) continue;
@@ -276,7 +284,7 @@ __device__ __forceinline__ void e_flagged3(T e[3], const T dA[3], const T dB[3],
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 if (s < ad.n_poles) {
-                    const long long o = s * 3 * st + c * st + cell;
+                    const long long o = s * 3 * ad.st + off[c];
                     J[o] = nj[s];
                     if (ad.has_p[s]) P[o] = np[s];
                 }
@@ -288,7 +296,7 @@ __device__ __forceinline__ void e_flagged3(T e[3], const T dA[3], const T dB[3],
             if (!ade_c[c]) continue;
             const int id = (ai >> (8 * c)) & 0xff;
             const Ade &ad = id < n_adec ? s_ade[id] : static_cast<const Ade *>(m.ade)[id];
-            e[c] = ade_edge<T>(ad, ca[c], cb[c], e[c], dA[c], dB[c], J, P, 3 * st, (long long)c * st + cell);
+            e[c] = ade_edge<T>(ad, ca[c], cb[c], e[c], dA[c], dB[c], J, P, 3 * ad.st, ade_off(ad, c, i, j, k));
         }
     }
 }
@@ -899,11 +907,8 @@ __device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_
         if (fl & RF_EDGE_TENSOR) return e_old;
         if (fl & RF_EDGE_ADE) {
             const Ade &ad = static_cast<const Ade *>(m.ade)[en.ade[c]];
-            const long long st = m.ade_stride;
-            const long long cell =
-                ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
             return ade_edge<T>(ad, en.Ca[c], en.Cb[c], e_old, dA, dB, static_cast<T *>(m.J),
-                               static_cast<T *>(m.P), 3 * st, (long long)c * st + cell);
+                               static_cast<T *>(m.P), 3 * ad.st, ade_off(ad, c, i, j, k));
         }
         return en.Ca[c] * e_old + en.Cb[c] * (dA - dB);
     }
@@ -1041,14 +1046,14 @@ int shell_e_t(const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3], void 
 // operations of ade_edge (yee_device.h) in the same order and precision,
 // so E, J and P are bitwise those of the one-pass update.
 //
-// One thread per node of the part of the ADE state box that lies in the
-// fused box, k fastest.  A thread reads its node's tag first and then only
-// what its dispersive edges use: E^n, t and, per pole of the edge's ADE
-// set, J and (Lorentz poles) P.  The box holds non-dispersive nodes too
-// (gaps between layers, the dielectric stripes of a grating), whose lines
-// are then not fetched at all; the second round trip is hidden by the
-// other wavefronts.
+// The pass covers the part of the ADE state box that lies in the fused
+// box, k fastest, ADE_NPT nodes per thread.  A thread reads its nodes'
+// tags first and then only what their dispersive edges use: E^n, t and,
+// per pole of the edge's ADE set, J and (Lorentz poles) P.  The box holds
+// non-dispersive nodes too (gaps between layers, the dielectric stripes of
+// a grating), whose lines are then not fetched at all.
 constexpr int ADE_PASS_MAX_SETS = 64;   // ADE sets staged in LDS
+constexpr int ADE_NPT = 1;              // nodes per thread of the ADE pass (2 and 4 measured slower)
 
 template <typename T, int NS>
 __device__ __forceinline__ T ade_finish(const typename EntryOf<T>::ade &a, T t, T e_old, const T jv[NS],
@@ -1099,79 +1104,104 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
     for (int e = threadIdx.x; e < m.n_ade; e += blockDim.x) s_ade[e] = ades[e];
     __syncthreads();
 
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    // ADE_NPT nodes per thread, ADE_NPT * (gridDim.x * blockDim.x) >= count
+    // apart: the tags of all of them are loaded, then the state of all
+    // their dispersive edges, then they are updated, so each round trip
+    // carries several nodes' loads
     const Geo g = geo_of(gg);
-    const unsigned q = t / nk;
-    const int k = lo2 + static_cast<int>(t - q * nk);
-    const int j = lo1 + static_cast<int>(q % nj);
-    const int i = lo0 + static_cast<int>(q / nj);
-    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
-    const long long st = m.ade_stride;
-    const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
     const T *__restrict__ J = static_cast<const T *>(m.J);
     const T *__restrict__ P = static_cast<const T *>(m.P);
-    const unsigned tag = load_tag<TagT>(m.tag, idx);
-
-    uint32_t fl, ai;
-    if (static_cast<int>(tag) < ncache) {
-        fl = s_ent[2 * tag];
-        ai = s_ent[2 * tag + 1];
-    } else {
-        const Entry &en = ents[tag];
-        fl = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
-        ai = (static_cast<uint32_t>(en.ade[0]) & 0xffu) | ((static_cast<uint32_t>(en.ade[1]) & 0xffu) << 8) |
-             ((static_cast<uint32_t>(en.ade[2]) & 0xffu) << 16);
+    long long idx[ADE_NPT];
+    unsigned tag[ADE_NPT];
+    bool node[ADE_NPT];
+    int ii[ADE_NPT], jj[ADE_NPT], kk[ADE_NPT];
+#pragma unroll
+    for (int u = 0; u < ADE_NPT; u++) {
+        const unsigned t = t0 + static_cast<unsigned>(u) * stride;
+        node[u] = t < count;
+        const unsigned tt = node[u] ? t : 0u;
+        const unsigned q = tt / nk;
+        kk[u] = lo2 + static_cast<int>(tt - q * nk);
+        jj[u] = lo1 + static_cast<int>(q % nj);
+        ii[u] = lo0 + static_cast<int>(q / nj);
+        idx[u] = (long long)ii[u] * g.sx + (long long)jj[u] * g.sy + kk[u];
+        tag[u] = node[u] ? load_tag<TagT>(m.tag, idx[u]) : 0u;
     }
-    // E ranges of the update (grid.c:95-117)
-    const bool in = i < g.nx && j < g.ny && k < g.nz;
-    const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
-    const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
-    bool need[3];
+    bool need[ADE_NPT][3];
+    uint32_t ai[ADE_NPT];
+    bool any = false;
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const uint32_t f = (fl >> (8 * c)) & 0xffu;
-        need[c] = on[c] && (f & RF_EDGE_ADE) && !(f & RF_EDGE_TENSOR);
-    }
-    if (!(need[0] || need[1] || need[2])) return;
-    // the loads of all dispersive edges of the node, then the updates
-    const T *ea_[3] = {exa, eya, eza};
-    T *eb[3] = {exb, eyb, ezb};
-    T eo[3], et[3], jv[3][NS], pv[3][NS];
+    for (int u = 0; u < ADE_NPT; u++) {
+        uint32_t fl;
+        if (static_cast<int>(tag[u]) < ncache) {
+            fl = s_ent[2 * tag[u]];
+            ai[u] = s_ent[2 * tag[u] + 1];
+        } else {
+            const Entry &en = ents[tag[u]];
+            fl = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+            ai[u] = (static_cast<uint32_t>(en.ade[0]) & 0xffu) | ((static_cast<uint32_t>(en.ade[1]) & 0xffu) << 8) |
+                    ((static_cast<uint32_t>(en.ade[2]) & 0xffu) << 16);
+        }
+        // E ranges of the update (grid.c:95-117)
+        const int i = ii[u], j = jj[u], k = kk[u];
+        const bool in = node[u] && i < g.nx && j < g.ny && k < g.nz;
+        const bool i_ok = g.px || i >= 1, j_ok = g.py || j >= 1, k_ok = g.pz || k >= 1;
+        const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        eo[c] = et[c] = T(0);
-#pragma unroll
-        for (int s = 0; s < NS; s++) jv[c][s] = pv[c][s] = T(0);
-        if (!need[c]) continue;
-        const Ade &ad = s_ade[(ai >> (8 * c)) & 0xff];
-        eo[c] = ea_[c][idx];
-        et[c] = eb[c][idx];
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            if (s >= ad.n_poles) continue;
-            const long long o = (long long)(3 * s + c) * st + cell;
-            jv[c][s] = J[o];
-            if (ad.has_p[s]) pv[c][s] = P[o];
+        for (int c = 0; c < 3; c++) {
+            const uint32_t f = (fl >> (8 * c)) & 0xffu;
+            need[u][c] = on[c] && (f & RF_EDGE_ADE) && !(f & RF_EDGE_TENSOR);
+            any = any || need[u][c];
         }
     }
+    if (!any) return;
+    // the loads of all dispersive edges of the nodes, then the updates
+    const T *ea_[3] = {exa, eya, eza};
+    T *eb[3] = {exb, eyb, ezb};
+    T eo[ADE_NPT][3], et[ADE_NPT][3], jv[ADE_NPT][3][NS], pv[ADE_NPT][3][NS];
+    long long off[ADE_NPT][3];
+#pragma unroll
+    for (int u = 0; u < ADE_NPT; u++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            eo[u][c] = et[u][c] = T(0);
+            off[u][c] = 0;
+#pragma unroll
+            for (int s = 0; s < NS; s++) jv[u][c][s] = pv[u][c][s] = T(0);
+            if (!need[u][c]) continue;
+            const Ade &ad = s_ade[(ai[u] >> (8 * c)) & 0xff];
+            off[u][c] = ade_off(ad, c, ii[u], jj[u], kk[u]);
+            eo[u][c] = ea_[c][idx[u]];
+            et[u][c] = eb[c][idx[u]];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                if (s >= ad.n_poles) continue;
+                const long long o = 3 * s * ad.st + off[u][c];
+                jv[u][c][s] = J[o];
+                if (ad.has_p[s]) pv[u][c][s] = P[o];
+            }
+        }
     T *__restrict__ Jw = static_cast<T *>(m.J);
     T *__restrict__ Pw = static_cast<T *>(m.P);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        if (!need[c]) continue;
-        const Ade &ad = s_ade[(ai >> (8 * c)) & 0xff];
-        T nj[NS], np[NS];
-        eb[c][idx] = ade_finish<T, NS>(ad, et[c], eo[c], jv[c], pv[c], nj, np);
-        const int npl = ad.n_poles;
+    for (int u = 0; u < ADE_NPT; u++)
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            if (s >= npl) continue;
-            const long long o = (long long)(3 * s + c) * st + cell;
-            Jw[o] = nj[s];
-            if (ad.has_p[s]) Pw[o] = np[s];
+        for (int c = 0; c < 3; c++) {
+            if (!need[u][c]) continue;
+            const Ade &ad = s_ade[(ai[u] >> (8 * c)) & 0xff];
+            T nj_[NS], np_[NS];
+            eb[c][idx[u]] = ade_finish<T, NS>(ad, et[u][c], eo[u][c], jv[u][c], pv[u][c], nj_, np_);
+            const int npl = ad.n_poles;
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                if (s >= npl) continue;
+                const long long o = 3 * s * ad.st + off[u][c];
+                Jw[o] = nj_[s];
+                if (ad.has_p[s]) Pw[o] = np_[s];
+            }
         }
-    }
 }
 
 template <typename T, typename TagT>
@@ -1179,7 +1209,7 @@ int ade_pass_tag(const rf_geom_t *g, const int lo[3], const int hi[3], void *con
                  const rf_mat_t *m, hipStream_t st) {
     const unsigned nj = static_cast<unsigned>(hi[1] - lo[1]), nk = static_cast<unsigned>(hi[2] - lo[2]);
     const unsigned count = static_cast<unsigned>(hi[0] - lo[0]) * nj * nk;
-    const dim3 grd((count + 255) / 256);
+    const dim3 grd((count + 256u * ADE_NPT - 1) / (256u * ADE_NPT));
     using Ade = typename EntryOf<T>::ade;
     const int ncache = m->n_entries < 4096 ? m->n_entries : 4096;
     const size_t lds = (size_t)m->n_ade * sizeof(Ade) + (size_t)ncache * 8;

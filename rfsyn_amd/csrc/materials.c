@@ -595,10 +595,28 @@ static int build_tensor_edges(const materials_t *m, const fdtd_grid_t *g, rf_ted
     return 0;
 }
 
+/* Node box of the edges of one ADE set and its state region. */
+typedef struct {
+    int used;
+    size_t lo[3], hi[3];
+    size_t base, st;
+} set_box_t;
+
+static void set_layout(const set_box_t *b, long long *base, long long *st, int b0[3], int bn[3]) {
+    const int used = b && b->used;
+    *base = used ? (long long)b->base : 0;
+    *st = used ? (long long)b->st : 0;
+    for (int a = 0; a < 3; a++) {
+        b0[a] = used ? (int)b->lo[a] : 0;
+        bn[a] = used ? (int)(b->hi[a] - b->lo[a]) : 0;
+    }
+}
+
 int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_mat_dev_t *d) {
     memset(d, 0, sizeof *d);
     int rc = -1;
     combo_set_t set = {0};
+    set_box_t *sbox = NULL;
     uint32_t *tags = NULL;
     rf_tedge64_t *edges = NULL;
     size_t total = m->total;
@@ -634,11 +652,19 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
     d->n_entries = set.n;
     d->d.n_entries = (int)set.n;
 
-    /* ADE box: nodes whose entry has a dispersive component */
+    /* ADE state: the union box of the nodes with a dispersive edge and,
+     * per ADE set (= material id), the box of the nodes with an edge of
+     * that set, in which the set's state is laid out */
     int any_ade = 0;
     size_t lo[3] = {(size_t)-1, (size_t)-1, (size_t)-1}, hi[3] = {0, 0, 0};
     for (size_t e = 0; e < set.n && !any_ade; e++)
         for (int c = 0; c < 3; c++) if (set.v[e].flags[c] & RF_EDGE_ADE) any_ade = 1;
+    if (m->count) {
+        sbox = calloc(m->count, sizeof *sbox);
+        if (!sbox) goto out;
+        for (size_t id = 0; id < m->count; id++)
+            for (int a = 0; a < 3; a++) { sbox[id].lo[a] = (size_t)-1; sbox[id].hi[a] = 0; }
+    }
     if (any_ade) {
         for (size_t q = 0; q < total; q++) {
             const rf_entry64_t *en = &set.v[tags[q]];
@@ -649,20 +675,35 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
                 if (x[a] < lo[a]) lo[a] = x[a];
                 if (x[a] + 1 > hi[a]) hi[a] = x[a] + 1;
             }
+            for (int c = 0; c < 3; c++) {
+                if (!(en->flags[c] & RF_EDGE_ADE) || en->ade[c] < 0 || (size_t)en->ade[c] >= m->count) continue;
+                set_box_t *b = &sbox[en->ade[c]];
+                b->used = 1;
+                for (int a = 0; a < 3; a++) {
+                    if (x[a] < b->lo[a]) b->lo[a] = x[a];
+                    if (x[a] + 1 > b->hi[a]) b->hi[a] = x[a] + 1;
+                }
+            }
         }
         int slots = 0;
         for (size_t id = 0; id < m->count; id++)
             if (m->table[id].dispersive && m->table[id].n_poles > slots) slots = m->table[id].n_poles;
         for (int a = 0; a < 3; a++) { d->d.b0[a] = (int)lo[a]; d->d.bn[a] = (int)(hi[a] - lo[a]); }
         d->d.n_slots = slots;
-        size_t cells = (size_t)d->d.bn[0] * d->d.bn[1] * d->d.bn[2];
         size_t es = prec ? sizeof(float) : sizeof(double);
-        /* 64-element aligned arrays plus an odd number of 64-element
-         * blocks, so the 3 * slots state streams of a node do not all
-         * start at one offset modulo a large power of two */
-        size_t stride = (cells + 63) / 64 * 64 + 64 * 37;
-        d->d.ade_stride = (long long)stride;
-        size_t sb = (size_t)(slots ? slots : 1) * 3 * stride * es;
+        /* each set's (pole, component) arrays: 64-element aligned plus an
+         * odd number of 64-element blocks, so the state streams of a node
+         * do not all start at one offset modulo a large power of two */
+        size_t base = 0;
+        for (size_t id = 0; id < m->count; id++) {
+            set_box_t *b = &sbox[id];
+            if (!b->used || m->table[id].n_poles < 1) continue;
+            const size_t cells = (b->hi[0] - b->lo[0]) * (b->hi[1] - b->lo[1]) * (b->hi[2] - b->lo[2]);
+            b->st = (cells + 63) / 64 * 64 + 64 * 37;
+            b->base = base;
+            base += (size_t)m->table[id].n_poles * 3 * b->st;
+        }
+        size_t sb = (base ? base : 64) * es;
         if (dev_upload(d, NULL, sb, &d->d.J) || dev_upload(d, NULL, sb, &d->d.P)) goto out;
         d->ade_bytes = sb;
     }
@@ -727,6 +768,7 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
                 }
                 a[id].Cu = me->Cu;
                 a[id].hdt = me->hdt;
+                set_layout(sbox ? &sbox[id] : NULL, &a[id].base, &a[id].st, a[id].b0, a[id].bn);
             }
             r = dev_upload(d, a, na * sizeof *a, &dp);
             free(a);
@@ -743,6 +785,7 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
                 }
                 a[id].Cu = (float)me->Cu;
                 a[id].hdt = (float)me->hdt;
+                set_layout(sbox ? &sbox[id] : NULL, &a[id].base, &a[id].st, a[id].b0, a[id].bn);
             }
             r = dev_upload(d, a, na * sizeof *a, &dp);
             free(a);
@@ -776,6 +819,7 @@ int rf_materials_build(const materials_t *m, const fdtd_grid_t *g, int prec, rf_
     rc = 0;
 out:
     free(edges);
+    free(sbox);
     free(tags);
     free(set.v);
     free(set.slots);

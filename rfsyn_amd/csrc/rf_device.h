@@ -91,11 +91,19 @@ typedef struct {
 
 #define RF_MAX_SLOTS 8
 /* Trapezoidal-ADE coefficients of one dispersive material. */
+/* Trapezoidal-ADE coefficients of one dispersive material, and where its
+ * state lives: the set's own node box [b0, b0+bn) (the nodes with an edge
+ * of this set), J and P of pole s, component c, node (i,j,k) at element
+ * base + (3 s + c) st + ((i-b0[0]) bn[1] + (j-b0[1])) bn[2] + (k-b0[2]),
+ * poles s < n_poles only.  A box per set keeps each layer's state dense
+ * (no lines of other layers, gaps or non-dispersive stripes in it). */
 typedef struct {
     int n_poles;
     int has_p[RF_MAX_SLOTS];
     double kj[RF_MAX_SLOTS], kp[RF_MAX_SLOTS], v[RF_MAX_SLOTS];
     double Cu, hdt;
+    long long base, st;
+    int b0[3], bn[3];
 } rf_ade64_t;
 
 typedef struct {
@@ -103,6 +111,8 @@ typedef struct {
     int has_p[RF_MAX_SLOTS];
     float kj[RF_MAX_SLOTS], kp[RF_MAX_SLOTS], v[RF_MAX_SLOTS];
     float Cu, hdt;
+    long long base, st;
+    int b0[3], bn[3];
 } rf_ade32_t;
 
 typedef struct {
@@ -110,14 +120,13 @@ typedef struct {
     int tag_bytes;              /* 1, 2 or 4 */
     const void *entries;        /* rf_entry64_t or rf_entry32_t */
     const void *ade;            /* rf_ade64_t or rf_ade32_t */
-    /* ADE state box [b0, b0+bn) per axis; state arrays [slot][c][cell],
-     * one (slot, c) array every ade_stride elements (>= cells, padded so
-     * the arrays do not start at the same offset modulo large powers of 2) */
+    /* union of the ADE sets' state boxes [b0, b0+bn) per axis (the nodes
+     * with a dispersive edge); the state of each set is laid out in its
+     * own box (rf_ade64_t / rf_ade32_t) */
     int b0[3], bn[3];
-    int n_slots;
+    int n_slots;                /* largest pole count of a set */
     int n_ade;                  /* ADE coefficient sets in `ade` */
-    long long ade_stride;
-    void *J, *P;                /* n_slots * 3 * ade_stride each */
+    void *J, *P;                /* the sets' state regions, back to back */
     double vac_ca;              /* vacuum Cb = dt/(eps0 dx) when tag == NULL */
     int n_entries;              /* entries in the table */
 } rf_mat_t;
@@ -176,8 +185,6 @@ typedef struct {
 typedef struct {
     const void *entries, *ade;
     void *J, *P;
-    int b0[3], bn[3];
-    long long ade_stride;
 } rf_leap_cold_t;
 
 /* Fused leapfrog step: one kernel computes H^{n+1/2} and then E^{n+1}

@@ -481,11 +481,6 @@ static int finalize(fdtd_sim_t *s) {
             cold.ade = m->ade;
             cold.J = m->J;
             cold.P = m->P;
-            for (int a = 0; a < 3; a++) {
-                cold.b0[a] = m->b0[a];
-                cold.bn[a] = m->bn[a];
-            }
-            cold.ade_stride = m->ade_stride;
         }
         if (dalloc(s, (void **)&s->d_cold, sizeof cold) || rf_h2d(s->d_cold, &cold, sizeof cold, s->st) ||
             rf_sync(s->st))

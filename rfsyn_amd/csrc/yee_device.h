@@ -70,6 +70,14 @@ __device__ __forceinline__ unsigned load_tag(const void *p, long long idx) {
     return static_cast<unsigned>(static_cast<const TagT *>(p)[idx]);
 }
 
+// Element of the state of pole 0, component c, node (i,j,k) in the box of
+// ADE set `a`; pole s is s * 3 * a.st further on.
+template <typename Ade>
+__device__ __forceinline__ long long ade_off(const Ade &a, int c, int i, int j, int k) {
+    return a.base + (long long)c * a.st +
+           ((long long)(i - a.b0[0]) * a.bn[1] + (j - a.b0[1])) * a.bn[2] + (k - a.b0[2]);
+}
+
 // Trapezoidal ADE edge (materials.h): returns E^{n+1} and advances J, P.
 template <typename T>
 __device__ __forceinline__ T ade_edge(const typename EntryOf<T>::ade &a, T ca, T cb, T e_old,

@@ -94,11 +94,8 @@ __device__ __forceinline__ void e_component(int c, T *__restrict__ E, long long 
         if (fl & RF_EDGE_TENSOR) return;
         if (fl & RF_EDGE_ADE) {
             const Ade &a = static_cast<const Ade *>(m.ade)[en.ade[c]];
-            const long long st = m.ade_stride;
-            const long long cell = ((long long)(i - m.b0[0]) * m.bn[1] + (j - m.b0[1])) * m.bn[2] + (k - m.b0[2]);
-            const long long off = (long long)c * st + cell;
             E[idx] = ade_edge<T>(a, en.Ca[c], en.Cb[c], E[idx], dA, dB, static_cast<T *>(m.J),
-                                 static_cast<T *>(m.P), 3 * st, off);
+                                 static_cast<T *>(m.P), 3 * a.st, ade_off(a, c, i, j, k));
             return;
         }
         E[idx] = en.Ca[c] * E[idx] + en.Cb[c] * (dA - dB);

@@ -170,17 +170,23 @@ def make_net(lib, n=64, batch=256, seed=1, width=512, channels=(16, 32, 64, 128)
     return Surrogate(lib, n, 3, channels, N_COORDS, width, 2 * N_FREQS, N_PROBES, batch, seed=seed)
 
 
-def train(net, maps, coords, labels, epochs, lr=3e-4, batch=256, seed=5, log=None):
+def train(net, maps, coords, labels, epochs, lr=3e-4, batch=256, seed=5, log=None, schedule="const"):
+    """Adam over shuffled batches; schedule "cos" anneals the learning
+    rate from lr to 0 over the run (half a cosine period)."""
     rng = np.random.default_rng(seed)
     N = maps.shape[0]
     losses = []
+    total = epochs * max(1, N // batch)
+    step = 0
     for ep in range(epochs):
         order = rng.permutation(N)
         tot, cnt = 0.0, 0
         for b0 in range(0, N - batch + 1, batch):
             idx = order[b0:b0 + batch]
-            tot += net.train_step(maps[idx], coords[idx], labels[idx], lr)
+            rate = lr * 0.5 * (1.0 + math.cos(math.pi * step / total)) if schedule == "cos" else lr
+            tot += net.train_step(maps[idx], coords[idx], labels[idx], rate)
             cnt += 1
+            step += 1
         losses.append(tot / max(cnt, 1))
         if log:
             log(f"epoch {ep}: train loss {losses[-1]:.4e}")
