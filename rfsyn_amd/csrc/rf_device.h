@@ -351,11 +351,23 @@ int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const in
                         int count, double *acc_re, double *acc_im, double *cmp_re, double *cmp_im,
                         int comp_sum, rf_stream_t *s);
 
-/* Plane monitor capture: copies component comp on the plane into
- * buf[(*d_step % ring_len) * na * nb + a * nb + b] (buf_prec == prec). */
-int rf_launch_plane_capture(int prec, const rf_geom_t *g, void *const f[6], int axis, int index,
-                            int comp, void *buf, int buf_prec, int ring_len,
-                            const long long *d_step, rf_stream_t *s);
+/* Plane-monitor captures in one launch: capture q copies the plane
+ * `index` of `axis` of field array f (cells (ia, ib), the other two axes
+ * in increasing order, ia < na, ib < nb) into slot step % ring_len of
+ * its ring buf (the field's precision); start is set by the launcher. */
+#define RF_MAX_CAPS 8
+typedef struct {
+    const void *f;
+    void *buf;
+    int axis, index, na, nb;
+    long long start;
+} rf_cap_t;
+typedef struct {
+    int n;
+    rf_cap_t c[RF_MAX_CAPS];
+} rf_caps_t;
+int rf_launch_plane_captures(int prec, const rf_geom_t *g, const rf_caps_t *cs, int ring_len,
+                             const long long *d_step, rf_stream_t *s);
 
 /* Plane monitor DFT flush over `count` (<= 64) buffered steps ending at
  * step *d_step - 1; accumulators [f][cell]; tw_scratch holds

@@ -864,12 +864,24 @@ static int enqueue_step(fdtd_sim_t *s) {
                 rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
                 return -1;
     }
+    /* the monitors' planes, up to RF_MAX_CAPS per launch */
+    rf_caps_t cs;
+    memset(&cs, 0, sizeof cs);
     for (size_t q = 0; q < s->n_mons; q++) {
         smon_t *m = &s->mons[q];
-        for (int w = 0; w < 2; w++)
-            if (rf_launch_plane_capture(s->prec, &s->geom, s->f, m->axis, m->index, m->comp[w], m->ring[w],
-                                        s->prec, s->K, s->d_step, s->st))
-                return -1;
+        for (int w = 0; w < 2; w++) {
+            rf_cap_t *c = &cs.c[cs.n++];
+            c->f = s->f[m->comp[w]];
+            c->buf = m->ring[w];
+            c->axis = m->axis;
+            c->index = m->index;
+            c->na = m->na;
+            c->nb = m->nb;
+            if (cs.n == RF_MAX_CAPS || (q + 1 == s->n_mons && w == 1)) {
+                if (rf_launch_plane_captures(s->prec, &s->geom, &cs, s->K, s->d_step, s->st)) return -1;
+                cs.n = 0;
+            }
+        }
     }
     return rf_launch_probe_record(s->prec, s->f, s->d_pidx, s->d_pcomp, (int)s->n_probes, s->d_ring, s->K,
                                   s->d_td, (long long)s->td_stride, s->d_step, s->st);

@@ -358,28 +358,34 @@ k_probe_dft(const double *ring, int ring_len, int n_probes, const int *freq_prob
 
 // ----------------------------------------------------- plane monitors
 
-template <typename T, typename B>
+// Every monitor component of a step in one launch: capture q takes the
+// threads [start[q], start[q] + na nb) and copies its plane, cell (ia, ib)
+// with ib fastest, into slot step % ring_len of its ring.  One launch of
+// all planes keeps more of the plane gathers (one line per value for a z
+// plane) in flight than a launch per plane.
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_plane_capture(rf_geom_t gg, const T *f, int axis, int index, B *buf, int ring_len,
-                const long long *d_step) {
+k_plane_captures(rf_geom_t gg, rf_caps_t cs, int ring_len, const long long *d_step) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int q = 0;
+    while (q + 1 < cs.n && t >= cs.c[q + 1].start) q++;
+    const rf_cap_t &c = cs.c[q];
+    const long long local = t - c.start;
+    if (q >= cs.n || local < 0 || local >= (long long)c.na * c.nb) return;
     const Geo g = geo_of(gg);
+    const int axis = c.axis;
     const int a = (axis + 1) % 3 < (axis + 2) % 3 ? (axis + 1) % 3 : (axis + 2) % 3;
     const int b = 3 - axis - a;
-    const int na = a == 0 ? g.nx : a == 1 ? g.ny : g.nz;
-    const int nb = b == 0 ? g.nx : b == 1 ? g.ny : g.nz;
-    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ia = blockIdx.y;
-    if (ib >= nb || ia >= na) return;
     Pos x;
-    x.x[axis] = index;
+    x.x[axis] = c.index;
     if ((gg.per >> axis) & 1) {
         const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
         x.x[axis] %= n;
     }
-    x.x[a] = ia;
-    x.x[b] = ib;
+    x.x[a] = static_cast<int>(local / c.nb);
+    x.x[b] = static_cast<int>(local % c.nb);
     const long long slot = *d_step % ring_len;
-    buf[slot * (long long)na * nb + (long long)ia * nb + ib] = static_cast<B>(f[lin(g, x)]);
+    static_cast<T *>(c.buf)[slot * (long long)c.na * c.nb + local] = static_cast<const T *>(c.f)[lin(g, x)];
 }
 
 // Twiddles for the buffered block, scaled by dt:
@@ -784,22 +790,24 @@ int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const in
     return launched();
 }
 
-int rf_launch_plane_capture(int prec, const rf_geom_t *g, void *const f[6], int axis, int index,
-                            int comp, void *buf, int buf_prec, int ring_len,
-                            const long long *d_step, rf_stream_t *s) {
-    const int a = (axis + 1) % 3 < (axis + 2) % 3 ? (axis + 1) % 3 : (axis + 2) % 3;
-    const int b = 3 - axis - a;
-    dim3 grd((g->n[b] + 255) / 256, g->n[a]);
-    if (prec == 0 && buf_prec == 0)
-        hipLaunchKernelGGL((k_plane_capture<double, double>), grd, dim3(256), 0, hs(s), g[0],
-                           static_cast<const double *>(f[comp]), axis, index,
-                           static_cast<double *>(buf), ring_len, d_step);
-    else if (prec == 1 && buf_prec == 1)
-        hipLaunchKernelGGL((k_plane_capture<float, float>), grd, dim3(256), 0, hs(s), g[0],
-                           static_cast<const float *>(f[comp]), axis, index,
-                           static_cast<float *>(buf), ring_len, d_step);
+int rf_launch_plane_captures(int prec, const rf_geom_t *g, const rf_caps_t *cs, int ring_len,
+                             const long long *d_step, rf_stream_t *s) {
+    if (cs->n <= 0) return 0;
+    if (cs->n > RF_MAX_CAPS) return -1;
+    rf_caps_t c = *cs;
+    long long total = 0;
+    for (int q = 0; q < c.n; q++) {
+        if (!c.c[q].f || !c.c[q].buf || c.c[q].na <= 0 || c.c[q].nb <= 0 || c.c[q].axis < 0 || c.c[q].axis > 2)
+            return -1;
+        c.c[q].start = total;
+        total += (long long)c.c[q].na * c.c[q].nb;
+    }
+    if (total >= (1LL << 31)) return -1;
+    const dim3 grd(static_cast<unsigned>((total + 255) / 256));
+    if (prec == 0)
+        hipLaunchKernelGGL((k_plane_captures<double>), grd, dim3(256), 0, hs(s), g[0], c, ring_len, d_step);
     else
-        return -1;
+        hipLaunchKernelGGL((k_plane_captures<float>), grd, dim3(256), 0, hs(s), g[0], c, ring_len, d_step);
     return launched();
 }
 

@@ -316,3 +316,43 @@ def test_ade_split_matches_inline_fp32(gpu, monkeypatch):
     assert same_values(a["td"], b["td"])
     for k in a["fields"]:
         assert same_values(a["fields"][k], b["fields"][k]), k
+
+
+def _monitor_runs(lib, precision):
+    """Plane DFTs of two scenes: the periodic dispersive stack with z planes
+    outside and inside the dispersive layers and a y plane across them,
+    and a box without CPML, with lossy blocks and sources, with x, y and z
+    planes (the last at the padded node nz)."""
+    out = []
+    freqs = np.linspace(25e9, 55e9, 8)
+    run = NativeRun(lib, (16, 16, 44), 0.5e-3, 0.99, periodic=3, cpml=capi.CpmlParams(8, 3.0, 1.0, 1.0, 0.05, 1e-8),
+                    precision=precision, material_setup=_dispersive_setup)
+    run.add_plane_source(capi.FIELD_EX, 2, 10, "source_waveform_modulated_gaussian", modgauss_params(40e9, 30e9))
+    mons = [(run.add_plane_monitor(2, k, freqs), (16, 16, 8)) for k in (9, 22, 36)]
+    mons.append((run.add_plane_monitor(1, 7, freqs), (16, 44, 8)))
+    run.run(150)
+    out += [run.plane_dft(m, w, sh) for m, sh in mons for w in (0, 1)]
+    run.close()
+    n = (16, 20, 30)
+    run = NativeRun(lib, n, 1e-3, 0.99, precision=precision, material_setup=_region_materials(n),
+                    init_fields=lambda g: random_fill(g, 4))
+    run.add_point_source(capi.FIELD_EZ, (8, 10, 15), "source_waveform_modulated_gaussian", modgauss_params(30e9, 20e9))
+    run.add_plane_source(capi.FIELD_EY, 2, 4, "source_waveform_modulated_gaussian", modgauss_params(30e9, 20e9),
+                         amp=0.3, after_e=1)
+    mons = [(run.add_plane_monitor(0, 5, freqs), (20, 30, 8)), (run.add_plane_monitor(1, 12, freqs), (16, 30, 8)),
+            (run.add_plane_monitor(2, 4, freqs), (16, 20, 8)), (run.add_plane_monitor(2, 30, freqs), (16, 20, 8))]
+    run.run(90)
+    out += [run.plane_dft(m, w, sh) for m, sh in mons for w in (0, 1)]
+    run.close()
+    return out
+
+
+def test_plane_monitors_all_axes_fp64(gpu, oracle):
+    """Eight plane monitors (x, y and z planes, one at the padded node)
+    captured in one launch per step: DFTs against the oracle."""
+    a = _monitor_runs(gpu, capi.FDTD_FP64)
+    b = _monitor_runs(oracle, capi.FDTD_FP64)
+    for q, (x, y) in enumerate(zip(a, b)):
+        assert _dft_close(x, y), q
+    for q in range(0, len(b), 2):   # each monitor sees a field (the stack's Ey is 0 by symmetry)
+        assert max(np.max(np.abs(b[q])), np.max(np.abs(b[q + 1]))) > 0, q
