@@ -332,6 +332,11 @@ def bench_c4(args, lib):
     cells = sum(sp["n"][0] * sp["n"][1] * sp["n"][2] for sp in specs)
     runs = [build_c4_scene(lib, sp, capi.FDTD_FP32, 0) for sp in specs]
     handles = [r for r, _ in runs]
+    if not args.no_autotune:   # per scene tiling; the scenes stay on the fused step (batched)
+        t_tune = time.perf_counter()
+        for r in handles:
+            assert lib.fdtd_sim_autotune(r.sim, 2, None) == 0, lib.fdtd_last_error()
+        print(f"[c4] autotuned 96 scenes in {time.perf_counter() - t_tune:.1f} s", file=sys.stderr, flush=True)
     run_many(lib, handles, args.warmup)
     t0 = time.perf_counter()
     run_many(lib, handles, args.steps)
@@ -358,6 +363,7 @@ def bench_c4(args, lib):
                      "frac": round(byt / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": byt},
         "sequential_8_scenes_mcell_s": round(cells8 * args.steps / one / 1e6, 3),
+        "autotuned": not args.no_autotune,
     }
     print(json.dumps(out))
 
@@ -461,8 +467,10 @@ def main():
         tag = "fp32" if prec == capi.FDTD_FP32 else "fp64"
         run, pids, mons = wl.build(lib, prec, total)
         chosen = (C.c_int * 4)(0, 0, 0, 0)
+        two = C.c_int(0)
         if not args.no_autotune:
             assert lib.fdtd_sim_autotune(run.sim, 3, chosen) == 0, lib.fdtd_last_error()
+            assert lib.fdtd_sim_select_path(run.sim, 3, C.byref(two)) == 0, lib.fdtd_last_error()
         run.run(args.warmup)
         run.set_profiling(True)
         t0 = time.perf_counter()
@@ -546,6 +554,7 @@ def main():
                             "value": round(cells * args.steps / other["wall"] / 1e6, 3)},
         "fp32_vs_fp64_probe_dft_err": err,
         "tiling": r["tiling"],
+        "step_path": "fused" if r["fused"] else "two-kernel",
         "accuracy_steps": total,
     }
     if not args.no_cpu_baseline:

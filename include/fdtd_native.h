@@ -96,11 +96,23 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps);
  * launches each, and keeps the fastest for this handle.  The tiling never
  * changes the results.  chosen (may be NULL) receives rows, planes per
  * item (0: automatic), the z split and the block mode (0: independent
- * wavefronts; else cooperative blocks of all z tiles by (mode & 15) row
- * tiles, whose wavefronts stay within mode >> 4 planes of each other, 0 =
+ * wavefronts; else cooperative blocks of (mode & 15) row tiles of one z
+ * tile, whose wavefronts stay within mode >> 4 planes of each other, 0 =
  * lockstep).  0 on success (also on the two-kernel path, where there is
  * nothing to tune). */
 int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]);
+
+/* Path choice (extension) for a handle run on its own: before the first
+ * step (after fdtd_sim_autotune, if used), times the fused step against
+ * the two-kernel step (separate H and E sweeps), `launches` steps each,
+ * and keeps the two-kernel step when it is faster by 5% or more (small
+ * grids with CPML on most faces), unless the handle has flux planes or
+ * RFSYN_FUSED is set.  Both paths give the same bits; the energy audit
+ * needs the fused step.  Handles advanced together with
+ * fdtd_sim_run_many are best left on the fused step, whose lower traffic
+ * wins when many scenes share the device.  *two_kernel (may be NULL)
+ * receives 1 when the two-kernel step was kept.  0 on success. */
+int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 
 /* Copies min(n, recorded) time-domain samples; returns the count or -1. */

@@ -223,6 +223,7 @@ EXTENSION_API = {
     "fdtd_sim_energy_audit": (C.c_int, [_vp, _dp, C.POINTER(_sz)]),
     "fdtd_sim_run_many": (C.c_int, [C.POINTER(_vp), _sz, _sz]),
     "fdtd_sim_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+    "fdtd_sim_select_path": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "fdtd_sur_create": (_vp, [C.POINTER(SurConfig), C.c_uint64]),
     "fdtd_sur_destroy": (None, [_vp]),
     "fdtd_sur_n_params": (_sz, [_vp]),

@@ -234,6 +234,9 @@ int rf_cpml_build(const cpml_t *p, int prec, rf_cpml_dev_t *c) {
         return -1;
     }
     c->bytes += 2 * psec * es;
+    c->psi_sec[0] = pe;
+    c->psi_sec[1] = ph;
+    c->psi_sec_bytes = psec * es;
     for (int comp = 0; comp < 3; comp++)
         for (int s2 = 0; s2 < 2; s2++) {
             size_t off = (size_t)psi_offset(sec, comp, s2) * es;

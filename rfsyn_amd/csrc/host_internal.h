@@ -59,6 +59,8 @@ typedef struct {
     void *owned[64];
     int n_owned;
     size_t bytes;
+    void *psi_sec[2];           /* the E and H psi sections (all components) */
+    size_t psi_sec_bytes;       /* bytes of each */
 } rf_cpml_dev_t;
 
 int rf_cpml_build(const cpml_t *p, int prec, rf_cpml_dev_t *out);

@@ -783,7 +783,8 @@ static int fill_leap(fdtd_sim_t *s, rf_leap_t *out, const rf_mat_t **mout, int a
     return 0;
 }
 
-static int enqueue_fused(fdtd_sim_t *s) {
+/* The fused step's launches (A -> B), without the buffer swap. */
+static int launch_fused(fdtd_sim_t *s) {
     rf_leap_t a;
     const rf_mat_t *m;
     int alo[3], ahi[3];
@@ -800,7 +801,11 @@ static int enqueue_fused(fdtd_sim_t *s) {
                               s->n_srcs, s->d_step, s->st))
             return -1;
     }
-    if (mark(s, 5)) return -1;
+    return mark(s, 5);
+}
+
+static int enqueue_fused(fdtd_sim_t *s) {
+    if (launch_fused(s)) return -1;
     /* B now holds state n+1 */
     for (int q = 0; q < 6; q++) {
         void *t = s->f[q];
@@ -1039,6 +1044,47 @@ int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps) {
  * hold their initial values before the first step (B = A, the rest zero)
  * and are restored afterwards.  The tiling changes which wavefront updates
  * a node, never the arithmetic, so results do not depend on it. */
+/* State before the first step again after timing launches: the queue
+ * counters, B = A (a_pristine) or A = B, and the psi memories and ADE
+ * state zero. */
+static int restore_initial(fdtd_sim_t *s, int a_pristine) {
+    const size_t fb = 6 * s->cells * s->es;
+    s->parity = 0;
+    s->zpar = 0;
+    return rf_memset(s->d_ctr, 0, 32 * sizeof(unsigned), s->st) ||
+           (a_pristine ? rf_d2d(s->blk2, s->blk, fb, s->st) : rf_d2d(s->blk, s->blk2, fb, s->st)) ||
+           (s->zpsi && rf_memset(s->zpsi, 0, (s->nx + 1) * (s->ny + 1) * (size_t)s->cp.d.len[2] * 6 * s->es,
+                                 s->st)) ||
+           (s->has_cpml && s->cp.psi_sec_bytes &&
+            (rf_memset(s->cp.psi_sec[0], 0, s->cp.psi_sec_bytes, s->st) ||
+             rf_memset(s->cp.psi_sec[1], 0, s->cp.psi_sec_bytes, s->st))) ||
+           (s->has_mat && s->mat.d.J &&
+            (rf_memset(s->mat.d.J, 0, s->mat.ade_bytes, s->st) || rf_memset(s->mat.d.P, 0, s->mat.ade_bytes, s->st))) ||
+           rf_sync(s->st);
+}
+
+/* Milliseconds of `launches` steps' field-update launches on one path (one
+ * warm-up step first): the fused step's launches (leapfrog, ADE pass,
+ * shell) or the two-kernel step's H and E updates.  The kernels the paths
+ * share (sources, tensor, monitors, probes) are left out. */
+static int time_path(fdtd_sim_t *s, int fused, int launches, rf_event_t *e0, rf_event_t *e1, float *ms) {
+    const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
+    const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
+    const double cb = s->dt / (MU0 * s->dx);
+    for (int w = 0; w <= launches; w++) {
+        if (w == 1 && rf_event_record(e0, s->st)) return -1;
+        if (fused) {
+            if (launch_fused(s)) return -1;
+            s->parity ^= 1;
+            s->zpar ^= 1;
+        } else if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st) ||
+                   rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) {
+            return -1;
+        }
+    }
+    return rf_event_record(e1, s->st) || rf_sync(s->st) || rf_event_elapsed_ms(e0, e1, ms);
+}
+
 int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]) {
     if (!s || launches < 1) return -1;
     if (s->steps) {
@@ -1087,24 +1133,14 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]) {
                     best[3] = coops[gi];
                 }
             }
-    rf_event_destroy(e0);
-    rf_event_destroy(e1);
     /* restore the state before the first step */
-    s->parity = 0;
-    s->zpar = 0;
-    if (!rc && (rf_memset(s->d_ctr, 0, 32 * sizeof(unsigned), s->st) ||
-                rf_d2d(s->blk2, s->blk, 6 * s->cells * s->es, s->st) ||
-                (s->zpsi && rf_memset(s->zpsi, 0, (s->nx + 1) * (s->ny + 1) * (size_t)s->cp.d.len[2] * 6 * s->es,
-                                      s->st)) ||
-                (s->has_mat && s->mat.d.J &&
-                 (rf_memset(s->mat.d.J, 0, s->mat.ade_bytes, s->st) ||
-                  rf_memset(s->mat.d.P, 0, s->mat.ade_bytes, s->st))) ||
-                rf_sync(s->st)))
-        rc = -1;
+    if (!rc && restore_initial(s, 1)) rc = -1;
     s->leap_rows = rc ? r0 : best[0];
     s->leap_chunk = rc ? c0 : best[1];
     s->leap_zsplit = rc ? z0 : best[2];
     s->leap_coop = rc ? g0 : best[3];
+    rf_event_destroy(e0);
+    rf_event_destroy(e1);
     if (chosen) {
         chosen[0] = s->leap_rows;
         chosen[1] = s->leap_chunk;
@@ -1112,6 +1148,35 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]) {
         chosen[3] = s->leap_coop;
     }
     return rc;
+}
+
+/* The fused step at its tiling against the two-kernel step, for a handle
+ * run on its own: the two-kernel step wins on small grids whose fused box
+ * holds few of the cells (the CPML shell is updated node by node, and a
+ * lone small step is bound by launch latency).  Both give the same bits.
+ * Kept fused with flux planes (they need the fused step's pair of states)
+ * or when RFSYN_FUSED is set. */
+int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel) {
+    if (!s || launches < 1) return -1;
+    if (s->steps) {
+        rf_set_error("fdtd_sim_select_path must run before the first step");
+        return -1;
+    }
+    if (run_prepare(s, 1)) return -1;
+    if (s->fused && s->n_flux == 0 && !getenv("RFSYN_FUSED")) {
+        rf_event_t *e0 = rf_event_create(), *e1 = rf_event_create();
+        int rc = (e0 && e1) ? 0 : -1;
+        float t_fused = 0.f, t_two = 0.f;
+        if (!rc && (time_path(s, 1, launches, e0, e1, &t_fused) || restore_initial(s, 1) ||
+                    time_path(s, 0, launches, e0, e1, &t_two) || restore_initial(s, 0)))
+            rc = -1;
+        rf_event_destroy(e0);
+        rf_event_destroy(e1);
+        if (rc) return -1;
+        if (t_two < 0.95f * t_fused) s->fused = 0;
+    }
+    if (two_kernel) *two_kernel = !s->fused;
+    return 0;
 }
 
 int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {

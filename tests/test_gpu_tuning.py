@@ -34,6 +34,9 @@ def _closed(lib, precision, tune):
     run.add_point_source(capi.FIELD_EY, (11, 10, 4), "source_waveform_modulated_gaussian", modgauss_params(40e9, 30e9))
     if tune:
         assert run.lib.fdtd_sim_autotune(run.sim, 1, None) == 0
+        two = C.c_int(-1)
+        assert run.lib.fdtd_sim_select_path(run.sim, 2, C.byref(two)) == 0
+        assert two.value in (0, 1)
     run.run(60)
     out = {"fields": run.fields()}
     run.close()
@@ -96,5 +99,20 @@ def test_cooperative_blocks_fp32_bitwise_independent(gpu, coop, monkeypatch):
     monkeypatch.setenv("RFSYN_LEAP_COOP", str(coop))
     b, _ = _stack(gpu, capi.FDTD_FP32, False)
     assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
+
+
+@pytest.mark.parametrize("precision", [capi.FDTD_FP32, capi.FDTD_FP64])
+@pytest.mark.parametrize("scene", [_stack, _closed])
+def test_two_kernel_step_bitwise_fused(gpu, precision, scene, monkeypatch):
+    """The autotuner may keep the two-kernel step instead of the fused one:
+    both give the same bits, in FP32 as in FP64."""
+    monkeypatch.setenv("RFSYN_FUSED", "1")
+    a, _ = scene(gpu, precision, False)
+    monkeypatch.setenv("RFSYN_FUSED", "0")
+    b, _ = scene(gpu, precision, False)
+    if "td" in a:
+        assert same_values(a["td"], b["td"])
     for k in a["fields"]:
         assert same_values(a["fields"][k], b["fields"][k]), k
