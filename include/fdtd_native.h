@@ -117,6 +117,25 @@ size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 
 /* Copies min(n, recorded) time-domain samples; returns the count or -1. */
 long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n);
+
+/* Dataset output stage on the device (extension): the first n recorded
+ * samples of time-domain probe `probe` as the FDTD stage's complex float
+ * signal (cli/fdtd_stage.c:20-24: I = (float)v, Q = 0) and, when
+ * add_noise, the AWGN stage of the reference on it (engine/noise/
+ * awgn.c:16-44): per-axis standard deviation sqrt(P / 10^(snr_db/10) / 2)
+ * from the signal's mean power P, and per sample two uniforms of
+ * PCG32(seed, stream) (engine/core/rng.c; the CLI passes the example index
+ * as the stream) into a float Box-Muller pair.  out_iq (host) receives 2n
+ * floats, I and Q interleaved: the layout of the .iq files.  The power and
+ * the uniforms are the reference's bits; the noise is within a few float
+ * ulps of it (device logf, cosf, sinf).  -1 if fewer than n samples are
+ * recorded. */
+int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                      uint64_t stream, float *out_iq);
+/* Writes n I/Q samples (2n floats) to path as raw float32, I then Q, the
+ * .iq file of the CLI (cli/main.c:70-85; little-endian on x86-64).  0 on
+ * success, -1 on any failure. */
+int fdtd_write_iq(const char *path, const float *iq, size_t n);
 /* Accumulated DFT (acc_re, acc_im exactly as probe_dft_t holds them). */
 int fdtd_sim_probe_dft(fdtd_sim_t *s, int probe, double *re, double *im);
 /* Plane DFT of tangential component `which` (0: lower axis, 1: upper);

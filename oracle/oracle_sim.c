@@ -509,3 +509,63 @@ int fdtd_sim_energy_audit(fdtd_sim_t *s, double out[5], size_t *step) {
 size_t fdtd_sim_device_bytes(const fdtd_sim_t *s) { (void)s; return 0; }
 
 const char *fdtd_last_error(void) { return ""; }
+
+/* ---- Dataset output stage (test infrastructure) ----
+ * PCG32 (engine/core/rng.c:3-23) and the AWGN stage (engine/noise/
+ * awgn.c:16-44) restated on a host I/Q buffer, with glibc's pow, sqrt,
+ * sqrtf, logf, cosf and sinf as the reference calls them; the samples of
+ * the FDTD stage are cli/fdtd_stage.c:20-24 (I = (float)v + 0, Q = 0). */
+typedef struct { uint64_t state, inc; } opcg_t;
+
+static uint32_t opcg_next(opcg_t *r) {
+    const uint64_t old = r->state;
+    r->state = old * 6364136223846793005ULL + r->inc;
+    const uint32_t x = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (x >> rot) | (x << ((-rot) & 31u));
+}
+
+static float opcg_uniform(opcg_t *r) { return (float)(opcg_next(r) >> 8) / (float)(1u << 24); }
+
+/* AWGN on n interleaved I/Q samples in place (awgn.c:20-41). */
+void oracle_awgn_iq(float *iq, size_t n, float snr_db, uint64_t seed, uint64_t stream) {
+    if (!iq || n == 0) return;
+    double signal_power = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const float mag = hypotf(iq[2 * i], iq[2 * i + 1]);   /* cabsf */
+        signal_power += (double)mag * (double)mag;
+    }
+    signal_power /= (double)n;
+    const double snr_linear = pow(10.0, (double)snr_db / 10.0);
+    const double noise_power = (snr_linear > 0.0) ? signal_power / snr_linear : 0.0;
+    const double per_axis_std = sqrt(noise_power / 2.0);
+    opcg_t r = {0u, (stream << 1u) | 1u};   /* rng_seed, rng.c:3-9 */
+    opcg_next(&r);
+    r.state += seed;
+    opcg_next(&r);
+    for (size_t i = 0; i < n; i++) {
+        float u1 = opcg_uniform(&r);
+        const float u2 = opcg_uniform(&r);
+        if (u1 <= 0.0f) u1 = 1e-9f;
+        const float rad = sqrtf(-2.0f * logf(u1));
+        const float nre = (float)per_axis_std * rad * cosf(2.0f * (float)3.14159265358979323846 * u2);
+        const float nim = (float)per_axis_std * rad * sinf(2.0f * (float)3.14159265358979323846 * u2);
+        /* samples[i] += nre + nim * I */
+        const float cre = nre + nim * 0.0f, cim = nim * 1.0f;
+        iq[2 * i] = iq[2 * i] + cre;
+        iq[2 * i + 1] = iq[2 * i + 1] + cim;
+    }
+}
+
+int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                      uint64_t stream, float *out_iq) {
+    if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out_iq)) return -1;
+    oprobe_t *p = &s->probes[probe];
+    if (n > p->td_n) return -1;
+    for (size_t i = 0; i < n; i++) {
+        out_iq[2 * i] = (float)p->td[i] + 0.0f;
+        out_iq[2 * i + 1] = 0.0f;
+    }
+    if (add_noise) oracle_awgn_iq(out_iq, n, snr_db, seed, stream);
+    return 0;
+}

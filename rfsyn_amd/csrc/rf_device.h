@@ -351,6 +351,16 @@ int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const in
                         int count, double *acc_re, double *acc_im, double *cmp_re, double *cmp_im,
                         int comp_sum, rf_stream_t *s);
 
+/* Dataset output stage (output.hip): the mean power of n time-domain
+ * samples as complex floats (re = (float)v, im = 0) summed in sample order
+ * into *power (device); then the n I/Q samples into out (device, 2n
+ * floats) with, when noise, the AWGN pair of draws 2i, 2i+1 of the PCG32
+ * stream whose seeded state is state0 (increment inc) at per-axis
+ * standard deviation std_f. */
+int rf_launch_iq_power(const double *td, long long n, double *power, rf_stream_t *s);
+int rf_launch_iq_noise(const double *td, long long n, int noise, float std_f, uint64_t state0, uint64_t inc,
+                       float *out, rf_stream_t *s);
+
 /* Plane-monitor captures in one launch: capture q copies the plane
  * `index` of `axis` of field array f (cells (ia, ib), the other two axes
  * in increasing order, ia < na, ib < nb) into slot step % ring_len of

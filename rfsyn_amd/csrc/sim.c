@@ -20,6 +20,7 @@
 #include "constants.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <limits.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1232,6 +1233,52 @@ long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n) {
     size_t c = n < have ? n : have;
     if (c && rf_d2h(out, s->d_td + (size_t)probe * s->td_stride, c * sizeof(double), s->st)) return -1;
     return (long)c;
+}
+
+int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                      uint64_t stream, float *out_iq) {
+    if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out_iq)) return -1;
+    size_t have = s->steps < s->probes[probe].td_cap ? s->steps : s->probes[probe].td_cap;
+    if (n > have) {
+        rf_set_error("probe %d holds %zu recorded samples, %zu asked", probe, have, n);
+        return -1;
+    }
+    if (n == 0) return 0;
+    const double *td = s->d_td + (size_t)probe * s->td_stride;
+    double *d_pow = NULL;
+    float *d_iq = NULL;
+    int rc = rf_malloc((void **)&d_pow, sizeof(double)) || rf_malloc((void **)&d_iq, 2 * n * sizeof(float));
+    float std_f = 0.f;
+    uint64_t state0 = 0, inc = (stream << 1u) | 1u;
+    if (!rc && add_noise) {
+        /* awgn.c:20-31: mean power in sample order (device), then the
+         * standard deviation with the host's pow and sqrt */
+        double power = 0.0;
+        rc = rf_launch_iq_power(td, (long long)n, d_pow, s->st) || rf_d2h(&power, d_pow, sizeof power, s->st);
+        if (!rc) {
+            const double snr_linear = pow(10.0, (double)snr_db / 10.0);
+            const double noise_power = (snr_linear > 0.0) ? power / snr_linear : 0.0;
+            std_f = (float)sqrt(noise_power / 2.0);
+            /* rng_seed (rng.c:3-9): the state before the first draw */
+            state0 = inc;   /* the first step from state 0 */
+            state0 += seed;
+            state0 = state0 * 6364136223846793005ULL + inc;
+        }
+    }
+    if (!rc) rc = rf_launch_iq_noise(td, (long long)n, add_noise != 0, std_f, state0, inc, d_iq, s->st) ||
+                  rf_d2h(out_iq, d_iq, 2 * n * sizeof(float), s->st);
+    rf_free(d_pow);
+    rf_free(d_iq);
+    return rc ? -1 : 0;
+}
+
+int fdtd_write_iq(const char *path, const float *iq, size_t n) {
+    if (!path || (n && !iq)) return -1;
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    const size_t written = n ? fwrite(iq, 2 * sizeof(float), n, f) : 0;
+    const int closed = fclose(f);
+    return (written == n && closed == 0) ? 0 : -1;
 }
 
 int fdtd_sim_probe_dft(fdtd_sim_t *s, int probe, double *re, double *im) {
