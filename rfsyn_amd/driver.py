@@ -89,7 +89,12 @@ def modgauss_params(f0, bw):
 
 
 def c1_probes(n=64, src=(32, 32, 32), count=8, seed=1, lo=10, hi=54, min_dist=15.0):
-    """Seeded Ez probe nodes in [lo,hi)^3 at least min_dist cells from the source."""
+    """Seeded Ez probe nodes in [lo,hi)^3 at least min_dist cells from the source.
+    Raises ValueError when the box holds no such node (rejection sampling
+    would never end)."""
+    far = max(math.dist((x, y, z), src) for x in (lo, hi - 1) for y in (lo, hi - 1) for z in (lo, hi - 1))
+    if hi <= lo or far < min_dist:
+        raise ValueError(f"no probe node in [{lo},{hi})^3 is {min_dist} cells from {src}")
     rng = Pcg32(seed, 0)
     out = []
     while len(out) < count:

@@ -229,3 +229,12 @@ def test_diagonal_tensor3_is_scalar(oracle):
         oracle.fdtd_grid_destroy(g)
     for k in res[0]:
         assert np.allclose(res[0][k], res[1][k], rtol=1e-12, atol=1e-13 * np.max(np.abs(res[1][k])))
+
+
+def test_probe_sampler_refuses_an_impossible_box():
+    """The seeded probe sampler raises, rather than looping, when no node of
+    its box lies far enough from the source (a 32^3 grid's [10, 22)^3)."""
+    from rfsyn_amd.driver import c1_probes
+    with pytest.raises(ValueError):
+        c1_probes(32, (16, 16, 16), 2, lo=10, hi=22)
+    assert len(c1_probes(48, (24, 24, 24), 2, lo=10, hi=38)) == 2
