@@ -282,6 +282,10 @@ def bench_c5(args, lib):
     mte, cte, lte, fdtd_te = sp.label_scenes(lib, list(range(n_tr, n_tr + n_te)), steps)
     label_wall = time.perf_counter() - t0
     log(f"labelled {n_tr + n_te} scenes: {label_wall:.1f} s wall, {fdtd_tr + fdtd_te:.1f} s of FDTD")
+    # the plain single-scene solver on the same scenes: one handle at a time
+    n_one = 16
+    _, _, _, fdtd_one = sp.label_scenes(lib, list(range(n_one)), steps, batch=1)
+    log(f"single-scene FDTD: {n_one} scenes in {fdtd_one:.2f} s")
     scale = float(np.std(ltr))
     net = sp.make_net(lib, batch=1024)
     t1 = time.perf_counter()
@@ -311,6 +315,8 @@ def bench_c5(args, lib):
                    "epochs": args.c5_epochs, "lr": args.c5_lr, "lr_schedule": "cosine", "batch_train": bt,
                    "batch_infer": 1024},
         "fdtd_scenes_per_s": round(fdtd_rate, 3),
+        "fdtd_single_scene_per_s": round(n_one / fdtd_one, 3),
+        "batching_gain": round(fdtd_rate / (n_one / fdtd_one), 3),
         "speedup_vs_fdtd": round(sur_rate / fdtd_rate, 3),
         "network_only_scenes_per_s": round(n_te / max(inf_t["network"], 1e-9), 1),
         "held_out_rel_l2_median": float(np.median(err)), "held_out_rel_l2_p90": float(np.percentile(err, 90)),

@@ -18,6 +18,8 @@
 #define RFSYN_FDTD_NATIVE_H
 
 #include <stddef.h>
+#include <stdint.h>
+#include "simulate.h"
 #include "grid.h"
 #include "cpml.h"
 #include "materials.h"
@@ -132,6 +134,23 @@ long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n);
  * recorded. */
 int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
                       uint64_t stream, float *out_iq);
+/* The CLI's FDTD stage and, optionally, its AWGN stage in one call
+ * (extension; cli/fdtd_stage.c:10-29 and settings.c:52-80): the echo of
+ * fdtd_simulate_echo run on the device in `precision` (FP64 gives the
+ * reference's samples), returned as the stage's I/Q signal (2 n_steps
+ * floats) with, when add_noise, the AWGN stage on the device
+ * (fdtd_sim_probe_iq; the CLI passes the example index as stream).
+ * sample_rate_out (may be NULL) receives 1/dt.  0 on success, -1 on the
+ * errors of fdtd_simulate_echo or a bad option. */
+typedef struct {
+    int precision;       /* FDTD_FP64 or FDTD_FP32 */
+    int add_noise;
+    float snr_db;
+    uint64_t seed, stream;
+} fdtd_echo_options_t;
+int fdtd_simulate_echo_iq(const fdtd_sim_config_t *cfg, const fdtd_echo_options_t *opt, float *out_iq,
+                          double *sample_rate_out);
+
 /* Writes n I/Q samples (2n floats) to path as raw float32, I then Q, the
  * .iq file of the CLI (cli/main.c:70-85; little-endian on x86-64).  0 on
  * success, -1 on any failure. */

@@ -1406,10 +1406,12 @@ size_t fdtd_sim_device_bytes(const fdtd_sim_t *s) { return s ? s->bytes : 0; }
 
 /* ------------------------------------------------- fdtd_simulate_echo */
 
-/* engine/fdtd/simulate.c:10-66 on the device-resident path in FP64. */
-int fdtd_simulate_echo_with_setup(const fdtd_sim_config_t *cfg, double *out_td, double *sample_rate_out,
-                                  fdtd_material_setup_fn setup, void *context) {
-    if (!cfg || !out_td || cfg->n_steps == 0) return -1;
+/* engine/fdtd/simulate.c:10-66 on the device-resident path: the handle
+ * in `precision`, stepped n_steps; then either the probe series (out_td)
+ * or the output stage's I/Q samples (out_iq, with `noise` if non-NULL). */
+static int echo_run(const fdtd_sim_config_t *cfg, int precision, fdtd_material_setup_fn setup, void *context,
+                    double *out_td, float *out_iq, const fdtd_echo_options_t *noise, double *sample_rate_out) {
+    if (!cfg || (!out_td && !out_iq) || cfg->n_steps == 0) return -1;
     fdtd_grid_t *g = fdtd_grid_create(cfg->nx, cfg->ny, cfg->nz, cfg->dx, cfg->courant_safety);
     if (!g) return -1;
     int rc = -1;
@@ -1426,7 +1428,7 @@ int fdtd_simulate_echo_with_setup(const fdtd_sim_config_t *cfg, double *out_td, 
     source_gaussian_deriv_params_t sp = {cfg->t0_over_dt * g->dt, cfg->tau_over_dt * g->dt};
     source_t src = source_make_point(cfg->src_i, cfg->src_j, cfg->src_k, cfg->src_component, SRC_SOFT, 1.0,
                                      source_waveform_gaussian_derivative, &sp);
-    fdtd_sim_options_t opt = {FDTD_FP64, 0, 0};
+    fdtd_sim_options_t opt = {precision, 0, 0};
     s = fdtd_sim_create(g, pml, mat, &opt);
     if (!s) goto out;
     /* source_apply ignores H components (source.c:45-49) */
@@ -1435,7 +1437,10 @@ int fdtd_simulate_echo_with_setup(const fdtd_sim_config_t *cfg, double *out_td, 
                            NULL, 0) != 0)
         goto out;
     if (fdtd_sim_run(s, cfg->n_steps) != 0) goto out;
-    if (fdtd_sim_probe_td(s, 0, out_td, cfg->n_steps) != (long)cfg->n_steps) goto out;
+    if (out_td && fdtd_sim_probe_td(s, 0, out_td, cfg->n_steps) != (long)cfg->n_steps) goto out;
+    if (out_iq && fdtd_sim_probe_iq(s, 0, cfg->n_steps, noise && noise->add_noise, noise ? noise->snr_db : 0.f,
+                                    noise ? noise->seed : 0, noise ? noise->stream : 0, out_iq) != 0)
+        goto out;
     if (sample_rate_out) *sample_rate_out = 1.0 / g->dt;
     rc = 0;
 out:
@@ -1446,6 +1451,18 @@ out:
     return rc;
 }
 
+int fdtd_simulate_echo_with_setup(const fdtd_sim_config_t *cfg, double *out_td, double *sample_rate_out,
+                                  fdtd_material_setup_fn setup, void *context) {
+    if (!out_td) return -1;
+    return echo_run(cfg, FDTD_FP64, setup, context, out_td, NULL, NULL, sample_rate_out);
+}
+
 int fdtd_simulate_echo(const fdtd_sim_config_t *cfg, double *out_td, double *sample_rate_out) {
     return fdtd_simulate_echo_with_setup(cfg, out_td, sample_rate_out, NULL, NULL);
+}
+
+int fdtd_simulate_echo_iq(const fdtd_sim_config_t *cfg, const fdtd_echo_options_t *opt, float *out_iq,
+                          double *sample_rate_out) {
+    if (!out_iq || !opt || (opt->precision != FDTD_FP64 && opt->precision != FDTD_FP32)) return -1;
+    return echo_run(cfg, opt->precision, NULL, NULL, NULL, out_iq, opt, sample_rate_out);
 }

@@ -156,3 +156,42 @@ def test_device_output_stage(gpu, oracle, stream):
     assert gpu.fdtd_sim_probe_iq(a.sim, pa, n + 1, 1, 15.0, 42, 0, out.ctypes.data_as(C.POINTER(C.c_float))) == -1
     a.close()
     b.close()
+
+
+STAGE_SO = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                        "libstage_native.so")
+
+
+class EchoOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("add_noise", C.c_int), ("snr_db", C.c_float), ("seed", C.c_uint64),
+                ("stream", C.c_uint64)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("noise", [0, 1])
+def test_native_stage_matches_reference_chain(gpu, ref, noise):
+    """rfsyn_amd/cli/fdtd_stage_native.c, built with the reference's own
+    signal code (oracle/Makefile), as the CLI would run it: the default
+    preset's FDTD stage (and AWGN at 15 dB, seed 42, stream 3, done on the
+    device) against the reference's fdtd_stage (+ awgn) transforms:
+    bitwise without noise, within 4e-6 of the noise magnitude with it."""
+    if not os.path.exists(STAGE_SO):
+        pytest.skip("stage harness not built (oracle/Makefile target ref)")
+    stage = C.CDLL(STAGE_SO)
+    stage.fdtd_stage_native_create.restype = Transform
+    stage.fdtd_stage_native_create.argtypes = [C.POINTER(capi.SimConfig), C.POINTER(EchoOptions)]
+    ref.fdtd_stage_create.restype = Transform
+    ref.fdtd_stage_create.argtypes = [C.POINTER(capi.SimConfig)]
+    cfg = _preset_cfg()
+    opt = EchoOptions(capi.FDTD_FP64, noise, 15.0, 42, 3)
+    got = _ref_chain(stage, [stage.fdtd_stage_native_create(C.byref(cfg), C.byref(opt))], 1024)
+    want = _ref_chain(ref, [ref.fdtd_stage_create(C.byref(cfg))] + ([_ref_awgn(ref, 15.0, 42, 3)] if noise else []),
+                      1024)
+    assert got.shape == want.shape == (200, 2)
+    if not noise:
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    else:
+        clean = _ref_chain(ref, [ref.fdtd_stage_create(C.byref(cfg))], 1024)
+        nz = want - clean
+        assert np.max(np.abs(nz)) > 0
+        assert np.max(np.abs(got - want)) <= 4e-6 * np.max(np.abs(nz))
