@@ -1083,9 +1083,17 @@ template <typename T, typename TagT, int NS>
 __global__ void __launch_bounds__(256)
 k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, unsigned count,
            const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
-           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache) {
+           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, unsigned ade_blocks, rf_caps_t cs,
+           int cap_ring, const long long *d_step) {
     using Entry = typename EntryOf<T>::type;
     using Ade = typename EntryOf<T>::ade;
+    // the blocks past the ADE range capture the plane monitors of the step
+    // (their planes lie outside the ADE box, so their values are final)
+    if (blockIdx.x >= ade_blocks) {
+        plane_capture_elem<T>(gg, cs, (long long)(blockIdx.x - ade_blocks) * blockDim.x + threadIdx.x, cap_ring,
+                              *d_step);
+        return;
+    }
     // every ADE set and, for the first ncache entries of the material
     // table, the flags and ADE set ids of the three components (one byte
     // each), staged in LDS
@@ -1206,10 +1214,13 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
 
 template <typename T, typename TagT>
 int ade_pass_tag(const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3], void *const eb[3],
-                 const rf_mat_t *m, hipStream_t st) {
+                 const rf_mat_t *m, const rf_caps_t &cs, int cap_ring, const long long *d_step, hipStream_t st) {
     const unsigned nj = static_cast<unsigned>(hi[1] - lo[1]), nk = static_cast<unsigned>(hi[2] - lo[2]);
     const unsigned count = static_cast<unsigned>(hi[0] - lo[0]) * nj * nk;
-    const dim3 grd((count + 256u * ADE_NPT - 1) / (256u * ADE_NPT));
+    const unsigned ade_blocks = (count + 256u * ADE_NPT - 1) / (256u * ADE_NPT);
+    long long cap_total = 0;
+    for (int q = 0; q < cs.n; q++) cap_total += (long long)cs.c[q].na * cs.c[q].nb;
+    const dim3 grd(ade_blocks + static_cast<unsigned>((cap_total + 255) / 256));
     using Ade = typename EntryOf<T>::ade;
     const int ncache = m->n_entries < 4096 ? m->n_entries : 4096;
     const size_t lds = (size_t)m->n_ade * sizeof(Ade) + (size_t)ncache * 8;
@@ -1217,7 +1228,7 @@ int ade_pass_tag(const rf_geom_t *g, const int lo[3], const int hi[3], void *con
     hipLaunchKernelGGL((k_ade_pass<T, TagT, NS>), grd, dim3(256), lds, st, *g, lo[0], lo[1], lo[2], nj, nk,    \
                        count, static_cast<const T *>(ea[0]), static_cast<const T *>(ea[1]),                    \
                        static_cast<const T *>(ea[2]), static_cast<T *>(eb[0]), static_cast<T *>(eb[1]),        \
-                       static_cast<T *>(eb[2]), *m, ncache)
+                       static_cast<T *>(eb[2]), *m, ncache, ade_blocks, cs, cap_ring, d_step)
     if (m->n_slots <= 1) RF_ADE_PASS(1);
     else if (m->n_slots == 2) RF_ADE_PASS(2);
     else if (m->n_slots <= 4) RF_ADE_PASS(4);
@@ -1228,11 +1239,11 @@ int ade_pass_tag(const rf_geom_t *g, const int lo[3], const int hi[3], void *con
 
 template <typename T>
 int ade_pass_t(const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3], void *const eb[3],
-               const rf_mat_t *m, hipStream_t st) {
+               const rf_mat_t *m, const rf_caps_t &cs, int cap_ring, const long long *d_step, hipStream_t st) {
     switch (m->tag_bytes) {
-    case 1: return ade_pass_tag<T, uint8_t>(g, lo, hi, ea, eb, m, st);
-    case 2: return ade_pass_tag<T, uint16_t>(g, lo, hi, ea, eb, m, st);
-    case 4: return ade_pass_tag<T, uint32_t>(g, lo, hi, ea, eb, m, st);
+    case 1: return ade_pass_tag<T, uint8_t>(g, lo, hi, ea, eb, m, cs, cap_ring, d_step, st);
+    case 2: return ade_pass_tag<T, uint16_t>(g, lo, hi, ea, eb, m, cs, cap_ring, d_step, st);
+    case 4: return ade_pass_tag<T, uint32_t>(g, lo, hi, ea, eb, m, cs, cap_ring, d_step, st);
     }
     return -1;
 }
@@ -1500,7 +1511,23 @@ extern "C" int rf_launch_shell_h(int prec, const rf_geom_t *g, const rf_shell_t 
 }
 
 extern "C" int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3],
-                                  void *const eb[3], const rf_mat_t *mat, rf_stream_t *s) {
+                                  void *const eb[3], const rf_mat_t *mat, const rf_caps_t *caps, int cap_ring,
+                                  const long long *d_step, rf_stream_t *s) {
+    rf_caps_t cs;
+    cs.n = 0;
+    if (caps && caps->n > 0) {
+        if (caps->n > RF_MAX_CAPS || cap_ring <= 0 || !d_step) return -1;
+        cs = *caps;
+        long long total = 0;
+        for (int q = 0; q < cs.n; q++) {
+            if (!cs.c[q].f || !cs.c[q].buf || cs.c[q].na <= 0 || cs.c[q].nb <= 0 || cs.c[q].axis < 0 ||
+                cs.c[q].axis > 2)
+                return -1;
+            cs.c[q].start = total;
+            total += (long long)cs.c[q].na * cs.c[q].nb;
+        }
+        if (total >= (1LL << 31)) return -1;
+    }
     long long count = 1;
     for (int q = 0; q < 3; q++) {
         if (hi[q] < lo[q] || lo[q] < 0) return -1;
@@ -1511,7 +1538,8 @@ extern "C" int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3],
         mat->n_ade < 1 || mat->n_ade > ADE_PASS_MAX_SETS)
         return -1;
     hipStream_t st = rf_raw_stream(s);
-    return prec == 0 ? ade_pass_t<double>(g, lo, hi, ea, eb, mat, st) : ade_pass_t<float>(g, lo, hi, ea, eb, mat, st);
+    return prec == 0 ? ade_pass_t<double>(g, lo, hi, ea, eb, mat, cs, cap_ring, d_step, st)
+                     : ade_pass_t<float>(g, lo, hi, ea, eb, mat, cs, cap_ring, d_step, st);
 }
 
 extern "C" int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *const ea[3],

@@ -308,13 +308,33 @@ int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *
                       const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
                       rf_stream_t *s);
 
+/* Plane-monitor captures in one launch: capture q copies the plane
+ * `index` of `axis` of field array f (cells (ia, ib), the other two axes
+ * in increasing order, ia < na, ib < nb) into slot step % ring_len of
+ * its ring buf (the field's precision); start is set by the launcher. */
+#define RF_MAX_CAPS 8
+typedef struct {
+    const void *f;
+    void *buf;
+    int axis, index, na, nb;
+    long long start;
+} rf_cap_t;
+typedef struct {
+    int n;
+    rf_cap_t c[RF_MAX_CAPS];
+} rf_caps_t;
+
 /* ADE pass of the split step: for the nodes of [lo, hi) (inside the ADE
  * state box and the fused box), every dispersive edge whose E^{n+1} in eb
  * holds the plain part Ca e^n + Cb (dA - dB) of its update is completed
  * with E^n from ea, and J, P advance (ade_edge, yee_device.h).  At most
- * 64 ADE sets and 1 <= n_slots. */
+ * 64 ADE sets and 1 <= n_slots.  When caps is given (n > 0), extra blocks
+ * of the same launch take the plane captures of step *d_step into rings
+ * of cap_ring slots; every captured plane must lie outside [lo, hi), whose
+ * values the pass is still writing. */
 int rf_launch_ade_pass(int prec, const rf_geom_t *g, const int lo[3], const int hi[3], void *const ea[3],
-                       void *const eb[3], const rf_mat_t *mat, rf_stream_t *s);
+                       void *const eb[3], const rf_mat_t *mat, const rf_caps_t *caps, int cap_ring,
+                       const long long *d_step, rf_stream_t *s);
 
 /* ----------------------------------------------------------- launches
  * `prec` selects the field precision: 0 = double, 1 = float.  All
@@ -361,21 +381,6 @@ int rf_launch_iq_power(const double *td, long long n, double *power, rf_stream_t
 int rf_launch_iq_noise(const double *td, long long n, int noise, float std_f, uint64_t state0, uint64_t inc,
                        float *out, rf_stream_t *s);
 
-/* Plane-monitor captures in one launch: capture q copies the plane
- * `index` of `axis` of field array f (cells (ia, ib), the other two axes
- * in increasing order, ia < na, ib < nb) into slot step % ring_len of
- * its ring buf (the field's precision); start is set by the launcher. */
-#define RF_MAX_CAPS 8
-typedef struct {
-    const void *f;
-    void *buf;
-    int axis, index, na, nb;
-    long long start;
-} rf_cap_t;
-typedef struct {
-    int n;
-    rf_cap_t c[RF_MAX_CAPS];
-} rf_caps_t;
 int rf_launch_plane_captures(int prec, const rf_geom_t *g, const rf_caps_t *cs, int ring_len,
                              const long long *d_step, rf_stream_t *s);
 

@@ -784,14 +784,53 @@ static int fill_leap(fdtd_sim_t *s, rf_leap_t *out, const rf_mat_t **mout, int a
     return 0;
 }
 
-/* The fused step's launches (A -> B), without the buffer swap. */
-static int launch_fused(fdtd_sim_t *s) {
+/* The plane captures of the step as extra blocks of the ADE pass: only
+ * when nothing after the pass changes a field (no CPML shell, no tensor
+ * edges, no post-E plane source outside the fused kernel), the monitors
+ * fit one launch and no E plane they read crosses the pass box [lo, hi).
+ * The captures read the new state in f2. */
+static int caps_in_ade_pass(const fdtd_sim_t *s, const int lo[3], const int hi[3], rf_caps_t *cs) {
+    if (!s->n_mons || 2 * s->n_mons > RF_MAX_CAPS || s->shell.n || (s->has_mat && s->mat.n_tensor)) return 0;
+    for (size_t p = 0; p < s->n_planes; p++)
+        if (s->planes[p].after_e && !s->post_in_kernel) return 0;
+    const size_t n[3] = {s->nx, s->ny, s->nz};
+    memset(cs, 0, sizeof *cs);
+    for (size_t q = 0; q < s->n_mons; q++) {
+        const smon_t *m = &s->mons[q];
+        long long index = m->index;
+        if ((s->per >> m->axis) & 1) index %= (long long)n[m->axis];
+        for (int w = 0; w < 2; w++) {
+            if (m->comp[w] < 3 && index >= lo[m->axis] && index < hi[m->axis]) return 0;
+            rf_cap_t *c = &cs->c[cs->n++];
+            c->f = s->f2[m->comp[w]];
+            c->buf = m->ring[w];
+            c->axis = m->axis;
+            c->index = m->index;
+            c->na = m->na;
+            c->nb = m->nb;
+        }
+    }
+    return 1;
+}
+
+/* The fused step's launches (A -> B), without the buffer swap.  With
+ * caps_done given, the step's plane captures join the ADE pass where
+ * caps_in_ade_pass allows it, and *caps_done says whether they did. */
+static int launch_fused(fdtd_sim_t *s, int *caps_done) {
     rf_leap_t a;
     const rf_mat_t *m;
     int alo[3], ahi[3];
+    rf_caps_t cs;
+    if (caps_done) *caps_done = 0;
     if (fill_leap(s, &a, &m, alo, ahi)) return -1;
     if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
-    if (a.ade_split && rf_launch_ade_pass(s->prec, &s->geom, alo, ahi, s->f, s->f2, m, s->st)) return -1;
+    if (a.ade_split) {
+        const int with_caps = caps_done && caps_in_ade_pass(s, alo, ahi, &cs);
+        if (rf_launch_ade_pass(s->prec, &s->geom, alo, ahi, s->f, s->f2, m, with_caps ? &cs : NULL, s->K,
+                               s->d_step, s->st))
+            return -1;
+        if (caps_done) *caps_done = with_caps;
+    }
     if (mark(s, 3) || mark(s, 4)) return -1;
     if (s->shell.n) {
         const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
@@ -805,8 +844,8 @@ static int launch_fused(fdtd_sim_t *s) {
     return mark(s, 5);
 }
 
-static int enqueue_fused(fdtd_sim_t *s) {
-    if (launch_fused(s)) return -1;
+static int enqueue_fused(fdtd_sim_t *s, int *caps_done) {
+    if (launch_fused(s, caps_done)) return -1;
     /* B now holds state n+1 */
     for (int q = 0; q < 6; q++) {
         void *t = s->f[q];
@@ -826,8 +865,9 @@ static int enqueue_step(fdtd_sim_t *s) {
     const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
     const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
     double cb = s->dt / (MU0 * s->dx);
+    int caps_done = 0;
     if (s->fused) {
-        if (enqueue_fused(s)) return -1;
+        if (enqueue_fused(s, &caps_done)) return -1;
         if (s->profiling) s->ev_used += 6;
         if (s->has_mat && s->mat.n_tensor &&
             rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, s->st))
@@ -873,7 +913,7 @@ static int enqueue_step(fdtd_sim_t *s) {
     /* the monitors' planes, up to RF_MAX_CAPS per launch */
     rf_caps_t cs;
     memset(&cs, 0, sizeof cs);
-    for (size_t q = 0; q < s->n_mons; q++) {
+    for (size_t q = 0; q < s->n_mons && !caps_done; q++) {
         smon_t *m = &s->mons[q];
         for (int w = 0; w < 2; w++) {
             rf_cap_t *c = &cs.c[cs.n++];
@@ -1075,7 +1115,7 @@ static int time_path(fdtd_sim_t *s, int fused, int launches, rf_event_t *e0, rf_
     for (int w = 0; w <= launches; w++) {
         if (w == 1 && rf_event_record(e0, s->st)) return -1;
         if (fused) {
-            if (launch_fused(s)) return -1;
+            if (launch_fused(s, NULL)) return -1;
             s->parity ^= 1;
             s->zpar ^= 1;
         } else if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st) ||

@@ -70,6 +70,30 @@ __device__ __forceinline__ unsigned load_tag(const void *p, long long idx) {
     return static_cast<unsigned>(static_cast<const TagT *>(p)[idx]);
 }
 
+// Element t of the plane captures cs (capture q takes the elements
+// [start, start + na nb), cell (ia, ib) with ib fastest): its field value
+// into slot step % ring_len of the capture's ring.
+template <typename T>
+__device__ __forceinline__ void plane_capture_elem(const rf_geom_t &gg, const rf_caps_t &cs, long long t,
+                                                   int ring_len, long long step) {
+    int q = 0;
+    while (q + 1 < cs.n && t >= cs.c[q + 1].start) q++;
+    if (q >= cs.n) return;
+    const rf_cap_t &c = cs.c[q];
+    const long long local = t - c.start;
+    if (local < 0 || local >= (long long)c.na * c.nb) return;
+    const Geo g = geo_of(gg);
+    int index = c.index;
+    if ((gg.per >> c.axis) & 1) index %= (c.axis == 0 ? g.nx : c.axis == 1 ? g.ny : g.nz);
+    const int ia = static_cast<int>(local / c.nb), ib = static_cast<int>(local % c.nb);
+    // the other two axes in increasing order: (y, z), (x, z) or (x, y)
+    const int i = c.axis == 0 ? index : ia;
+    const int j = c.axis == 0 ? ia : (c.axis == 1 ? index : ib);
+    const int k = c.axis == 2 ? index : ib;
+    const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
+    static_cast<T *>(c.buf)[(step % ring_len) * (long long)c.na * c.nb + local] = static_cast<const T *>(c.f)[idx];
+}
+
 // Element of the state of pole 0, component c, node (i,j,k) in the box of
 // ADE set `a`; pole s is s * 3 * a.st further on.
 template <typename Ade>

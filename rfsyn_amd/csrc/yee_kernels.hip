@@ -367,25 +367,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_plane_captures(rf_geom_t gg, rf_caps_t cs, int ring_len, const long long *d_step) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int q = 0;
-    while (q + 1 < cs.n && t >= cs.c[q + 1].start) q++;
-    const rf_cap_t &c = cs.c[q];
-    const long long local = t - c.start;
-    if (q >= cs.n || local < 0 || local >= (long long)c.na * c.nb) return;
-    const Geo g = geo_of(gg);
-    const int axis = c.axis;
-    const int a = (axis + 1) % 3 < (axis + 2) % 3 ? (axis + 1) % 3 : (axis + 2) % 3;
-    const int b = 3 - axis - a;
-    Pos x;
-    x.x[axis] = c.index;
-    if ((gg.per >> axis) & 1) {
-        const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
-        x.x[axis] %= n;
-    }
-    x.x[a] = static_cast<int>(local / c.nb);
-    x.x[b] = static_cast<int>(local % c.nb);
-    const long long slot = *d_step % ring_len;
-    static_cast<T *>(c.buf)[slot * (long long)c.na * c.nb + local] = static_cast<const T *>(c.f)[lin(g, x)];
+    plane_capture_elem<T>(gg, cs, t, ring_len, *d_step);
 }
 
 // Twiddles for the buffered block, scaled by dt:

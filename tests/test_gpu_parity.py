@@ -308,7 +308,10 @@ def test_ade_pass_box_and_shell_fp64(gpu, oracle, split, src_in_ade, monkeypatch
 def test_ade_split_matches_inline_fp32(gpu, monkeypatch):
     """FP32 periodic dispersive stack: the split ADE step and the in-kernel
     update perform the same operations in the same order, so the fields
-    agree bit for bit."""
+    agree bit for bit.  The split step's ADE pass also takes the plane
+    captures (both monitor planes lie outside the dispersive layers); the
+    in-kernel step captures in a launch of its own: the plane DFTs agree
+    bit for bit too."""
     a = _run_periodic_stack(gpu, capi.FDTD_FP32, 120)
     monkeypatch.setenv("RFSYN_ADE_SPLIT", "0")
     b = _run_periodic_stack(gpu, capi.FDTD_FP32, 120)
@@ -316,6 +319,8 @@ def test_ade_split_matches_inline_fp32(gpu, monkeypatch):
     assert same_values(a["td"], b["td"])
     for k in a["fields"]:
         assert same_values(a["fields"][k], b["fields"][k]), k
+    assert np.max(np.abs(a["T"])) > 0
+    assert same_values(a["R"], b["R"]) and same_values(a["T"], b["T"])
 
 
 def _monitor_runs(lib, precision):
