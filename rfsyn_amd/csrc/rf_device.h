@@ -384,6 +384,41 @@ int rf_launch_iq_noise(const double *td, long long n, int noise, float std_f, ui
 int rf_launch_plane_captures(int prec, const rf_geom_t *g, const rf_caps_t *cs, int ring_len,
                              const long long *d_step, rf_stream_t *s);
 
+/* Scene painter (scene_kernels.hip, include/fdtd_scene.h).  Shapes as the
+ * device reads them: kind 0 sphere, 1 slab, 2 box; nrm the unit slab
+ * normal; bb_lo/bb_hi the bounding box.  Records: kind 0 nothing, 1 scalar
+ * override eps v[0], 2 scalar mixed eps v[0], 3 tensor v[0..5] (xx xy xz
+ * yy yz zz). */
+typedef struct {
+    int kind;
+    double eps_r;
+    double p[3], radius, nrm[3], half_thickness, lo[3], hi[3];
+    double bb_lo[3], bb_hi[3];
+} rf_paint_shape_t;
+typedef struct {
+    int kind;
+    double v[6];
+} rf_paint_rec_t;
+/* ids[(i ext1 + j) ext2 + k] = last shape containing node (i,j,k), 255
+ * none; *painted += number of (shape, node) containments. */
+int rf_launch_paint_nodes(const rf_paint_shape_t *sh, int ns, const int ext[3], uint8_t *ids,
+                          unsigned long long *painted, rf_stream_t *s);
+/* flags[(i n1 + j) n2 + k] bit c: the E_c edge exists and its dual cell may
+ * contain a shape boundary. */
+int rf_launch_paint_flags(const rf_paint_shape_t *sh, int ns, const int n[3], unsigned per, uint8_t *flags,
+                          rf_stream_t *s);
+/* count[i n1 + j]: flagged edges of row (i, j). */
+int rf_launch_paint_rows(const uint8_t *flags, const int n[3], int *count, rf_stream_t *s);
+/* codes[off[row] ...]: ((i n1 + j) n2 + k) 3 + c of the row's flagged
+ * edges, k then c ascending. */
+int rf_launch_paint_list(const uint8_t *flags, const int n[3], const long long *off, long long *codes,
+                         rf_stream_t *s);
+/* recs[e]: the painted coefficient of edge codes[e] (ss^3 sub-samples,
+ * scalar band `keep` cells at non-periodic faces). */
+int rf_launch_paint_classify(const rf_paint_shape_t *sh, int ns, const long long *codes, long long n_edges,
+                             const int n[3], unsigned per, int ss, long long keep, rf_paint_rec_t *recs,
+                             rf_stream_t *s);
+
 /* Plane monitor DFT flush over `count` (<= 64) buffered steps ending at
  * step *d_step - 1; accumulators [f][cell]; tw_scratch holds
  * 2 * nf * count doubles. */

@@ -1,88 +1,36 @@
 /*
- * Scene painter with subpixel smoothing (include/fdtd_scene.h).
+ * Scene painter with subpixel smoothing (include/fdtd_scene.h), painted on
+ * the device.
  *
- * Written against the public materials API only, so it paints any library
- * exporting that API (the CPU oracle links the same file to build identical
- * scenes for parity tests).  Host-side set-up, O(shape volume) for node
- * painting and O(interface edges x samples) for smoothing.
+ * The device computes, for the whole grid at once (scene_kernels.hip):
+ *   1. the node ids: the last shape containing each node;
+ *   2. the E edges whose dual cell may contain a shape boundary, listed in
+ *      (i, j, k, component) order;
+ *   3. each listed edge's coefficient from ss^3 sub-samples of its dual
+ *      cell: a scalar override, the quantised scalar of the face band, or
+ *      the smoothed 3x3 inverse-permittivity tensor.
+ * The host then applies the node ids as k-runs of materials_assign_box and
+ * the edge records, in list order, through materials_set_component and
+ * materials_set_component_tensor3: the materials end in the same state as
+ * painting shape by shape and edge by edge on the host.  The public
+ * materials API is the only way in, so any library exporting it can be
+ * painted.
  */
 #include "fdtd_scene.h"
+#include "host_internal.h"
+#include "rf_device.h"
 
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
-typedef struct {
-    fdtd_shape_t s;
-    double nrm[3];
-    double bb_lo[3], bb_hi[3];     /* bounding box in cell units */
-    material_id_t id;
-} shape_t;
-
-static int inside(const shape_t *sh, const double x[3]) {
-    const fdtd_shape_t *s = &sh->s;
-    if (s->kind == FDTD_SHAPE_SPHERE) {
-        double d0 = x[0] - s->p[0], d1 = x[1] - s->p[1], d2 = x[2] - s->p[2];
-        return d0 * d0 + d1 * d1 + d2 * d2 <= s->radius * s->radius;
-    }
-    for (int a = 0; a < 3; a++)
-        if (x[a] < s->lo[a] || x[a] > s->hi[a]) return 0;
-    if (s->kind == FDTD_SHAPE_BOX) return 1;
-    double d = sh->nrm[0] * (x[0] - s->p[0]) + sh->nrm[1] * (x[1] - s->p[1]) + sh->nrm[2] * (x[2] - s->p[2]);
-    return fabs(d) <= s->half_thickness;
-}
-
-/* 1 when the unit cube centred at x may contain a boundary of the shape. */
-static int near_boundary(const shape_t *sh, const double x[3]) {
-    const fdtd_shape_t *s = &sh->s;
-    for (int a = 0; a < 3; a++)
-        if (x[a] < sh->bb_lo[a] - 1.0 || x[a] > sh->bb_hi[a] + 1.0) return 0;
-    if (s->kind == FDTD_SHAPE_SPHERE) {
-        double d0 = x[0] - s->p[0], d1 = x[1] - s->p[1], d2 = x[2] - s->p[2];
-        return fabs(sqrt(d0 * d0 + d1 * d1 + d2 * d2) - s->radius) <= 1.0;
-    }
-    for (int a = 0; a < 3; a++)
-        if (fabs(x[a] - s->lo[a]) <= 1.0 || fabs(x[a] - s->hi[a]) <= 1.0) return 1;
-    if (s->kind == FDTD_SHAPE_BOX) return 0;
-    double d = sh->nrm[0] * (x[0] - s->p[0]) + sh->nrm[1] * (x[1] - s->p[1]) + sh->nrm[2] * (x[2] - s->p[2]);
-    return fabs(fabs(d) - s->half_thickness) <= 1.0;
-}
-
-static void normal_at(const shape_t *sh, const double x[3], double n[3]) {
-    const fdtd_shape_t *s = &sh->s;
-    if (s->kind == FDTD_SHAPE_SPHERE) {
-        double d[3] = {x[0] - s->p[0], x[1] - s->p[1], x[2] - s->p[2]};
-        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        if (r == 0.0) { n[0] = 0; n[1] = 0; n[2] = 1; return; }
-        for (int a = 0; a < 3; a++) n[a] = d[a] / r;
-        return;
-    }
-    if (s->kind == FDTD_SHAPE_SLAB) {
-        /* the slab faces, unless a clip face is nearer */
-        double d = sh->nrm[0] * (x[0] - s->p[0]) + sh->nrm[1] * (x[1] - s->p[1]) + sh->nrm[2] * (x[2] - s->p[2]);
-        double best = fabs(fabs(d) - s->half_thickness);
-        memcpy(n, sh->nrm, sizeof sh->nrm);
-        for (int a = 0; a < 3; a++) {
-            double dl = fabs(x[a] - s->lo[a]), dh = fabs(x[a] - s->hi[a]);
-            double m = dl < dh ? dl : dh;
-            if (m < best) { best = m; n[0] = n[1] = n[2] = 0.0; n[a] = 1.0; }
-        }
-        return;
-    }
-    double best = INFINITY;
-    for (int a = 0; a < 3; a++) {
-        double dl = fabs(x[a] - s->lo[a]), dh = fabs(x[a] - s->hi[a]);
-        double m = dl < dh ? dl : dh;
-        if (m < best) { best = m; n[0] = n[1] = n[2] = 0.0; n[a] = 1.0; }
-    }
-}
-
-/* Index of the topmost shape containing x among cand[0..nc), or -1. */
-static int material_at(const shape_t *sh, const int *cand, int nc, const double x[3]) {
-    for (int q = nc - 1; q >= 0; q--)
-        if (inside(&sh[cand[q]], x)) return cand[q];
-    return -1;
+static int shape_ok(const fdtd_shape_t *s) {
+    const double v[] = {s->eps_r, s->p[0], s->p[1], s->p[2], s->radius, s->normal[0], s->normal[1], s->normal[2],
+                        s->half_thickness, s->lo[0], s->lo[1], s->lo[2], s->hi[0], s->hi[1], s->hi[2]};
+    for (size_t q = 0; q < sizeof v / sizeof v[0]; q++)
+        if (!isfinite(v[q])) return 0;
+    return s->eps_r >= 1.0;
 }
 
 int fdtd_scene_paint(materials_t *mat, const fdtd_grid_t *g, const fdtd_shape_t *shapes, size_t n_shapes,
@@ -93,187 +41,178 @@ int fdtd_scene_paint(materials_t *mat, const fdtd_grid_t *g, const fdtd_shape_t 
     size_t keep = opt ? opt->keepout : 0;
     if (keep < 2) keep = 2;     /* the tensor stencil needs one cell of margin */
     fdtd_scene_stats_t st = {0, 0, 0, 0};
-    size_t n[3] = {g->nx, g->ny, g->nz};
-    unsigned per = g->periodic_axes;
+    const size_t n[3] = {g->nx, g->ny, g->nz};
+    const unsigned per = g->periodic_axes;
+    if (n[0] > 65535 || n[1] > 65535 || n[2] > 65535 || keep > (1u << 30)) return -1;
 
-    shape_t *sh = calloc(n_shapes ? n_shapes : 1, sizeof *sh);
-    if (!sh) return -1;
+    rf_paint_shape_t *sh = calloc(n_shapes ? n_shapes : 1, sizeof *sh);
+    material_id_t *mid = calloc(n_shapes ? n_shapes : 1, sizeof *mid);
+    if (!sh || !mid) {
+        free(sh);
+        free(mid);
+        return -1;
+    }
     int rc = -1;
-    uint8_t *flag = NULL;
-    int *node_mat = NULL;
+    rf_stream_t *s = NULL;
+    rf_paint_shape_t *d_sh = NULL;
+    uint8_t *d_ids = NULL, *d_flags = NULL, *ids = NULL;
+    unsigned long long *d_painted = NULL;
+    int *d_count = NULL, *count = NULL;
+    long long *d_off = NULL, *off = NULL, *d_codes = NULL, *codes = NULL;
+    rf_paint_rec_t *d_recs = NULL, *recs = NULL;
+
     for (size_t q = 0; q < n_shapes; q++) {
-        sh[q].s = shapes[q];
-        const fdtd_shape_t *s = &shapes[q];
-        if (!(isfinite(s->eps_r) && s->eps_r >= 1.0)) goto out;
-        if (materials_register(mat, s->eps_r, 0.0, &sh[q].id) != 0) goto out;
-        if (s->kind == FDTD_SHAPE_SPHERE) {
-            if (!(s->radius > 0.0)) goto out;
+        const fdtd_shape_t *in = &shapes[q];
+        rf_paint_shape_t *o = &sh[q];
+        if (!shape_ok(in)) goto out;
+        if (materials_register(mat, in->eps_r, 0.0, &mid[q]) != 0) goto out;
+        o->kind = in->kind;
+        o->eps_r = in->eps_r;
+        o->radius = in->radius;
+        o->half_thickness = in->half_thickness;
+        for (int a = 0; a < 3; a++) {
+            o->p[a] = in->p[a];
+            o->lo[a] = in->lo[a];
+            o->hi[a] = in->hi[a];
+        }
+        if (in->kind == FDTD_SHAPE_SPHERE) {
+            if (!(in->radius > 0.0)) goto out;
             for (int a = 0; a < 3; a++) {
-                sh[q].bb_lo[a] = s->p[a] - s->radius;
-                sh[q].bb_hi[a] = s->p[a] + s->radius;
+                o->bb_lo[a] = in->p[a] - in->radius;
+                o->bb_hi[a] = in->p[a] + in->radius;
             }
         } else {
             for (int a = 0; a < 3; a++) {
-                if (!(s->hi[a] >= s->lo[a])) goto out;
-                sh[q].bb_lo[a] = s->lo[a];
-                sh[q].bb_hi[a] = s->hi[a];
+                if (!(in->hi[a] >= in->lo[a])) goto out;
+                o->bb_lo[a] = in->lo[a];
+                o->bb_hi[a] = in->hi[a];
             }
-            if (s->kind == FDTD_SHAPE_SLAB) {
-                double l = sqrt(s->normal[0] * s->normal[0] + s->normal[1] * s->normal[1] + s->normal[2] * s->normal[2]);
-                if (!(l > 0.0) || !(s->half_thickness > 0.0)) goto out;
-                for (int a = 0; a < 3; a++) sh[q].nrm[a] = s->normal[a] / l;
-            } else if (s->kind != FDTD_SHAPE_BOX) {
+            if (in->kind == FDTD_SHAPE_SLAB) {
+                double l = sqrt(in->normal[0] * in->normal[0] + in->normal[1] * in->normal[1] +
+                                in->normal[2] * in->normal[2]);
+                if (!(l > 0.0) || !(in->half_thickness > 0.0)) goto out;
+                for (int a = 0; a < 3; a++) o->nrm[a] = in->normal[a] / l;
+            } else if (in->kind != FDTD_SHAPE_BOX) {
                 goto out;
             }
         }
     }
+    if (!n_shapes) {
+        rc = 0;
+        goto out;
+    }
+    if (rf_init() != 0) {
+        rf_set_error("no GPU: %s", rf_error_string());
+        goto out;
+    }
+    s = rf_stream_create();
+    if (!s) goto out;
 
     /* node extents: the padded node exists on non-periodic axes */
-    size_t ext[3];
-    for (int a = 0; a < 3; a++) ext[a] = ((per >> a) & 1u) ? n[a] : n[a] + 1;
-    size_t stride_j = g->nz + 1, stride_i = (g->ny + 1) * (g->nz + 1);
-    size_t total = (g->nx + 1) * stride_i;
-    node_mat = malloc(total * sizeof *node_mat);
-    flag = calloc(total, 1);
-    if (!node_mat || !flag) goto out;
-    for (size_t q = 0; q < total; q++) node_mat[q] = -1;
-
-    /* 1. node ids: material at the node position, later shapes win */
-    for (size_t q = 0; q < n_shapes; q++) {
-        size_t lo[3], hi[3];
-        for (int a = 0; a < 3; a++) {
-            double l = floor(sh[q].bb_lo[a]), h = ceil(sh[q].bb_hi[a]);
-            lo[a] = l < 0 ? 0 : (size_t)l;
-            hi[a] = h < 0 ? 0 : (size_t)h + 1;
-            if (hi[a] > ext[a]) hi[a] = ext[a];
-        }
-        for (size_t i = lo[0]; i < hi[0]; i++)
-            for (size_t j = lo[1]; j < hi[1]; j++) {
-                size_t k = lo[2];
-                while (k < hi[2]) {
-                    double x[3] = {(double)i, (double)j, (double)k};
-                    if (!inside(&sh[q], x)) { k++; continue; }
-                    size_t k0 = k;
-                    while (k < hi[2]) {
-                        double y[3] = {(double)i, (double)j, (double)k};
-                        if (!inside(&sh[q], y)) break;
-                        node_mat[i * stride_i + j * stride_j + k] = (int)q;
-                        k++;
-                    }
-                    if (materials_assign_box(mat, g, i, i + 1, j, j + 1, k0, k, sh[q].id) != 0) goto out;
-                    st.nodes_painted += k - k0;
-                }
-            }
+    int ext[3], ni[3];
+    for (int a = 0; a < 3; a++) {
+        ext[a] = ((per >> a) & 1u) ? (int)n[a] : (int)n[a] + 1;
+        ni[a] = (int)n[a];
     }
+    const size_t nodes = (size_t)ext[0] * ext[1] * ext[2];
+    const size_t cells = n[0] * n[1] * n[2];
+    const size_t rows = n[0] * n[1];
 
-    /* 2. edges whose dual cell may cross a boundary (bit c per node) */
-    for (size_t q = 0; q < n_shapes; q++) {
-        size_t lo[3], hi[3];
-        for (int a = 0; a < 3; a++) {
-            double l = floor(sh[q].bb_lo[a]) - 2, h = ceil(sh[q].bb_hi[a]) + 2;
-            lo[a] = l < 0 ? 0 : (size_t)l;
-            hi[a] = h < 0 ? 0 : (size_t)h + 1;
-            if (hi[a] > n[a]) hi[a] = n[a];
-        }
-        for (size_t i = lo[0]; i < hi[0]; i++)
-            for (size_t j = lo[1]; j < hi[1]; j++)
-                for (size_t k = lo[2]; k < hi[2]; k++)
-                    for (int c = 0; c < 3; c++) {
-                        double x[3] = {(double)i, (double)j, (double)k};
-                        x[c] += 0.5;
-                        if (near_boundary(&sh[q], x)) flag[i * stride_i + j * stride_j + k] |= (uint8_t)(1u << c);
-                    }
-    }
-
-    /* 3. classify the flagged edges by sub-sampling their dual cells */
-    int cand[256];
-    int counts[257];
-    for (size_t i = 0; i < n[0]; i++)
-        for (size_t j = 0; j < n[1]; j++)
-            for (size_t k = 0; k < n[2]; k++) {
-                size_t idx = i * stride_i + j * stride_j + k;
-                if (!flag[idx]) continue;
-                for (int c = 0; c < 3; c++) {
-                    if (!((flag[idx] >> c) & 1u)) continue;
-                    /* the E_c edge exists only inside its component range */
-                    size_t x_int[3] = {i, j, k};
-                    int valid = 1;
-                    for (int a = 0; a < 3; a++)
-                        if (a != c && !((per >> a) & 1u) && (x_int[a] == 0 || x_int[a] >= n[a])) valid = 0;
-                    if (!valid) continue;
-                    double xe[3] = {(double)i, (double)j, (double)k};
-                    xe[c] += 0.5;
-                    int nc = 0;
-                    for (size_t q = 0; q < n_shapes; q++) {
-                        int hit = 1;
-                        for (int a = 0; a < 3; a++)
-                            if (xe[a] < sh[q].bb_lo[a] - 1.0 || xe[a] > sh[q].bb_hi[a] + 1.0) hit = 0;
-                        if (hit) cand[nc++] = (int)q;
-                    }
-                    memset(counts, 0, sizeof counts);
-                    for (int a0 = 0; a0 < ss; a0++)
-                        for (int a1 = 0; a1 < ss; a1++)
-                            for (int a2 = 0; a2 < ss; a2++) {
-                                double y[3] = {xe[0] + ((a0 + 0.5) / ss - 0.5), xe[1] + ((a1 + 0.5) / ss - 0.5),
-                                               xe[2] + ((a2 + 0.5) / ss - 0.5)};
-                                counts[material_at(sh, cand, nc, y) + 1]++;
-                            }
-                    /* the two most frequent materials (index -1 = background) */
-                    int m1 = -2, m2 = -2;
-                    for (int m = -1; m < (int)n_shapes; m++) {
-                        if (!counts[m + 1]) continue;
-                        if (m1 == -2 || counts[m + 1] > counts[m1 + 1]) { m2 = m1; m1 = m; }
-                        else if (m2 == -2 || counts[m + 1] > counts[m2 + 1]) m2 = m;
-                    }
-                    int nm = node_mat[idx];
-                    field_component_t comp = (field_component_t)c;
-                    if (m2 == -2) {
-                        if (m1 != nm) {
-                            double eps = m1 < 0 ? 1.0 : sh[m1].s.eps_r;
-                            if (materials_set_component(mat, g, comp, i, j, k, eps, 0.0) != 0) goto out;
-                            st.override_edges++;
-                        }
-                        continue;
-                    }
-                    int in = m1 > m2 ? m1 : m2, outm = m1 > m2 ? m2 : m1;
-                    double e_in = sh[in].s.eps_r, e_out = outm < 0 ? 1.0 : sh[outm].s.eps_r;
-                    double f = (double)counts[in + 1] / (double)(counts[in + 1] + counts[outm + 1]);
-                    double nv[3];
-                    normal_at(&sh[in], xe, nv);
-                    double ln = (1.0 - f) / e_out + f / e_in;
-                    double lt = 1.0 / ((1.0 - f) * e_out + f * e_in);
-                    double M[3][3];
-                    for (int a = 0; a < 3; a++)
-                        for (int b = 0; b < 3; b++)
-                            M[a][b] = (ln - lt) * nv[a] * nv[b] + (a == b ? lt : 0.0);
-                    int near_face = 0;
-                    for (int a = 0; a < 3; a++)
-                        if (!((per >> a) & 1u) && (xe[a] < (double)keep || xe[a] > (double)n[a] - (double)keep))
-                            near_face = 1;
-                    if (near_face) {
-                        /* scalar 1/M_cc from the fill fraction and normal
-                         * quantised to 1/64 steps, so the band holds a small
-                         * set of distinct coefficients (compact device tags) */
-                        double fq = floor(f * 64.0 + 0.5) / 64.0;
-                        double nc = floor(nv[c] * nv[c] * 64.0 + 0.5) / 64.0;
-                        double lnq = (1.0 - fq) / e_out + fq / e_in;
-                        double ltq = 1.0 / ((1.0 - fq) * e_out + fq * e_in);
-                        double mcc = (lnq - ltq) * nc + ltq;
-                        double eps_c = 1.0 / mcc < 1.0 ? 1.0 : 1.0 / mcc;
-                        if (materials_set_component(mat, g, comp, i, j, k, eps_c, 0.0) != 0) goto out;
-                        st.scalar_mixed_edges++;
-                        continue;
-                    }
-                    double m6[6] = {M[0][0], M[0][1], M[0][2], M[1][1], M[1][2], M[2][2]};
-                    if (materials_set_component_tensor3(mat, g, comp, i, j, k, m6) != 0) goto out;
-                    st.tensor_edges++;
+    /* 1. node ids, applied as k-runs of one material */
+    unsigned long long painted = 0;
+    if (rf_malloc((void **)&d_sh, n_shapes * sizeof *d_sh) || rf_malloc((void **)&d_ids, nodes) ||
+        rf_malloc((void **)&d_painted, sizeof *d_painted) || !(ids = malloc(nodes)))
+        goto out;
+    if (rf_h2d(d_sh, sh, n_shapes * sizeof *sh, s) ||
+        rf_launch_paint_nodes(d_sh, (int)n_shapes, ext, d_ids, d_painted, s) || rf_d2h(ids, d_ids, nodes, s) ||
+        rf_d2h(&painted, d_painted, sizeof painted, s) || rf_sync(s))
+        goto out;
+    rf_free(d_ids);
+    d_ids = NULL;
+    st.nodes_painted = (size_t)painted;
+    for (size_t i = 0; i < (size_t)ext[0]; i++)
+        for (size_t j = 0; j < (size_t)ext[1]; j++) {
+            const uint8_t *row = ids + (i * ext[1] + j) * ext[2];
+            size_t k = 0;
+            while (k < (size_t)ext[2]) {
+                if (row[k] == 255) {
+                    k++;
+                    continue;
                 }
+                const material_id_t m = mid[row[k]];
+                const size_t k0 = k;
+                while (k < (size_t)ext[2] && row[k] != 255 && mid[row[k]] == m) k++;
+                if (materials_assign_box(mat, g, i, i + 1, j, j + 1, k0, k, m) != 0) goto out;
             }
+        }
+    free(ids);
+    ids = NULL;
+
+    /* 2. the edges that may straddle an interface, listed in order */
+    if (rf_malloc((void **)&d_flags, cells) || rf_malloc((void **)&d_count, rows * sizeof *d_count) ||
+        !(count = malloc(rows * sizeof *count)) || !(off = malloc(rows * sizeof *off)))
+        goto out;
+    if (rf_launch_paint_flags(d_sh, (int)n_shapes, ni, per, d_flags, s) ||
+        rf_launch_paint_rows(d_flags, ni, d_count, s) || rf_d2h(count, d_count, rows * sizeof *count, s) ||
+        rf_sync(s))
+        goto out;
+    long long n_edges = 0;
+    for (size_t r = 0; r < rows; r++) {
+        off[r] = n_edges;
+        n_edges += count[r];
+    }
+    if (n_edges > 0) {
+        /* 3. classify them and apply the records in list order */
+        if (rf_malloc((void **)&d_off, rows * sizeof *d_off) ||
+            rf_malloc((void **)&d_codes, (size_t)n_edges * sizeof *d_codes) ||
+            rf_malloc((void **)&d_recs, (size_t)n_edges * sizeof *d_recs) ||
+            !(codes = malloc((size_t)n_edges * sizeof *codes)) ||
+            !(recs = malloc((size_t)n_edges * sizeof *recs)))
+            goto out;
+        if (rf_h2d(d_off, off, rows * sizeof *off, s) || rf_launch_paint_list(d_flags, ni, d_off, d_codes, s) ||
+            rf_launch_paint_classify(d_sh, (int)n_shapes, d_codes, n_edges, ni, per, ss, (long long)keep, d_recs,
+                                     s) ||
+            rf_d2h(codes, d_codes, (size_t)n_edges * sizeof *codes, s) ||
+            rf_d2h(recs, d_recs, (size_t)n_edges * sizeof *recs, s) || rf_sync(s))
+            goto out;
+        for (long long e = 0; e < n_edges; e++) {
+            const rf_paint_rec_t *r = &recs[e];
+            if (r->kind == 0) continue;
+            const long long node = codes[e] / 3;
+            const field_component_t comp = (field_component_t)(codes[e] % 3);
+            const size_t k = (size_t)(node % ni[2]), j = (size_t)((node / ni[2]) % ni[1]),
+                         i = (size_t)(node / ((long long)ni[1] * ni[2]));
+            if (r->kind == 3) {
+                if (materials_set_component_tensor3(mat, g, comp, i, j, k, r->v) != 0) goto out;
+                st.tensor_edges++;
+            } else {
+                if (materials_set_component(mat, g, comp, i, j, k, r->v[0], 0.0) != 0) goto out;
+                if (r->kind == 1)
+                    st.override_edges++;
+                else
+                    st.scalar_mixed_edges++;
+            }
+        }
+    }
     rc = 0;
 out:
+    if (s) rf_sync(s);
+    rf_free(d_sh);
+    rf_free(d_ids);
+    rf_free(d_painted);
+    rf_free(d_flags);
+    rf_free(d_count);
+    rf_free(d_off);
+    rf_free(d_codes);
+    rf_free(d_recs);
+    if (s) rf_stream_destroy(s);
+    free(ids);
+    free(count);
+    free(off);
+    free(codes);
+    free(recs);
     free(sh);
-    free(flag);
-    free(node_mat);
+    free(mid);
     if (stats) *stats = st;
     return rc;
 }

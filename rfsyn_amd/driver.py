@@ -6,6 +6,7 @@ and drives it only through the C API of include/*.h, so the same scene can
 be run on each and compared.
 """
 import ctypes as C
+import time
 import math
 
 import numpy as np
@@ -381,10 +382,11 @@ def c2_paint(lib, g, mat, shapes, npml, stats_out=None):
     arr = (capi.Shape * len(shapes))(*shapes)
     opt = capi.SceneOptions(4, npml + 2)
     st = capi.SceneStats()
+    t0 = time.perf_counter()
     assert lib.fdtd_scene_paint(mat, g, arr, len(shapes), C.byref(opt), C.byref(st)) == 0
     if stats_out is not None:
         stats_out.update(nodes=st.nodes_painted, overrides=st.override_edges, tensors=st.tensor_edges,
-                         scalar_mixed=st.scalar_mixed_edges)
+                         scalar_mixed=st.scalar_mixed_edges, paint_s=round(time.perf_counter() - t0, 4))
 
 
 def build_c2(lib, precision, td_cap, n=256, npml=12, n_spheres=40, r_lo=4.0, r_hi=24.0, n_probes=16,
