@@ -155,6 +155,17 @@ int fdtd_simulate_echo_iq(const fdtd_sim_config_t *cfg, const fdtd_echo_options_
  * .iq file of the CLI (cli/main.c:70-85; little-endian on x86-64).  0 on
  * success, -1 on any failure. */
 int fdtd_write_iq(const char *path, const float *iq, size_t n);
+
+/* Dataset transfer-function file, the companion of the .iq files: the
+ * n_probes x n_freq complex values (re[p n_freq + f], im[...]) as raw
+ * float64 pairs, real then imaginary, probe-major, no header (the scene
+ * JSON beside it records the shape and the frequencies; little-endian on
+ * x86-64).  The reference has no such file (its CLI writes time-domain
+ * .iq only); the learned surrogate's labels are these values.  Read
+ * checks that the file holds exactly n_probes x n_freq values.  0 on
+ * success, -1 on any failure. */
+int fdtd_write_transfer(const char *path, const double *re, const double *im, size_t n_probes, size_t n_freq);
+int fdtd_read_transfer(const char *path, double *re, double *im, size_t n_probes, size_t n_freq);
 /* Accumulated DFT (acc_re, acc_im exactly as probe_dft_t holds them). */
 int fdtd_sim_probe_dft(fdtd_sim_t *s, int probe, double *re, double *im);
 /* Plane DFT of tangential component `which` (0: lower axis, 1: upper);

@@ -214,6 +214,8 @@ EXTENSION_API = {
     "fdtd_sim_probe_iq": (C.c_int, [_vp, C.c_int, _sz, C.c_int, C.c_float, C.c_uint64, C.c_uint64,
                                     C.POINTER(C.c_float)]),
     "fdtd_write_iq": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), _sz]),
+    "fdtd_write_transfer": (C.c_int, [C.c_char_p, _dp, _dp, _sz, _sz]),
+    "fdtd_read_transfer": (C.c_int, [C.c_char_p, _dp, _dp, _sz, _sz]),
     "fdtd_simulate_echo_iq": (C.c_int, [_vp, _vp, C.POINTER(C.c_float), _dp]),
     "oracle_awgn_iq": (None, [C.POINTER(C.c_float), _sz, C.c_float, C.c_uint64, C.c_uint64]),
     "fdtd_sim_probe_dft": (C.c_int, [_vp, C.c_int, _dp, _dp]),

@@ -20,6 +20,7 @@
 #include "constants.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <limits.h>
 #include <stdlib.h>
@@ -1319,6 +1320,51 @@ int fdtd_write_iq(const char *path, const float *iq, size_t n) {
     const size_t written = n ? fwrite(iq, 2 * sizeof(float), n, f) : 0;
     const int closed = fclose(f);
     return (written == n && closed == 0) ? 0 : -1;
+}
+
+int fdtd_write_transfer(const char *path, const double *re, const double *im, size_t n_probes, size_t n_freq) {
+    if (!path || (n_freq && n_probes > SIZE_MAX / 16 / n_freq)) return -1;
+    const size_t n = n_probes * n_freq;
+    if (n && (!re || !im)) return -1;
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    double buf[2 * 512];
+    size_t done = 0;
+    int ok = 1;
+    while (ok && done < n) {
+        const size_t m = n - done < 512 ? n - done : 512;
+        for (size_t q = 0; q < m; q++) {
+            buf[2 * q] = re[done + q];
+            buf[2 * q + 1] = im[done + q];
+        }
+        ok = fwrite(buf, 2 * sizeof(double), m, f) == m;
+        done += m;
+    }
+    return (fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+int fdtd_read_transfer(const char *path, double *re, double *im, size_t n_probes, size_t n_freq) {
+    if (!path || (n_freq && n_probes > SIZE_MAX / 16 / n_freq)) return -1;
+    const size_t n = n_probes * n_freq;
+    if (n && (!re || !im)) return -1;
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    double buf[2 * 512];
+    size_t done = 0;
+    int ok = 1;
+    while (ok && done < n) {
+        const size_t m = n - done < 512 ? n - done : 512;
+        ok = fread(buf, 2 * sizeof(double), m, f) == m;
+        for (size_t q = 0; ok && q < m; q++) {
+            re[done + q] = buf[2 * q];
+            im[done + q] = buf[2 * q + 1];
+        }
+        done += m;
+    }
+    /* the file holds exactly n values */
+    if (ok) ok = fgetc(f) == EOF;
+    fclose(f);
+    return ok ? 0 : -1;
 }
 
 int fdtd_sim_probe_dft(fdtd_sim_t *s, int probe, double *re, double *im) {

@@ -16,7 +16,9 @@ scenes with the FDTD, and falls back to the FDTD for the batch when a
 spot check's relative L2 error exceeds the threshold.
 """
 import ctypes as C
+import json
 import math
+import os
 import time
 
 import numpy as np
@@ -143,9 +145,11 @@ def source_spectrum(lib, dt, steps):
     return (w[None, :] * np.exp(-1j * om[:, None] * (t[None, :] + dt))).sum(axis=1) * dt
 
 
-def label_scenes(lib, indices, steps, batch=128, n=64, npml=8, seed=21):
+def label_scenes(lib, indices, steps, batch=128, n=64, npml=8, seed=21, out_dir=None):
     """(maps [N, n, n, 3], coords [N, 8, 6], labels [N, 8, 128], seconds of
-    FDTD) for the scenes, stepped together in batches (fdtd_sim_run_many)."""
+    FDTD) for the scenes, stepped together in batches (fdtd_sim_run_many).
+    With out_dir, each scene's transfer functions and its JSON description
+    are written there too (write_scene)."""
     maps, coords, labels = [], [], []
     spec = None
     elapsed = 0.0
@@ -155,15 +159,84 @@ def label_scenes(lib, indices, steps, batch=128, n=64, npml=8, seed=21):
         t0 = time.perf_counter()
         run_many(lib, [r for r, _ in runs], steps)
         elapsed += time.perf_counter() - t0
+        dt = runs[0][0].g.contents.dt
         if spec is None:
-            spec = source_spectrum(lib, runs[0][0].g.contents.dt, steps)
-        for sc, (run, pids) in zip(scs, runs):
+            spec = source_spectrum(lib, dt, steps)
+        for index, sc, (run, pids) in zip(indices[b0:b0 + batch], scs, runs):
             tf = np.stack([run.dft(p) for p in pids]) / spec[None, :]
+            if out_dir is not None:
+                write_scene(lib, out_dir, index, sc, tf, dt, steps, seed)
             labels.append(np.concatenate([tf.real, tf.imag], axis=1).astype(np.float32))
             maps.append(scene_map(sc))
             coords.append(scene_coords(sc))
             run.close()
     return np.stack(maps), np.stack(coords), np.stack(labels), elapsed
+
+
+SCENE_FORMAT = "rfsyn-fdtd-scene/1"
+
+
+def scene_record(index, sc, dt, steps, seed, tf_file):
+    """The JSON description of a labelled scene: the grid, the boundary,
+    the source, every layer with its material, the probes and the layout of
+    its transfer-function file (probe x frequency complex values, the
+    probe's DFT over the source waveform's, as fdtd_write_transfer writes
+    them)."""
+    n = sc["n"]
+    return {
+        "format": SCENE_FORMAT,
+        "index": int(index), "seed": int(seed),
+        "grid": [n, n, n], "dx_m": 0.5e-3, "courant": 0.99, "dt_s": float(dt), "steps": int(steps),
+        "periodic_axes": ["x", "y"],
+        "cpml": {"axes": ["z"], "cells": sc["npml"], "m": 3.0, "ma": 1.0, "kappa_max": 1.0,
+                 "alpha_max": 0.05, "r0": 1e-8},
+        "source": {"kind": "plane", "soft": True, "component": "Ex", "axis": "z", "index": sc["src_k"],
+                   "waveform": "modulated_gaussian", "f0_hz": 40e9, "bandwidth_20db_hz": 30e9},
+        "layers": [
+            {"name": "grating", "z": list(sc["grating"]), "period_cells": sc["period"], "duty": sc["duty"],
+             "width_cells": max(1, int(round(sc["duty"] * sc["period"]))),
+             "material": {"eps_inf": 1.0, "drude": [{"wp_rad_s": 1.37e16, "gamma_rad_s": 1e14}]}},
+            {"name": "lorentz", "z": list(sc["lorentz"]),
+             "material": {"eps_inf": 2.25, "lorentz": [
+                 {"delta_eps": 1.0, "w0_rad_s": 2 * math.pi * 60e9, "delta_rad_s": 0.05 * 2 * math.pi * 60e9},
+                 {"delta_eps": 0.5, "w0_rad_s": 2 * math.pi * 90e9, "delta_rad_s": 0.05 * 2 * math.pi * 90e9}]}},
+            {"name": "substrate", "z": list(sc["substrate"]), "material": {"eps_r": 4.0, "sigma_s_m": 0.0}},
+        ],
+        "probes": [{"component": "Ex", "node": [int(v) for v in p]} for p in sc["probes"]],
+        "freqs_hz": [float(f) for f in freqs()],
+        "transfer": {"file": tf_file, "dtype": "complex128 as float64 (re, im) pairs, little-endian",
+                     "shape": [len(sc["probes"]), N_FREQS], "order": "probe-major"},
+    }
+
+
+def write_scene(lib, out_dir, index, sc, tf, dt, steps, seed):
+    """scene_<index>.tf (fdtd_write_transfer) and scene_<index>.json."""
+    os.makedirs(out_dir, exist_ok=True)
+    tf = np.ascontiguousarray(tf, dtype=np.complex128)
+    re, im = np.ascontiguousarray(tf.real), np.ascontiguousarray(tf.imag)
+    name = f"scene_{index:06d}"
+    dp = C.POINTER(C.c_double)
+    if lib.fdtd_write_transfer(os.path.join(out_dir, name + ".tf").encode(), re.ctypes.data_as(dp),
+                               im.ctypes.data_as(dp), tf.shape[0], tf.shape[1]) != 0:
+        raise OSError(f"writing {name}.tf failed")
+    with open(os.path.join(out_dir, name + ".json"), "w") as f:
+        json.dump(scene_record(index, sc, dt, steps, seed, name + ".tf"), f, indent=1)
+
+
+def read_scene(lib, out_dir, index):
+    """(record, transfer functions [probes, freqs] complex128) of a scene
+    written by write_scene."""
+    with open(os.path.join(out_dir, f"scene_{index:06d}.json")) as f:
+        rec = json.load(f)
+    if rec.get("format") != SCENE_FORMAT:
+        raise ValueError(f"not a {SCENE_FORMAT} record")
+    p, nf = rec["transfer"]["shape"]
+    re, im = np.zeros((p, nf)), np.zeros((p, nf))
+    dp = C.POINTER(C.c_double)
+    if lib.fdtd_read_transfer(os.path.join(out_dir, rec["transfer"]["file"]).encode(), re.ctypes.data_as(dp),
+                              im.ctypes.data_as(dp), p, nf) != 0:
+        raise OSError(f"reading {rec['transfer']['file']} failed")
+    return rec, re + 1j * im
 
 
 def make_net(lib, n=64, batch=256, seed=1, width=512, channels=(16, 32, 64, 128)):
