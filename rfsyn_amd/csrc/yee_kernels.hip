@@ -231,23 +231,27 @@ k_tensor(rf_geom_t gg, T *ex, T *ey, T *ez, const T *hx, const T *hy, const T *h
     } else {
         value = e.diag * curl_full<T>(g, H, c, x);
     }
-    int slot = 0;
-    for (int d = 0; d < 3; d++) {
-        if (d == c) continue;
+    // The eight neighbour curls are gathered first, whatever the mask (every
+    // neighbour position is a wrapped in-grid node), so their loads issue
+    // together; the masked ones are then added in slot order, the same
+    // operations as adding each present curl in turn.
+    T cd[8];
+    for (int slot = 0; slot < 2; slot++) {
+        // the other two components in increasing order
+        const int d = slot == 0 ? (c == 0 ? 1 : 0) : (c == 2 ? 1 : 2);
         const int lo = c < d ? c : d;
         for (int q = 0; q < 4; q++) {
-            if (!((e.mask >> (slot * 4 + q)) & 1u)) continue;
             const int s_lo = q & 1, s_hi = (q >> 1) & 1;
             const int s_c = (c == lo) ? s_lo : s_hi;
             const int s_d = (d == lo) ? s_lo : s_hi;
             Pos y = x;
             if (s_c) y.x[c] = nxt(g, c, x.x[c]);
             if (s_d) y.x[d] = prv(g, d, x.x[d]);
-            const T cd = e.kind == 2 ? curl_hz<T>(g, hz, d, y) : curl_full<T>(g, H, d, y);
-            value += e.w[slot * 4 + q] * cd;
+            cd[slot * 4 + q] = e.kind == 2 ? curl_hz<T>(g, hz, d, y) : curl_full<T>(g, H, d, y);
         }
-        slot++;
     }
+    for (int t8 = 0; t8 < 8; t8++)
+        if ((e.mask >> t8) & 1u) value += e.w[t8] * cd[t8];
     E[c][idx] += e.scale * value;
 }
 
