@@ -42,6 +42,10 @@ rf_event_t *rf_event_create(void);
 void rf_event_destroy(rf_event_t *e);
 int rf_event_record(rf_event_t *e, rf_stream_t *s);
 int rf_event_elapsed_ms(rf_event_t *a, rf_event_t *b, float *ms);
+/* An event for ordering only (no timing), and a wait of stream s on it:
+ * inside a graph capture the pair forks and joins a second stream. */
+rf_event_t *rf_event_create_sync(void);
+int rf_stream_wait(rf_stream_t *s, rf_event_t *e);
 
 /* ----------------------------------------------------------- geometry */
 

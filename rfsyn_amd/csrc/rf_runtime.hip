@@ -187,6 +187,21 @@ int rf_event_record(rf_event_t *e, rf_stream_t *s) {
     return 0;
 }
 
+rf_event_t *rf_event_create_sync(void) {
+    if (rf_init() != 0) return nullptr;
+    rf_event_t *e = new rf_event;
+    if (hipEventCreateWithFlags(&e->e, hipEventDisableTiming) != hipSuccess) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+int rf_stream_wait(rf_stream_t *s, rf_event_t *e) {
+    RF_CHECK(hipStreamWaitEvent(raw(s), e->e, 0));
+    return 0;
+}
+
 int rf_event_elapsed_ms(rf_event_t *a, rf_event_t *b, float *ms) {
     RF_CHECK(hipEventElapsedTime(ms, a->e, b->e));
     return 0;

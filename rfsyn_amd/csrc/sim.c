@@ -125,6 +125,9 @@ struct fdtd_sim {
     unsigned *d_ctr;         /* work-queue counters of the fused kernel */
     int src_pre, src_post;   /* sourced components before / after E */
     rf_shell_t shell;        /* the nodes around it (CPML slabs + margin) */
+    rf_stream_t *st2;        /* the shell's H update, beside the fused kernel */
+    rf_event_t *fork, *join;
+    int shell_overlap;       /* st2 in use (off for handles stepped in a batch) */
     int n_srcs, src_box[6];
     int post_in_kernel;      /* post-E plane sources inside the fused kernel */
     int ade_split_off;       /* keep the ADE update inside the fused kernel */
@@ -259,6 +262,12 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     free(s->flux);
     for (size_t q = 0; q < s->n_ev; q++) rf_event_destroy(s->ev[q]);
     free(s->ev);
+    if (s->st2) {
+        rf_sync(s->st2);
+        rf_stream_destroy(s->st2);
+    }
+    if (s->fork) rf_event_destroy(s->fork);
+    if (s->join) rf_event_destroy(s->join);
     rf_stream_destroy(s->st);
     free(s);
 }
@@ -502,6 +511,19 @@ static int finalize(fdtd_sim_t *s) {
         s->leap_coop = (e = getenv("RFSYN_LEAP_COOP")) ? atoi(e) : 0;
         if (s->leap_coop < 0 || s->leap_coop > 255) s->leap_coop = 0;
         s->ade_split_off = (e = getenv("RFSYN_ADE_SPLIT")) && atoi(e) == 0;
+        /* the shell's H update reads state n only and writes H^{n+1/2} of
+         * nodes outside the fused box, so it runs on a second stream beside
+         * the fused kernel (RFSYN_SHELL_OVERLAP=0: after it, on one stream) */
+        if (s->shell.n && !((e = getenv("RFSYN_SHELL_OVERLAP")) && atoi(e) == 0)) {
+            s->st2 = rf_stream_create();
+            s->fork = rf_event_create_sync();
+            s->join = rf_event_create_sync();
+            if (!s->st2 || !s->fork || !s->join) {
+                rf_set_error("second stream for the shell: %s", rf_error_string());
+                return -1;
+            }
+            s->shell_overlap = 1;
+        }
     }
     if (s->n_flux) {
         if (!s->fused) {
@@ -720,7 +742,9 @@ static int ade_pass_box(const fdtd_sim_t *s, int lo[3], int hi[3]) {
 
 /* The fused step: the leapfrog kernel on the plain box (timed in event
  * slots 0-1), then the split ADE pass (slots 2-3) and the shell kernels
- * around the box (slots 4-5). */
+ * around the box (slots 4-5; with a second stream the shell's H update
+ * runs beside the leapfrog and slots 4-5 time the wait for it and the
+ * shell's E update). */
 /* The fused kernel's arguments for the current buffers and the handle's
  * tiling requests; *m receives the materials it reads. */
 static int fill_leap(fdtd_sim_t *s, rf_leap_t *out, const rf_mat_t **mout, int alo[3], int ahi[3]) {
@@ -824,6 +848,18 @@ static int launch_fused(fdtd_sim_t *s, int *caps_done) {
     rf_caps_t cs;
     if (caps_done) *caps_done = 0;
     if (fill_leap(s, &a, &m, alo, ahi)) return -1;
+    const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
+    rf_zpsi_t z = {s->zpsi, s->zpar, s->zc ? s->cp.d.len[2] : 0};
+    const rf_zpsi_t *zp = s->zc ? &z : NULL;
+    /* the shell's H update beside the fused kernel: it reads state n only
+     * and writes H^{n+1/2} and the psi of nodes outside the fused box (z
+     * psi: the write slot of its own columns, which the fused kernel only
+     * reads from the read slot); the shell's E update waits for both */
+    const int overlap = s->shell.n && s->shell_overlap;
+    if (overlap && (rf_event_record(s->fork, s->st) || rf_stream_wait(s->st2, s->fork) ||
+                    rf_launch_shell_h(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, a.cb, s->dx, cp, zp, s->st2) ||
+                    rf_event_record(s->join, s->st2)))
+        return -1;
     if (mark(s, 0) || rf_launch_leapfrog(s->prec, &a, s->st) || mark(s, 1) || mark(s, 2)) return -1;
     if (a.ade_split) {
         const int with_caps = caps_done && caps_in_ade_pass(s, alo, ahi, &cs);
@@ -834,11 +870,10 @@ static int launch_fused(fdtd_sim_t *s, int *caps_done) {
     }
     if (mark(s, 3) || mark(s, 4)) return -1;
     if (s->shell.n) {
-        const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
-        rf_zpsi_t z = {s->zpsi, s->zpar, s->zc ? s->cp.d.len[2] : 0};
-        const rf_zpsi_t *zp = s->zc ? &z : NULL;
-        if (rf_launch_shell_h(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, a.cb, s->dx, cp, zp, s->st) ||
-            rf_launch_shell_e(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, s->f2, s->dx, cp, zp, m, s->d_srcs,
+        if (overlap ? rf_stream_wait(s->st, s->join)
+                    : rf_launch_shell_h(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, a.cb, s->dx, cp, zp, s->st))
+            return -1;
+        if (rf_launch_shell_e(s->prec, &s->geom, &s->shell, s->f, s->f2 + 3, s->f2, s->dx, cp, zp, m, s->d_srcs,
                               s->n_srcs, s->d_step, s->st))
             return -1;
     }
@@ -1230,6 +1265,20 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
                 rf_set_error("fdtd_sim_run_many: a handle appears twice");
                 return -1;
             }
+    }
+    /* scenes stepped together already overlap one another: a second stream
+     * per handle only adds cross-stream waits between the few hardware
+     * queues, so batched handles update their shell on their one stream
+     * (graphs captured with the fork are dropped and captured again) */
+    for (size_t q = 0; q < n_sims && n_sims > 1; q++) {
+        fdtd_sim_t *s = sims[q];
+        if (!s->shell_overlap) continue;
+        if (rf_sync(s->st)) return -1;
+        s->shell_overlap = 0;
+        for (int p = 0; p < 2; p++) {
+            rf_graph_destroy(s->graphs[p]);
+            s->graphs[p] = NULL;
+        }
     }
     for (size_t q = 0; q < n_sims; q++)
         if (run_prepare(sims[q], n_steps)) return -1;

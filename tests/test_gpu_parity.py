@@ -165,6 +165,20 @@ def test_smoothed_dielectric_fp32_vs_fp64(gpu):
     assert np.max(np.abs(b["dft"] - a["dft"]) / peak) <= 1e-4
 
 
+def test_shell_overlap_bitwise_fp32(gpu, monkeypatch):
+    """The CPML shell's H update on a second stream beside the fused kernel
+    (the default) and after it on the one stream give the same bits (c2's
+    scene at 48^3: a shell on every face, z psi in the store, tensor
+    edges)."""
+    a = _c2_small(gpu, capi.FDTD_FP32, 120)
+    monkeypatch.setenv("RFSYN_SHELL_OVERLAP", "0")
+    b = _c2_small(gpu, capi.FDTD_FP32, 120)
+    assert np.max(np.abs(a["td"])) > 0
+    assert same_values(a["td"], b["td"])
+    for k in a["fields"]:
+        assert same_values(a["fields"][k], b["fields"][k]), k
+
+
 def _wide_plane_run(lib, with_monitors, steps=40, n=(512, 512, 24), nf=128):
     """Periodic x/y, 4-cell CPML on z, Ex plane source; two full-plane
     monitors of nf frequencies (2 x 2 x nf x 512^2 doubles of accumulators)
