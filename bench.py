@@ -241,21 +241,25 @@ class C2:
     npml = (12, 12, 12)
 
     def __init__(self):
-        self.stats = {}
         self.tag_bytes = 2
         self.ade_values = 0
         self.config = {"workload": "c2: 256^3 40 spheres + 23-degree tilted slab, 3x3 subpixel tensors, "
                                    "12-cell CPML, 16 Ez probes x 64-freq DFT, 3000 steps",
                        "grid": list(self.n), "dx_m": 1e-3, "courant": 0.99}
 
-    def build(self, lib, prec, td_cap):
+    def build(self, lib, prec, td_cap, record=True):
+        """The scene painted by lib (on the device for the product library);
+        with record, its counts and paint time go into the config (the last
+        device build, FP64, paints with the runtime already initialised)."""
         from rfsyn_amd.driver import build_c2
-        run, pids = build_c2(lib, prec, td_cap, stats_out=self.stats)
-        self.config["scene"] = dict(self.stats)
+        stats = {}
+        run, pids = build_c2(lib, prec, td_cap, stats_out=stats)
+        if record:
+            self.config["scene"] = stats
         return run, pids, []
 
     def cpu_sample(self, lib, steps):
-        run, pids, _ = self.build(lib, capi.FDTD_FP64, 0)
+        run, pids, _ = self.build(lib, capi.FDTD_FP64, 0, record=False)
         run.run(1)
         t0 = time.perf_counter()
         run.run(steps)
