@@ -116,6 +116,10 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]);
  * receives 1 when the two-kernel step was kept.  0 on success. */
 int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
+/* Diagnostics: streams the handle's next step uses (1, or 2 when the
+ * CPML shell's H update runs on a second stream beside the fused kernel;
+ * handles advanced together by fdtd_sim_run_many use one). */
+int fdtd_sim_streams(const fdtd_sim_t *s);
 
 /* Copies min(n, recorded) time-domain samples; returns the count or -1. */
 long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n);

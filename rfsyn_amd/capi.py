@@ -210,6 +210,7 @@ EXTENSION_API = {
     "fdtd_sim_add_plane_monitor": (C.c_int, [_vp, C.c_int, _sz, _dp, _sz]),
     "fdtd_sim_run": (C.c_int, [_vp, _sz]),
     "fdtd_sim_steps_done": (_sz, [_vp]),
+    "fdtd_sim_streams": (C.c_int, [_vp]),
     "fdtd_sim_probe_td": (C.c_long, [_vp, C.c_int, _dp, _sz]),
     "fdtd_sim_probe_iq": (C.c_int, [_vp, C.c_int, _sz, C.c_int, C.c_float, C.c_uint64, C.c_uint64,
                                     C.POINTER(C.c_float)]),

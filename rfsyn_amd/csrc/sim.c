@@ -1266,6 +1266,9 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
                 return -1;
             }
     }
+    /* first-run set-up (finalize may enable the shell's second stream) */
+    for (size_t q = 0; q < n_sims; q++)
+        if (run_prepare(sims[q], n_steps)) return -1;
     /* scenes stepped together already overlap one another: a second stream
      * per handle only adds cross-stream waits between the few hardware
      * queues, so batched handles update their shell on their one stream
@@ -1280,8 +1283,6 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
             s->graphs[p] = NULL;
         }
     }
-    for (size_t q = 0; q < n_sims; q++)
-        if (run_prepare(sims[q], n_steps)) return -1;
     /* one block of each handle in turn: each handle's stream runs on its
      * own, so small scenes fill the GPU together */
     size_t *left = malloc((n_sims ? n_sims : 1) * sizeof *left);
@@ -1305,6 +1306,11 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
 }
 
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }
+
+int fdtd_sim_streams(const fdtd_sim_t *s) {
+    if (!s) return -1;
+    return 1 + (s->st2 && s->shell_overlap && s->shell.n ? 1 : 0);
+}
 
 int fdtd_sim_fused(const fdtd_sim_t *s) { return s ? s->fused : -1; }
 

@@ -45,3 +45,43 @@ def test_run_many_rejects_repeated_handle(gpu):
     arr = (C.c_void_p * 2)(run.sim, run.sim)
     assert gpu.fdtd_sim_run_many(arr, 2, 5) == -1
     run.close()
+
+
+def test_run_many_keeps_one_stream_per_handle(gpu):
+    """Handles stepped only through fdtd_sim_run_many (here never run
+    alone, so their first-run set-up happens inside the batch call) update
+    their CPML shell on their one stream; a handle run alone uses a second
+    stream for the shell's H update."""
+    specs = _specs(3)
+    runs = [build_c4_scene(gpu, sp, capi.FDTD_FP32, 0, npml=6)[0] for sp in specs]
+    run_many(gpu, runs, 40)
+    assert [gpu.fdtd_sim_streams(r.sim) for r in runs] == [1, 1, 1]
+    lone, _ = build_c4_scene(gpu, specs[0], capi.FDTD_FP32, 0, npml=6)
+    lone.run(40)
+    assert gpu.fdtd_sim_streams(lone.sim) == 2
+    for r in runs + [lone]:
+        r.close()
+
+
+def test_c4_scenes_fp64_vs_oracle(gpu, oracle):
+    """Three seeded scenes of the 96-scene batch workload at its smallest
+    sizes (n in [48, 64] per axis, 10-cell CPML, lossy boxes, a dipole on a
+    random axis, 4 probes x 64-frequency DFTs) advanced together with
+    fdtd_sim_run_many in FP64: series and fields bit for bit against the
+    oracle, DFTs within 1e-12 of the scene's largest accumulator."""
+    steps = 220
+    specs = [c4_scene(q, seed=4, npml=10, n_lo=48, n_hi=64) for q in range(3)]
+    runs = [build_c4_scene(gpu, sp, capi.FDTD_FP64, steps, npml=10) for sp in specs]
+    run_many(gpu, [r for r, _ in runs], steps)
+    for sp, (run, pids) in zip(specs, runs):
+        td, dft, f = _outputs(run, pids, steps)
+        run.close()
+        ref, rp = build_c4_scene(oracle, sp, capi.FDTD_FP64, steps, npml=10)
+        ref.run(steps)
+        td_o, dft_o, f_o = _outputs(ref, rp, steps)
+        ref.close()
+        assert max(np.max(np.abs(x)) for x in td_o) > 0
+        assert all(same_values(a, b) for a, b in zip(td, td_o))
+        for k in f_o:
+            assert same_values(f[k], f_o[k]), k
+        assert np.max(np.abs(dft - dft_o)) <= 1e-12 * np.max(np.abs(dft_o))

@@ -74,6 +74,17 @@ def test_c1_fp64_reduced(gpu, oracle, block):
     assert _dft_close(dft_g, dft_o)
 
 
+def test_c1_fp64_full(gpu, oracle):
+    """The whole 64^3 dipole workload (2000 steps, 8 probes x 64
+    frequencies) in FP64: every probe series bit for bit against the
+    oracle, every DFT accumulator within 1e-12 of the largest."""
+    td_g, dft_g = run_c1(gpu, steps=2000)
+    td_o, dft_o = run_c1(oracle, steps=2000)
+    assert np.max(np.abs(td_o)) > 0
+    assert same_values(td_g, td_o)
+    assert _dft_close(dft_g, dft_o)
+
+
 def test_c1_fp32_vs_fp64(gpu):
     """FP32 fields with FP64 compensated DFT against the FP64 GPU run."""
     _, d64 = run_c1(gpu, steps=600)

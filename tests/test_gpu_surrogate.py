@@ -20,7 +20,22 @@ CONFIGS = {
     # grid, C0, channels, coords, width, outputs, queries, batch
     "tiny": (16, 2, (4, 8, 8, 16), 6, 32, 6, 3, 3),
     "wide": (32, 3, (8, 16, 16, 32), 6, 96, 70, 4, 4),
+    # the network of the surrogate workload: 64^3 voxels, 3 input channels,
+    # 16-32-64-128 channels, 78 coordinate features, width 512, 8 probes x
+    # 64 frequencies (real and imaginary), two scenes
+    "c5": (64, 3, (16, 32, 64, 128), 78, 512, 128, 8, 2),
 }
+
+
+# Forward tolerance relative to the largest output.  At the surrogate
+# workload's size (64^3 voxels, 16-32-64-128 channels) the oracle's own
+# outputs move by about 2e-3 under an FP32-level (1e-7 relative) change of
+# the parameters: such a change flips BF16 roundings of a few dozen
+# activations of the deeper convolutions, and the flips cascade.  The
+# device, which accumulates in FP32 in its own order, is held to five times
+# that spread there; the parameter gradients move by at most 3e-3 under the
+# same change, within their 2e-2 bound.
+FWD_TOL = {"tiny": 1e-3, "wide": 1e-3, "c5": 1e-2}
 
 
 def _setup(gpu, name, seed=3):
@@ -41,7 +56,7 @@ def test_forward_and_gradient_match_oracle(gpu, name):
     params = cfg.unflatten(flat.astype(np.float64))
     y_ref, _ = sr.forward(cfg, params, maps, coords)
     y = net.predict(maps, coords)
-    assert np.max(np.abs(y - y_ref)) <= 1e-3 * np.max(np.abs(y_ref)) + 1e-6
+    assert np.max(np.abs(y - y_ref)) <= FWD_TOL[name] * np.max(np.abs(y_ref)) + 1e-6
     loss, g = net.gradient(maps, coords, labels)
     loss_ref, g_ref = sr.loss_and_grad(cfg, params, maps, coords, labels)
     assert abs(loss - loss_ref) <= 1e-4 * loss_ref
