@@ -138,6 +138,17 @@ long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n);
  * recorded. */
 int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
                       uint64_t stream, float *out_iq);
+/* fdtd_sim_probe_iq without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: tight
+ * serving loops, callers that forbid hipMalloc).  The same kernels, the
+ * same results; the power and the I/Q samples are staged in a fixed
+ * device array of the library (about 512 KiB per GPU, held until the
+ * process ends) instead of buffers allocated per call, so n is at most
+ * FDTD_IQ_NOALLOC_MAX (-1 beyond).  Every call on one GPU uses the same
+ * array: calls that overlap on a GPU (from several threads) race. */
+#define FDTD_IQ_NOALLOC_MAX 65536
+int fdtd_sim_probe_iq_noalloc(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                              uint64_t stream, float *out_iq);
 /* The CLI's FDTD stage and, optionally, its AWGN stage in one call
  * (extension; cli/fdtd_stage.c:10-29 and settings.c:52-80): the echo of
  * fdtd_simulate_echo run on the device in `precision` (FP64 gives the

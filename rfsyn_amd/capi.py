@@ -214,6 +214,8 @@ EXTENSION_API = {
     "fdtd_sim_probe_td": (C.c_long, [_vp, C.c_int, _dp, _sz]),
     "fdtd_sim_probe_iq": (C.c_int, [_vp, C.c_int, _sz, C.c_int, C.c_float, C.c_uint64, C.c_uint64,
                                     C.POINTER(C.c_float)]),
+    "fdtd_sim_probe_iq_noalloc": (C.c_int, [_vp, C.c_int, _sz, C.c_int, C.c_float, C.c_uint64, C.c_uint64,
+                                            C.POINTER(C.c_float)]),
     "fdtd_write_iq": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), _sz]),
     "fdtd_write_transfer": (C.c_int, [C.c_char_p, _dp, _dp, _sz, _sz]),
     "fdtd_read_transfer": (C.c_int, [C.c_char_p, _dp, _dp, _sz, _sz]),

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "rf_device.h"
 #include "rf_hip.h"
@@ -86,6 +87,30 @@ __global__ void k_iq_noise(const double *__restrict__ td, long long n, int noise
 }
 
 }  // namespace
+
+// Device memory of the output stage's variant that allocates nothing
+// (rf_iq_scratch): the mean power in the first 256 bytes, then up to
+// RF_IQ_SCRATCH_SAMPLES I/Q samples.  Each GPU holds its own copy until the
+// process ends; every call on a GPU shares it.
+static __device__ __align__(256) unsigned char scratch[256 + 2 * RF_IQ_SCRATCH_SAMPLES * sizeof(float)];
+
+extern "C" int rf_iq_scratch(double **power, float **iq) {
+    // The array's name is a host-side stand-in; its address on the current
+    // device comes from the runtime, once per device.
+    static std::mutex mu;
+    static void *addr[RF_MAX_DEVICES];
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= RF_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        addr[dev] = p;
+    }
+    *power = static_cast<double *>(addr[dev]);
+    *iq = reinterpret_cast<float *>(static_cast<unsigned char *>(addr[dev]) + 256);
+    return 0;
+}
 
 extern "C" int rf_launch_iq_power(const double *td, long long n, double *power, rf_stream_t *s) {
     if (n < 0 || (n && !td) || !power) return -1;

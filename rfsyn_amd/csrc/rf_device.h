@@ -384,6 +384,13 @@ int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const in
 int rf_launch_iq_power(const double *td, long long n, double *power, rf_stream_t *s);
 int rf_launch_iq_noise(const double *td, long long n, int noise, float std_f, uint64_t state0, uint64_t inc,
                        float *out, rf_stream_t *s);
+/* The output stage's device memory that is never allocated: a file-scope
+ * array of the library, on the current device (set up on the first call
+ * per device).  *power: one double; *iq: RF_IQ_SCRATCH_SAMPLES I/Q pairs.
+ * 0 on success. */
+#define RF_IQ_SCRATCH_SAMPLES 65536
+#define RF_MAX_DEVICES 64
+int rf_iq_scratch(double **power, float **iq);
 
 int rf_launch_plane_captures(int prec, const rf_geom_t *g, const rf_caps_t *cs, int ring_len,
                              const long long *d_step, rf_stream_t *s);

@@ -1331,19 +1331,27 @@ long fdtd_sim_probe_td(fdtd_sim_t *s, int probe, double *out, size_t n) {
     return (long)c;
 }
 
-int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
-                      uint64_t stream, float *out_iq) {
-    if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out_iq)) return -1;
+/* Arguments of the output stage; -1 (with the error set) when they are
+ * invalid or fewer than n samples are recorded. */
+static int probe_iq_check(fdtd_sim_t *s, int probe, size_t n, const float *out_iq) {
+    if (!s || probe < 0 || (size_t)probe >= s->n_probes || (n && !out_iq)) {
+        rf_set_error("bad probe or output");
+        return -1;
+    }
     size_t have = s->steps < s->probes[probe].td_cap ? s->steps : s->probes[probe].td_cap;
     if (n > have) {
         rf_set_error("probe %d holds %zu recorded samples, %zu asked", probe, have, n);
         return -1;
     }
-    if (n == 0) return 0;
+    return 0;
+}
+
+/* The output stage with its device memory given: the mean power in
+ * d_pow[0], the n I/Q samples in d_iq (2n floats). */
+static int probe_iq_into(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                         uint64_t stream, float *out_iq, double *d_pow, float *d_iq) {
     const double *td = s->d_td + (size_t)probe * s->td_stride;
-    double *d_pow = NULL;
-    float *d_iq = NULL;
-    int rc = rf_malloc((void **)&d_pow, sizeof(double)) || rf_malloc((void **)&d_iq, 2 * n * sizeof(float));
+    int rc = 0;
     float std_f = 0.f;
     uint64_t state0 = 0, inc = (stream << 1u) | 1u;
     if (!rc && add_noise) {
@@ -1363,9 +1371,39 @@ int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float s
     }
     if (!rc) rc = rf_launch_iq_noise(td, (long long)n, add_noise != 0, std_f, state0, inc, d_iq, s->st) ||
                   rf_d2h(out_iq, d_iq, 2 * n * sizeof(float), s->st);
+    return rc ? -1 : 0;
+}
+
+int fdtd_sim_probe_iq(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                      uint64_t stream, float *out_iq) {
+    if (probe_iq_check(s, probe, n, out_iq)) return -1;
+    if (n == 0) return 0;
+    double *d_pow = NULL;
+    float *d_iq = NULL;
+    int rc = rf_malloc((void **)&d_pow, sizeof(double)) || rf_malloc((void **)&d_iq, 2 * n * sizeof(float));
+    if (!rc) rc = probe_iq_into(s, probe, n, add_noise, snr_db, seed, stream, out_iq, d_pow, d_iq);
     rf_free(d_pow);
     rf_free(d_iq);
     return rc ? -1 : 0;
+}
+
+_Static_assert(FDTD_IQ_NOALLOC_MAX <= RF_IQ_SCRATCH_SAMPLES, "the scratch array holds the largest no-alloc call");
+
+int fdtd_sim_probe_iq_noalloc(fdtd_sim_t *s, int probe, size_t n, int add_noise, float snr_db, uint64_t seed,
+                              uint64_t stream, float *out_iq) {
+    if (probe_iq_check(s, probe, n, out_iq)) return -1;
+    if (n > FDTD_IQ_NOALLOC_MAX) {
+        rf_set_error("fdtd_sim_probe_iq_noalloc: %zu samples asked, at most %d", n, FDTD_IQ_NOALLOC_MAX);
+        return -1;
+    }
+    if (n == 0) return 0;
+    double *d_pow = NULL;
+    float *d_iq = NULL;
+    if (rf_iq_scratch(&d_pow, &d_iq)) {
+        rf_set_error("fdtd_sim_probe_iq_noalloc: device scratch: %s", rf_error_string());
+        return -1;
+    }
+    return probe_iq_into(s, probe, n, add_noise, snr_db, seed, stream, out_iq, d_pow, d_iq);
 }
 
 int fdtd_write_iq(const char *path, const float *iq, size_t n) {
