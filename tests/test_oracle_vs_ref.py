@@ -1,7 +1,8 @@
 """The CPU oracle against the rfsyn reference compiled from its own sources
 (oracle/Makefile `ref`), bit for bit.  Skipped where the reference sources
-are absent; the committed golden vectors (test_oracle_golden.py) pin the
-oracle there."""
+are absent; there the reference's own output fixture (tests/golden/,
+read by test_golden_echo.py and test_output_stage.py), the validators'
+published figures and the closed forms of the other oracle tests pin it."""
 import ctypes as C
 
 import numpy as np
