@@ -336,8 +336,12 @@ def bench_c5(args, lib):
 def bench_c4(args, lib):
     """96 seeded scenes (sizes U{48..192} per axis, 10-cell CPML, 1-12
     lossy boxes, one dipole, 4 probes x 64-frequency DFT), FP32, advanced
-    together with fdtd_sim_run_many; Mcell-updates/s over all scenes."""
-    from rfsyn_amd.driver import build_c4_scene, c4_scene, run_many
+    together by a packed batch (fdtd_batch_*: one leapfrog launch per
+    tiling variant, one launch each for all shells' H and E, one for all
+    probe gathers per step, a block of steps captured as one graph);
+    Mcell-updates/s over all scenes.  The same handles are then stepped
+    with one stream per handle (the earlier batching) for comparison."""
+    from rfsyn_amd.driver import Batch, build_c4_scene, c4_scene, run_many
     specs = [c4_scene(q) for q in range(96)]
     cells = sum(sp["n"][0] * sp["n"][1] * sp["n"][2] for sp in specs)
     runs = [build_c4_scene(lib, sp, capi.FDTD_FP32, 0) for sp in specs]
@@ -347,10 +351,19 @@ def bench_c4(args, lib):
         for r in handles:
             assert lib.fdtd_sim_autotune(r.sim, 2, None) == 0, lib.fdtd_last_error()
         print(f"[c4] autotuned 96 scenes in {time.perf_counter() - t_tune:.1f} s", file=sys.stderr, flush=True)
-    run_many(lib, handles, args.warmup)
+    batch = Batch(lib, handles)
+    batch.run(args.warmup)
     t0 = time.perf_counter()
-    run_many(lib, handles, args.steps)
+    batch.run(args.steps)
     wall = time.perf_counter() - t0
+    launches = batch.launches()
+    batch.close()
+    os.environ["RFSYN_PACKED"] = "0"
+    run_many(lib, handles, args.warmup)
+    t1 = time.perf_counter()
+    run_many(lib, handles, args.steps)
+    wall_streams = time.perf_counter() - t1
+    os.environ.pop("RFSYN_PACKED")
     one = time.perf_counter()
     for r in handles[:8]:   # the same 8 scenes one after another, for the batching gain
         r.run(args.steps)
@@ -372,6 +385,8 @@ def bench_c4(args, lib):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(byt / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": byt},
+        "packed_launches_per_step": {"leapfrog": launches[0], "shell": launches[1], "probes": 1},
+        "streams_per_scene_mcell_s": round(cells * args.steps / wall_streams / 1e6, 3),
         "sequential_8_scenes_mcell_s": round(cells8 * args.steps / one / 1e6, 3),
         "autotuned": not args.no_autotune,
     }

@@ -84,12 +84,40 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index,
 int fdtd_sim_run(fdtd_sim_t *s, size_t n_steps);
 
 /* Batch of independent scenes (extension: the reference runs one scene
- * per process): advances every handle n_steps, enqueueing one block of
- * steps of each handle in turn on its own stream without waiting, so the
- * scenes run concurrently on the GPU; returns when all are done.  Each
- * handle ends in the state fdtd_sim_run would give it.  A handle may
- * appear once. */
+ * per process): advances every handle n_steps; returns when all are done.
+ * Each handle ends in the state fdtd_sim_run would give it.  A handle may
+ * appear once.  When every handle is on the fused step with the same
+ * precision and dft_block, the scenes are stepped by packed launches (a
+ * temporary fdtd_batch_t, below); otherwise one block of steps of each
+ * handle is enqueued in turn on its own stream. */
 int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps);
+
+/* Packed batch (extension): many scenes of uneven sizes stepped together,
+ * each kernel kind once per step for all scenes.  Per step, one fused
+ * leapfrog launch covers the boxes of every scene whose tiling variant
+ * matches (rows per tile, material-tag width, z CPML): the work items of
+ * all scenes form one queue and each descriptor of a device table holds
+ * one scene's buffers, CPML, materials and sources.  The CPML shells of
+ * all scenes take one launch for H and one for E, all probe gathers one,
+ * and per block of dft_block steps all point-probe DFTs one.  A block of
+ * steps of the whole batch is one captured graph.  A scene's other
+ * launches (split ADE pass, tensor edges, post-E plane sources, flux
+ * planes, plane monitors) keep their place in its step.  Every handle
+ * ends in the state fdtd_sim_run would give it, bit for bit.
+ * fdtd_batch_create: 1..65535 distinct handles with equal precision and
+ * dft_block (NULL otherwise); the handles must outlive the batch.
+ * fdtd_batch_run: -1 if a handle is not on the fused step (see
+ * fdtd_sim_select_path) or is profiling.  A handle stepped on its own
+ * between runs is picked up where it is. */
+typedef struct fdtd_batch fdtd_batch_t;
+fdtd_batch_t *fdtd_batch_create(fdtd_sim_t *const *sims, size_t n_sims);
+int fdtd_batch_run(fdtd_batch_t *b, size_t n_steps);
+void fdtd_batch_destroy(fdtd_batch_t *b);
+size_t fdtd_batch_size(const fdtd_batch_t *b);
+/* Diagnostics after a run: packed leapfrog launches per step (one per
+ * tiling variant) and shell launches per step.  0, or -1 before the
+ * first run. */
+int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell);
 
 /* Tiling autotuner (extension): before the first step, times the fused
  * step's kernel for each candidate tiling (rows per tile, planes per work

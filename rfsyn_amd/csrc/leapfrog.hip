@@ -334,25 +334,42 @@ __device__ __forceinline__ T zterm(T diff, T kinv, T b, T aa, bool slab, T old, 
     return diff * kinv + dx * psi;
 }
 
-template <typename T, typename TagT, bool ZC, int R, bool COOP>
-__global__ void __launch_bounds__(WL *WPB)
-k_leapfrog(rf_leap_t a) {
-    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+// LDS view of the tables the fused step stages: the sources with their
+// value at the step, the ADE coefficient sets and the first ncache entries
+// of the material table (Ca, Cb per component, packed flags, packed ADE
+// set ids).
+template <typename T>
+struct LeapLds {
+    LSrc *src;
+    typename EntryOf<T>::ade *ade;
+    T *ca, *cb;
+    uint32_t *fl, *ai;
+    int nls, ncache, n_adec;
+};
+
+template <typename T>
+__device__ __forceinline__ LeapLds<T> leap_lds_view(unsigned char *base, int n_srcs, int n_adec, int ncache) {
     using Ade = typename EntryOf<T>::ade;
-    // the ADE coefficient sets and the first a.ncache entries of the
-    // material table, staged in LDS before any wavefront of the block may
-    // leave
-    extern __shared__ __align__(16) unsigned char s_raw[];
-    const int ncache = a.ncache, n_adec = a.n_adec;
-    const int nls = a.n_srcs <= MAX_LSRC ? a.n_srcs : 0;   // more sources: read from HBM
-    LSrc *s_src = reinterpret_cast<LSrc *>(s_raw);
-    Ade *s_ade = reinterpret_cast<Ade *>(s_raw + (size_t)nls * sizeof(LSrc));
-    T *s_ca = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(s_ade) + (size_t)n_adec * sizeof(Ade));
-    T *s_cb = s_ca + 3 * ncache;
-    uint32_t *s_fl = reinterpret_cast<uint32_t *>(s_cb + 3 * ncache);
-    uint32_t *s_ai = s_fl + ncache;
-    const long long step = *a.d_step;
-    for (int e = threadIdx.x; e < nls; e += blockDim.x) {
+    LeapLds<T> v;
+    v.nls = n_srcs <= MAX_LSRC ? n_srcs : 0;   // more sources: read from HBM
+    v.ncache = ncache;
+    v.n_adec = n_adec;
+    v.src = reinterpret_cast<LSrc *>(base);
+    v.ade = reinterpret_cast<Ade *>(base + (size_t)v.nls * sizeof(LSrc));
+    v.ca = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(v.ade) + (size_t)n_adec * sizeof(Ade));
+    v.cb = v.ca + 3 * ncache;
+    v.fl = reinterpret_cast<uint32_t *>(v.cb + 3 * ncache);
+    v.ai = v.fl + ncache;
+    return v;
+}
+
+// Stages the tables of `a` at `step` into v: the caller's threads t0,
+// t0 + nt, ... share the work; the caller orders the stores before the
+// reads (a block barrier, or a wavefront fence within one wavefront).
+template <typename T, bool TAGS>
+__device__ __forceinline__ void leap_stage(const rf_leap_t &a, const LeapLds<T> &v, long long step, int t0, int nt) {
+    using Ade = typename EntryOf<T>::ade;
+    for (int e = t0; e < v.nls; e += nt) {
         const rf_leap_src_t &p = a.srcs[e];
         LSrc q;
         q.w = p.wave[step];
@@ -365,99 +382,47 @@ k_leapfrog(rf_leap_t a) {
         q.i = p.i;
         q.j = p.j;
         q.k = p.k;
-        s_src[e] = q;
+        v.src[e] = q;
     }
-    if (!TAGS && nls > 0) __syncthreads();
     if constexpr (TAGS) {
         using Entry = typename EntryOf<T>::type;
         const Entry *ents = static_cast<const Entry *>(a.entries);
-        for (int e = threadIdx.x; e < ncache; e += blockDim.x) {
+        for (int e = t0; e < v.ncache; e += nt) {
             const Entry &en = ents[e];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                s_ca[3 * e + c] = en.Ca[c];
-                s_cb[3 * e + c] = en.Cb[c];
+                v.ca[3 * e + c] = en.Ca[c];
+                v.cb[3 * e + c] = en.Cb[c];
             }
-            s_fl[e] = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
+            v.fl[e] = en.flags[0] | (en.flags[1] << 8) | (en.flags[2] << 16);
             uint32_t ai = 0;
 #pragma unroll
             for (int c = 0; c < 3; c++) ai |= (static_cast<uint32_t>(en.ade[c]) & 0xffu) << (8 * c);
-            s_ai[e] = ai;
+            v.ai[e] = ai;
         }
         const Ade *ades = static_cast<const Ade *>(a.ade);
-        for (int e = threadIdx.x; e < n_adec; e += blockDim.x) s_ade[e] = ades[e];
-        __syncthreads();
+        for (int e = t0; e < v.n_adec; e += nt) v.ade[e] = ades[e];
     }
+}
 
-    const int lane = threadIdx.x & (WL - 1);
-    // Work queue: the items are split in N_XCD contiguous ranges, one per
-    // XCD (blocks are dealt round-robin to the XCDs), so the tiles an XCD
-    // runs at a time are neighbours that share halo rows in its L2; each
-    // taker gets the next item of its XCD's counter until the range is
-    // exhausted, which balances tiles of unequal cost (dispersive, source
-    // and CPML columns).  The counters of this launch are a.ctr[a.qpar][*]
-    // (a.ctr points at the pair of sets of its part); the other set is
-    // zeroed here for the next step.
-    //
-    // Independent mode (a.gz == 0): an item is one (tile, x chunk) and each
-    // wavefront takes its own.  Cooperative mode: an item is a group of
-    // gz x gy tiles (gz neighbouring z tiles by gy neighbouring row tiles)
-    // of one x chunk, taken by a block of gz * gy wavefronts that march the
-    // planes together (a block barrier per plane, or at most gk planes
-    // apart), so the lines that neighbouring tiles share (the halo rows)
-    // are fetched once and hit in L1/L2 for the other wavefront instead of
-    // being fetched again planes later.  The plan uses gz = 1: tiles of
-    // one z tile cost the same per plane (z CPML columns), so no wavefront
-    // of a block idles waiting for a costlier neighbour.
-    const int xcd = blockIdx.x % N_XCD;
-    if (blockIdx.x == 0 && threadIdx.x < N_XCD) a.ctr[(1 - a.qpar) * N_XCD + threadIdx.x] = 0u;
-    const int item_lo = (int)(((long long)a.n_items * xcd) / N_XCD);
-    const int item_hi = (int)(((long long)a.n_items * (xcd + 1)) / N_XCD);
-    unsigned *my_ctr = a.ctr + a.qpar * N_XCD + xcd;
+// One work item of the fused step: z tile tl (of the launch's part), row
+// tile tj and x chunk tc, marched plane by plane through the chunk.  In
+// cooperative mode the wavefront keeps pace with the others of its block
+// (lane_wave, wid and s_prog as in k_leapfrog).
+template <typename T, typename TagT, bool ZC, int R, bool COOP>
+__device__ __forceinline__ void leap_item(const rf_leap_t &a, const LeapLds<T> &lv, long long step, int tl, int tj,
+                                          int tc, int lane, bool lane_wave, int wid, int *s_prog) {
+    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+    using Ade = typename EntryOf<T>::ade;
     const Geo g = geo_of(a.g);
-    const int wid = static_cast<int>(threadIdx.x) / WL;
-    // cooperative mode: the block's ticket and, with a plane slack
-    // (a.gk > 0), the planes each wavefront of the group has completed
-    __shared__ unsigned s_ticket;
-    __shared__ int s_prog[WPB_MAX];
-    for (;;) {
-    const int ntl = a.tkr[1] + a.tkr[3];   // z tiles of this part
-    int tl, tj, tc;
-    bool valid = true;
-    bool lane_wave = false;   // lane < group size and that wavefront has a tile
-    if constexpr (COOP) {
-        __syncthreads();   // every wavefront has read the previous ticket
-        if (threadIdx.x == 0) s_ticket = atomicAdd(my_ctr, 1u);
-        if (lane == 0) s_prog[wid] = 0;
-        __syncthreads();
-        const int grp = item_lo + static_cast<int>(s_ticket);
-        if (grp >= item_hi) break;   // block-uniform
-        const int nzg = (ntl + a.gz - 1) / a.gz, nyg = (a.ntj + a.gy - 1) / a.gy;
-        const int tl0 = (grp % nzg) * a.gz, tj0 = ((grp / nzg) % nyg) * a.gy;
-        tl = tl0 + wid % a.gz;
-        tj = tj0 + wid / a.gz;
-        tc = grp / (nzg * nyg);
-        valid = tl < ntl && tj < a.ntj;
-        lane_wave = lane < a.gz * a.gy && tl0 + lane % a.gz < ntl && tj0 + lane / a.gz < a.ntj;
-    } else {
-        unsigned ticket = 0;
-        if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
-        const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
-        if (item >= item_hi) break;
-        tl = item % ntl;
-        tj = (item / ntl) % a.ntj;
-        tc = item / (ntl * a.ntj);
-    }
-    if (!valid) {
-        // a wavefront of the block without a tile: only the barriers of
-        // the group's planes (the count depends on the chunk alone)
-        if (a.gk == 0) {
-            const int ib = a.ib0[0] + tc * a.chunk, ie = min(ib + a.chunk, a.ib1[0]);
-            const int n_it = (ie - ib) + ((ib > 0 || g.px) ? 1 : 0);
-            for (int it = 0; it < n_it; it++) __builtin_amdgcn_s_barrier();
-        }
-        continue;
-    }
+    const int nls = lv.nls, ncache = lv.ncache, n_adec = lv.n_adec;
+    const LSrc *s_src = lv.src;
+    const Ade *s_ade = lv.ade;
+    const T *s_ca = lv.ca, *s_cb = lv.cb;
+    const uint32_t *s_fl = lv.fl, *s_ai = lv.ai;
+    (void)lane_wave;
+    (void)wid;
+    (void)s_prog;
     const int tk = tl < a.tkr[1] ? a.tkr[0] + tl : a.tkr[2] + (tl - a.tkr[1]);
 
     // the box of nodes this launch owns, [ib0, ib1) per axis; along z the
@@ -766,6 +731,142 @@ k_leapfrog(rf_leap_t a) {
                 *reinterpret_cast<volatile int *>(&s_prog[wid]) = it + 1;
         }
     }
+}
+
+template <typename T, typename TagT, bool ZC, int R, bool COOP>
+__global__ void __launch_bounds__(WL *WPB)
+k_leapfrog(rf_leap_t a) {
+    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+    // the sources, the ADE coefficient sets and the first a.ncache entries
+    // of the material table, staged in LDS before any wavefront of the
+    // block may leave
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    const LeapLds<T> lv = leap_lds_view<T>(s_raw, a.n_srcs, a.n_adec, a.ncache);
+    const long long step = *a.d_step;
+    leap_stage<T, TAGS>(a, lv, step, threadIdx.x, blockDim.x);
+    if (TAGS || lv.nls > 0) __syncthreads();
+    const int lane = threadIdx.x & (WL - 1);
+    // Work queue: the items are split in N_XCD contiguous ranges, one per
+    // XCD (blocks are dealt round-robin to the XCDs), so the tiles an XCD
+    // runs at a time are neighbours that share halo rows in its L2; each
+    // taker gets the next item of its XCD's counter until the range is
+    // exhausted, which balances tiles of unequal cost (dispersive, source
+    // and CPML columns).  The counters of this launch are a.ctr[a.qpar][*]
+    // (a.ctr points at the pair of sets of its part); the other set is
+    // zeroed here for the next step.
+    //
+    // Independent mode (a.gz == 0): an item is one (tile, x chunk) and each
+    // wavefront takes its own.  Cooperative mode: an item is a group of
+    // gz x gy tiles (gz neighbouring z tiles by gy neighbouring row tiles)
+    // of one x chunk, taken by a block of gz * gy wavefronts that march the
+    // planes together (a block barrier per plane, or at most gk planes
+    // apart), so the lines that neighbouring tiles share (the halo rows)
+    // are fetched once and hit in L1/L2 for the other wavefront instead of
+    // being fetched again planes later.  The plan uses gz = 1: tiles of
+    // one z tile cost the same per plane (z CPML columns), so no wavefront
+    // of a block idles waiting for a costlier neighbour.
+    const int xcd = blockIdx.x % N_XCD;
+    if (blockIdx.x == 0 && threadIdx.x < N_XCD) a.ctr[(1 - a.qpar) * N_XCD + threadIdx.x] = 0u;
+    const int item_lo = (int)(((long long)a.n_items * xcd) / N_XCD);
+    const int item_hi = (int)(((long long)a.n_items * (xcd + 1)) / N_XCD);
+    unsigned *my_ctr = a.ctr + a.qpar * N_XCD + xcd;
+    const Geo g = geo_of(a.g);
+    const int wid = static_cast<int>(threadIdx.x) / WL;
+    // cooperative mode: the block's ticket and, with a plane slack
+    // (a.gk > 0), the planes each wavefront of the group has completed
+    __shared__ unsigned s_ticket;
+    __shared__ int s_prog[WPB_MAX];
+    for (;;) {
+    const int ntl = a.tkr[1] + a.tkr[3];   // z tiles of this part
+    int tl, tj, tc;
+    bool valid = true;
+    bool lane_wave = false;   // lane < group size and that wavefront has a tile
+    if constexpr (COOP) {
+        __syncthreads();   // every wavefront has read the previous ticket
+        if (threadIdx.x == 0) s_ticket = atomicAdd(my_ctr, 1u);
+        if (lane == 0) s_prog[wid] = 0;
+        __syncthreads();
+        const int grp = item_lo + static_cast<int>(s_ticket);
+        if (grp >= item_hi) break;   // block-uniform
+        const int nzg = (ntl + a.gz - 1) / a.gz, nyg = (a.ntj + a.gy - 1) / a.gy;
+        const int tl0 = (grp % nzg) * a.gz, tj0 = ((grp / nzg) % nyg) * a.gy;
+        tl = tl0 + wid % a.gz;
+        tj = tj0 + wid / a.gz;
+        tc = grp / (nzg * nyg);
+        valid = tl < ntl && tj < a.ntj;
+        lane_wave = lane < a.gz * a.gy && tl0 + lane % a.gz < ntl && tj0 + lane / a.gz < a.ntj;
+    } else {
+        unsigned ticket = 0;
+        if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
+        const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
+        if (item >= item_hi) break;
+        tl = item % ntl;
+        tj = (item / ntl) % a.ntj;
+        tc = item / (ntl * a.ntj);
+    }
+    if (!valid) {
+        // a wavefront of the block without a tile: only the barriers of
+        // the group's planes (the count depends on the chunk alone)
+        if (a.gk == 0) {
+            const int ib = a.ib0[0] + tc * a.chunk, ie = min(ib + a.chunk, a.ib1[0]);
+            const int n_it = (ie - ib) + ((ib > 0 || g.px) ? 1 : 0);
+            for (int it = 0; it < n_it; it++) __builtin_amdgcn_s_barrier();
+        }
+        continue;
+    }
+    leap_item<T, TagT, ZC, R, COOP>(a, lv, step, tl, tj, tc, lane, lane_wave, wid, s_prog);
+    }
+}
+
+// ------------------------------------------------------- packed scenes
+// The fused steps of many scenes in one launch.  The work items of all
+// scenes form one queue (scene sc owns the items item_base[sc] ..
+// item_base[sc] + tab[sc].n_items, item_scene names the scene of each),
+// split in N_XCD contiguous ranges as in k_leapfrog, so an XCD works
+// through whole scenes and keeps their halos in its L2.  The four
+// wavefronts of a block take items on their own, from any scene: each
+// stages the tables of the scene it works on in its own LDS region when
+// it moves to an item of another scene.  LDS operations of one wavefront
+// complete in order, so its staging stores precede its later reads and
+// no block barrier is needed.
+template <typename T, typename TagT, bool ZC, int R>
+__global__ void __launch_bounds__(WL *WPB)
+k_leapfrog_packed(const rf_leap_t *__restrict__ tab, const int *__restrict__ item_base,
+                  const unsigned short *__restrict__ item_scene, int n_items, unsigned *ctr, int qpar,
+                  int lds_wave) {
+    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    const int lane = threadIdx.x & (WL - 1);
+    const int wid = static_cast<int>(threadIdx.x) / WL;
+    unsigned char *mine = s_raw + (size_t)wid * lds_wave;
+    const int xcd = blockIdx.x % N_XCD;
+    if (blockIdx.x == 0 && threadIdx.x < N_XCD) ctr[(1 - qpar) * N_XCD + threadIdx.x] = 0u;
+    const int item_lo = (int)(((long long)n_items * xcd) / N_XCD);
+    const int item_hi = (int)(((long long)n_items * (xcd + 1)) / N_XCD);
+    unsigned *my_ctr = ctr + qpar * N_XCD + xcd;
+    int cur = -1;
+    LeapLds<T> lv = leap_lds_view<T>(mine, 0, 0, 0);
+    long long step = 0;
+    for (;;) {
+        unsigned ticket = 0;
+        if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
+        const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
+        if (item >= item_hi) break;
+        const int sc = __builtin_amdgcn_readfirstlane(static_cast<int>(item_scene[item]));
+        const rf_leap_t &a = tab[sc];
+        if (sc != cur) {
+            lv = leap_lds_view<T>(mine, a.n_srcs, a.n_adec, a.ncache);
+            step = *a.d_step;
+            leap_stage<T, TAGS>(a, lv, step, lane, WL);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            cur = sc;
+        }
+        const int local = item - item_base[sc];
+        const int ntl = a.tkr[1] + a.tkr[3];
+        leap_item<T, TagT, ZC, R, false>(a, lv, step, local % ntl, (local / ntl) % a.ntj, local / (ntl * a.ntj),
+                                         lane, false, wid, nullptr);
     }
 }
 
@@ -835,15 +936,17 @@ __device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, 
     return diff * kinv[k] + dx * psi;
 }
 
+// H^{n+1/2} of shell node t.
 template <typename T, bool CPML>
-__global__ void __launch_bounds__(256)
-k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__restrict__ ey,
-          const T *__restrict__ ez, const T *__restrict__ hxa, const T *__restrict__ hya,
-          const T *__restrict__ hza, T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb, T dx,
-          rf_cpml_t cp, rf_zpsi_t zs) {
+__device__ __forceinline__ void shell_h_node(const rf_geom_t &gg, const rf_shell_t &sh, long long t,
+                                             const T *__restrict__ ex, const T *__restrict__ ey,
+                                             const T *__restrict__ ez, const T *__restrict__ hxa,
+                                             const T *__restrict__ hya, const T *__restrict__ hza,
+                                             T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb,
+                                             T dx, const rf_cpml_t &cp, const rf_zpsi_t &zs) {
     const Geo g = geo_of(gg);
     int i, j, k;
-    if (!shell_node(sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!shell_node(sh, t, i, j, k)) return;
     const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
     const bool do_x = (i < g.nx + !g.px) && j < g.ny && k < g.nz;
     const bool do_y = i < g.nx && (j < g.ny + !g.py) && k < g.nz;
@@ -892,6 +995,29 @@ k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__rest
     hzb[idx] = hz;
 }
 
+template <typename T, bool CPML>
+__global__ void __launch_bounds__(256)
+k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__restrict__ ey,
+          const T *__restrict__ ez, const T *__restrict__ hxa, const T *__restrict__ hya,
+          const T *__restrict__ hza, T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb, T dx,
+          rf_cpml_t cp, rf_zpsi_t zs) {
+    shell_h_node<T, CPML>(gg, sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, ex, ey, ez, hxa, hya, hza, hxb,
+                          hyb, hzb, cb, dx, cp, zs);
+}
+
+// The shell H updates of many scenes: block b updates nodes of the job
+// block_job[b], from the job's first block on.
+template <typename T, bool CPML>
+__global__ void __launch_bounds__(256)
+k_shell_h_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *__restrict__ block_job) {
+    const rf_shell_job_t &jb = jobs[block_job[blockIdx.x]];
+    const T *const *fa = reinterpret_cast<const T *const *>(jb.fa);
+    T *const *hb = reinterpret_cast<T *const *>(jb.hb);
+    shell_h_node<T, CPML>(jb.g, jb.sh, (long long)(blockIdx.x - jb.block0) * blockDim.x + threadIdx.x, fa[0], fa[1],
+                          fa[2], fa[3], fa[4], fa[5], hb[0], hb[1], hb[2], static_cast<T>(jb.cb),
+                          static_cast<T>(jb.dx), jb.cp, jb.zs);
+}
+
 // E update of one component on a shell node (e_component of the
 // two-kernel path, returning the value).
 template <typename T, typename TagT>
@@ -914,15 +1040,18 @@ __device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_
     }
 }
 
+// Sources and E^{n+1} of shell node t.
 template <typename T, typename TagT, bool CPML>
-__global__ void __launch_bounds__(256)
-k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__restrict__ eya,
-          const T *__restrict__ eza, const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
-          T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_zpsi_t zs,
-          rf_mat_t m, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
+__device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell_t &sh, long long t,
+                                             const T *__restrict__ exa, const T *__restrict__ eya,
+                                             const T *__restrict__ eza, const T *__restrict__ hx,
+                                             const T *__restrict__ hy, const T *__restrict__ hz,
+                                             T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx,
+                                             const rf_cpml_t &cp, const rf_zpsi_t &zs, const rf_mat_t &m,
+                                             const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
     const Geo g = geo_of(gg);
     int i, j, k;
-    if (!shell_node(sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!shell_node(sh, t, i, j, k)) return;
     const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
     // E ranges (grid.c:95-117): own axis from 0, the other axes from 1
     // unless periodic (PEC planes at 0 and n are never updated)
@@ -983,6 +1112,29 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
     exb[idx] = e0;
     eyb[idx] = e1;
     ezb[idx] = e2;
+}
+
+template <typename T, typename TagT, bool CPML>
+__global__ void __launch_bounds__(256)
+k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__restrict__ eya,
+          const T *__restrict__ eza, const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
+          T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_zpsi_t zs,
+          rf_mat_t m, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
+    shell_e_node<T, TagT, CPML>(gg, sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, exa, eya, eza, hx, hy, hz,
+                                exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step);
+}
+
+// The shell E updates of many scenes (block b: job block_job[b]).
+template <typename T, typename TagT, bool CPML>
+__global__ void __launch_bounds__(256)
+k_shell_e_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *__restrict__ block_job) {
+    const rf_shell_job_t &jb = jobs[block_job[blockIdx.x]];
+    const T *const *fa = reinterpret_cast<const T *const *>(jb.fa);
+    const T *const *hb = reinterpret_cast<const T *const *>(jb.hb);
+    T *const *eb = reinterpret_cast<T *const *>(jb.eb);
+    shell_e_node<T, TagT, CPML>(jb.g, jb.sh, (long long)(blockIdx.x - jb.block0) * blockDim.x + threadIdx.x, fa[0],
+                                fa[1], fa[2], hb[0], hb[1], hb[2], eb[0], eb[1], eb[2], static_cast<T>(jb.dx), jb.cp,
+                                jb.zs, jb.m, jb.srcs, jb.n_srcs, jb.d_step);
 }
 
 template <typename T>
@@ -1320,6 +1472,53 @@ int launch_rows(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
     return -1;
 }
 
+// ------------------------------------------------ packed-scene launchers
+template <typename T, typename TagT, int R>
+const void *packed_fn(int zc) {
+    return zc ? reinterpret_cast<const void *>(k_leapfrog_packed<T, TagT, true, R>)
+              : reinterpret_cast<const void *>(k_leapfrog_packed<T, TagT, false, R>);
+}
+
+template <typename T, int R>
+const void *packed_fn_rows(int tag_bytes, int zc) {
+    switch (tag_bytes) {
+    case 0: return packed_fn<T, NoTag, R>(zc);
+    case 1: return packed_fn<T, uint8_t, R>(zc);
+    case 2: return packed_fn<T, uint16_t, R>(zc);
+    case 4: return packed_fn<T, uint32_t, R>(zc);
+    }
+    return nullptr;
+}
+
+const void *packed_kernel(int prec, const rf_leap_pack_t *pk) {
+    if (prec == 0) {
+        if (pk->rows == 2) return packed_fn_rows<double, 2>(pk->tag_bytes, pk->zc);
+        if (pk->rows == 4) return packed_fn_rows<double, 4>(pk->tag_bytes, pk->zc);
+    } else {
+        if (pk->rows == 4) return packed_fn_rows<float, 4>(pk->tag_bytes, pk->zc);
+        if (pk->rows == 8) return packed_fn_rows<float, 8>(pk->tag_bytes, pk->zc);
+    }
+    return nullptr;
+}
+
+template <typename T, typename TagT, bool CPML>
+void shell_e_packed_launch(const rf_shell_job_t *jobs, const unsigned short *bj, int n_blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_shell_e_packed<T, TagT, CPML>), dim3(n_blocks), dim3(256), 0, st, jobs, bj);
+}
+
+template <typename T, bool CPML>
+int shell_e_packed_tag(const rf_shell_job_t *jobs, const unsigned short *bj, int n_blocks, int tag_bytes,
+                       hipStream_t st) {
+    switch (tag_bytes) {
+    case 0: shell_e_packed_launch<T, NoTag, CPML>(jobs, bj, n_blocks, st); break;
+    case 1: shell_e_packed_launch<T, uint8_t, CPML>(jobs, bj, n_blocks, st); break;
+    case 2: shell_e_packed_launch<T, uint16_t, CPML>(jobs, bj, n_blocks, st); break;
+    case 4: shell_e_packed_launch<T, uint32_t, CPML>(jobs, bj, n_blocks, st); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace
 
 extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves) {
@@ -1553,4 +1752,67 @@ extern "C" int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t 
     hipStream_t st = rf_raw_stream(s);
     return prec == 0 ? shell_e_t<double>(g, sh, ea, hb, eb, dx, cpml, z, mat, srcs, n_srcs, d_step, st)
                      : shell_e_t<float>(g, sh, ea, hb, eb, dx, cpml, z, mat, srcs, n_srcs, d_step, st);
+}
+
+extern "C" size_t rf_leap_lds_bytes(int prec, const rf_leap_t *a) {
+    const size_t b = prec == 0 ? leap_lds<double>(a) : leap_lds<float>(a);
+    return (b + 15) / 16 * 16;
+}
+
+extern "C" int rf_leap_pack_plan(int prec, rf_leap_pack_t *pk) {
+    const void *fn = packed_kernel(prec, pk);
+    if (!fn || pk->n_items <= 0 || pk->lds_wave < 0 || pk->lds_wave % 16) return -1;
+    int nb = 0, cus = 0, dev = 0;
+    const size_t lds = (size_t)pk->lds_wave * WPB;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, WL * WPB, lds) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nb <= 0)
+        return -1;
+    long long blocks = (long long)nb * cus;
+    const long long need = (pk->n_items + WPB - 1) / WPB;
+    if (blocks > need) blocks = need;
+    pk->blocks = (int)((blocks + N_XCD - 1) / N_XCD * N_XCD);
+    return 0;
+}
+
+extern "C" int rf_launch_leapfrog_packed(int prec, const rf_leap_pack_t *pk, rf_stream_t *s) {
+    if (pk->n_items <= 0) return 0;
+    const void *fn = packed_kernel(prec, pk);
+    if (!fn || !pk->tab || !pk->item_base || !pk->item_scene || !pk->ctr || pk->blocks <= 0 ||
+        pk->n_scenes <= 0 || pk->n_scenes > 65535 || pk->qpar < 0 || pk->qpar > 1)
+        return -1;
+    const size_t lds = (size_t)pk->lds_wave * WPB;
+    void *args[] = {const_cast<rf_leap_t **>(&pk->tab), const_cast<int **>(&pk->item_base),
+                    const_cast<unsigned short **>(&pk->item_scene), const_cast<int *>(&pk->n_items),
+                    const_cast<unsigned **>(&pk->ctr), const_cast<int *>(&pk->qpar),
+                    const_cast<int *>(&pk->lds_wave)};
+    if (hipLaunchKernel(fn, dim3(pk->blocks), dim3(WL * WPB), args, lds, rf_raw_stream(s)) != hipSuccess) return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int rf_launch_shell_h_packed(int prec, const rf_shell_job_t *jobs, const unsigned short *block_job,
+                                        int n_blocks, int has_cpml, rf_stream_t *s) {
+    if (n_blocks <= 0) return 0;
+    if (!jobs || !block_job) return -1;
+    hipStream_t st = rf_raw_stream(s);
+    if (prec == 0) {
+        if (has_cpml) hipLaunchKernelGGL((k_shell_h_packed<double, true>), dim3(n_blocks), dim3(256), 0, st, jobs, block_job);
+        else hipLaunchKernelGGL((k_shell_h_packed<double, false>), dim3(n_blocks), dim3(256), 0, st, jobs, block_job);
+    } else {
+        if (has_cpml) hipLaunchKernelGGL((k_shell_h_packed<float, true>), dim3(n_blocks), dim3(256), 0, st, jobs, block_job);
+        else hipLaunchKernelGGL((k_shell_h_packed<float, false>), dim3(n_blocks), dim3(256), 0, st, jobs, block_job);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int rf_launch_shell_e_packed(int prec, const rf_shell_job_t *jobs, const unsigned short *block_job,
+                                        int n_blocks, int tag_bytes, int has_cpml, rf_stream_t *s) {
+    if (n_blocks <= 0) return 0;
+    if (!jobs || !block_job) return -1;
+    hipStream_t st = rf_raw_stream(s);
+    if (prec == 0)
+        return has_cpml ? shell_e_packed_tag<double, true>(jobs, block_job, n_blocks, tag_bytes, st)
+                        : shell_e_packed_tag<double, false>(jobs, block_job, n_blocks, tag_bytes, st);
+    return has_cpml ? shell_e_packed_tag<float, true>(jobs, block_job, n_blocks, tag_bytes, st)
+                    : shell_e_packed_tag<float, false>(jobs, block_job, n_blocks, tag_bytes, st);
 }

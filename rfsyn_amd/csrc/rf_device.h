@@ -312,6 +312,92 @@ int rf_launch_shell_e(int prec, const rf_geom_t *g, const rf_shell_t *sh, void *
                       const rf_mat_t *mat, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
                       rf_stream_t *s);
 
+/* ------------------------------------------------------ packed scenes
+ * The steps of many scenes in one launch per kernel kind.  Every scene's
+ * arguments are a descriptor in a device table; the launch's work items
+ * or blocks carry the scene they belong to. */
+
+/* Fused steps of n_scenes scenes: tab[sc] is scene sc's rf_leap_t
+ * (planned with one part, independent wavefronts, ncache and n_adec
+ * within lds_wave bytes per wavefront), its work items are the global
+ * items item_base[sc] .. item_base[sc] + tab[sc].n_items, and
+ * item_scene[item] names the scene of each.  All scenes use the variant
+ * (rows, tag_bytes, zc).  ctr: 2 x 8 zero-initialised counters, set qpar
+ * this launch (the other set is zeroed by it). */
+typedef struct {
+    const rf_leap_t *tab;
+    const int *item_base;
+    const unsigned short *item_scene;
+    int n_items, n_scenes;
+    unsigned *ctr;
+    int qpar;
+    int rows, tag_bytes, zc;
+    int lds_wave;
+    int blocks;                 /* set by rf_leap_pack_plan */
+} rf_leap_pack_t;
+/* LDS bytes one wavefront needs to stage a's tables. */
+size_t rf_leap_lds_bytes(int prec, const rf_leap_t *a);
+/* pk->blocks for a launch of pk->n_items items (resident wavefronts of
+ * the variant, at most one block per 4 items). */
+int rf_leap_pack_plan(int prec, rf_leap_pack_t *pk);
+int rf_launch_leapfrog_packed(int prec, const rf_leap_pack_t *pk, rf_stream_t *s);
+
+/* The shell update of one scene in a packed launch: its arguments as in
+ * rf_launch_shell_h / rf_launch_shell_e, and the first block of its
+ * nodes in the launch (block0; its nodes take the following
+ * ceil(nodes / 256) blocks). */
+typedef struct {
+    rf_geom_t g;
+    rf_shell_t sh;
+    const void *fa[6];          /* state n: ex ey ez hx hy hz (A) */
+    void *hb[3];                /* H^{n+1/2} (B) */
+    void *eb[3];                /* E^{n+1} (B) */
+    double cb, dx;
+    rf_cpml_t cp;               /* valid when the launch's has_cpml */
+    rf_zpsi_t zs;               /* zpsi NULL: psi arrays of cp */
+    rf_mat_t m;
+    const rf_leap_src_t *srcs;
+    int n_srcs;
+    const long long *d_step;
+    unsigned block0;
+} rf_shell_job_t;
+/* jobs, block_job: device; block_job[b] is the job of block b of n_blocks.
+ * Every job uses the variant (tag_bytes, has_cpml). */
+int rf_launch_shell_h_packed(int prec, const rf_shell_job_t *jobs, const unsigned short *block_job, int n_blocks,
+                             int has_cpml, rf_stream_t *s);
+int rf_launch_shell_e_packed(int prec, const rf_shell_job_t *jobs, const unsigned short *block_job, int n_blocks,
+                             int tag_bytes, int has_cpml, rf_stream_t *s);
+
+/* Probe gathers of many scenes (rf_launch_probe_record per job, one block
+ * each; every job's step counter advances). */
+typedef struct {
+    const void *f[6];
+    const long long *idx;
+    const int *comp;
+    int n;
+    double *ring;
+    int ring_len;
+    double *td;
+    long long td_cap;
+    long long *d_step;
+} rf_probe_job_t;
+int rf_launch_probe_record_packed(int prec, const rf_probe_job_t *jobs, int n_jobs, rf_stream_t *s);
+
+/* Point-probe DFTs of many scenes over the last `count` steps
+ * (rf_launch_probe_dft per job; jobs with n_freq 0 do nothing). */
+typedef struct {
+    const double *ring;
+    int ring_len, n_probes;
+    const int *freq_probe;
+    int n_freq;
+    const double *freq_w;
+    double dt;
+    const long long *d_step;
+    double *acc_re, *acc_im, *cmp_re, *cmp_im;
+    int comp_sum;
+} rf_pdft_job_t;
+int rf_launch_probe_dft_packed(const rf_pdft_job_t *jobs, int n_jobs, int max_freq, int count, rf_stream_t *s);
+
 /* Plane-monitor captures in one launch: capture q copies the plane
  * `index` of `axis` of field array f (cells (ia, ib), the other two axes
  * in increasing order, ia < na, ib < nb) into slot step % ring_len of

@@ -84,6 +84,7 @@ struct fdtd_sim {
 
     wave_t *waves; size_t n_waves;
     size_t wave_cap;
+    unsigned wave_gen;    /* counts reallocations of the wave tables */
     int *pt_wave; source_t *pts; size_t n_pts;
     rf_point_src_t *d_pts;
     splane_t *planes; size_t n_planes;
@@ -685,6 +686,7 @@ static int ensure_waves(fdtd_sim_t *s, size_t need) {
     }
     free(host);
     s->wave_cap = cap;
+    s->wave_gen++;
     if (s->n_pts) {
         rf_point_src_t *hp = calloc(s->n_pts, sizeof *hp);
         if (!hp) return -1;
@@ -1256,6 +1258,8 @@ int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel) {
     return 0;
 }
 
+static int batch_run(fdtd_batch_t *b, size_t n_steps);
+
 int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
     if (!sims && n_sims) return -1;
     for (size_t q = 0; q < n_sims; q++) {
@@ -1265,6 +1269,17 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
                 rf_set_error("fdtd_sim_run_many: a handle appears twice");
                 return -1;
             }
+    }
+    /* packed launches for all scenes when they allow it (a scene that
+     * cannot be packed is found before any step is enqueued) */
+    const char *pe = getenv("RFSYN_PACKED");
+    if (n_sims > 1 && !(pe && pe[0] == '0')) {
+        fdtd_batch_t *b = fdtd_batch_create(sims, n_sims);
+        if (b) {
+            const int rc = batch_run(b, n_steps);
+            fdtd_batch_destroy(b);
+            if (rc != -2) return rc ? -1 : 0;
+        }
     }
     /* first-run set-up (finalize may enable the shell's second stream) */
     for (size_t q = 0; q < n_sims; q++)
@@ -1303,6 +1318,616 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps) {
     }
     free(left);
     return rc;
+}
+
+/* ------------------------------------------------------------ batches */
+
+/* Scenes stepped by packed launches (fdtd_batch_*): per step one fused
+ * leapfrog launch over the boxes of every scene of a variant (rows per
+ * tile, tag width, z CPML), one launch each for the shells' H and E
+ * updates, one for all probe gathers, and per block of K steps one for
+ * all point-probe DFTs.  A scene's other launches of the step (split ADE
+ * pass, tensor edges, post-E plane sources, flux planes, plane-monitor
+ * captures and DFTs) run in their place of the step's order.  Descriptors
+ * of every scene for both buffer parities live in device tables, so a
+ * block of K steps (K even) is one captured graph for the whole batch. */
+#define BATCH_LDS_WAVE 8192     /* LDS bytes per wavefront of the packed leapfrog */
+
+typedef struct {
+    int rows, tag_bytes, zc;
+    int n;                      /* scenes of the variant */
+    int *scene;                 /* their batch indices */
+    rf_leap_t *d_tab[2];
+    int *d_base;
+    unsigned short *d_item_scene;
+    unsigned *d_ctr;
+    rf_leap_pack_t pk;
+} bgroup_t;
+
+typedef struct {
+    int tag_bytes, has_cpml;
+    int n_blocks;
+    rf_shell_job_t *d_jobs[2];
+    unsigned short *d_block_job;
+} bshell_t;
+
+typedef struct {
+    int ade;                    /* split ADE pass box */
+    int alo[3], ahi[3];
+    size_t expect;              /* the handle's steps after the batch's last run */
+    unsigned wave_gen;          /* its wave tables when the tables were built */
+    int parity;                 /* its buffer parity at phase 0 */
+} bscene_t;
+
+struct fdtd_batch {
+    fdtd_sim_t **sims;
+    size_t n;
+    int prec, K;
+    rf_stream_t *st;
+    bscene_t *sc;
+    bgroup_t *groups;
+    int n_groups;
+    bshell_t *shells;
+    int n_shells;
+    rf_probe_job_t *d_probe[2];
+    rf_pdft_job_t *d_pdft;
+    int max_freq;
+    int built;
+    int phase;                  /* steps through the batch since the tables were built, mod 2 */
+    rf_graph_t *graphs[2];      /* captured block per phase */
+};
+
+static void batch_drop_graphs(fdtd_batch_t *b) {
+    for (int q = 0; q < 2; q++) {
+        rf_graph_destroy(b->graphs[q]);
+        b->graphs[q] = NULL;
+    }
+}
+
+static void batch_free_tables(fdtd_batch_t *b) {
+    batch_drop_graphs(b);
+    for (int g = 0; g < b->n_groups; g++) {
+        bgroup_t *gr = &b->groups[g];
+        free(gr->scene);
+        rf_free(gr->d_tab[0]);
+        rf_free(gr->d_tab[1]);
+        rf_free(gr->d_base);
+        rf_free(gr->d_item_scene);
+        rf_free(gr->d_ctr);
+    }
+    free(b->groups);
+    b->groups = NULL;
+    b->n_groups = 0;
+    for (int q = 0; q < b->n_shells; q++) {
+        rf_free(b->shells[q].d_jobs[0]);
+        rf_free(b->shells[q].d_jobs[1]);
+        rf_free(b->shells[q].d_block_job);
+    }
+    free(b->shells);
+    b->shells = NULL;
+    b->n_shells = 0;
+    rf_free(b->d_probe[0]);
+    rf_free(b->d_probe[1]);
+    rf_free(b->d_pdft);
+    b->d_probe[0] = b->d_probe[1] = NULL;
+    b->d_pdft = NULL;
+    b->built = 0;
+}
+
+void fdtd_batch_destroy(fdtd_batch_t *b) {
+    if (!b) return;
+    if (b->st) rf_sync(b->st);
+    batch_free_tables(b);
+    free(b->sc);
+    free(b->sims);
+    rf_stream_destroy(b->st);
+    free(b);
+}
+
+/* The host-side buffer swap of one fused step (enqueue_fused). */
+static void swap_state(fdtd_sim_t *s) {
+    for (int q = 0; q < 6; q++) {
+        void *t = s->f[q];
+        s->f[q] = s->f2[q];
+        s->f2[q] = t;
+    }
+    void *tb = s->blk;
+    s->blk = s->blk2;
+    s->blk2 = tb;
+    s->parity ^= 1;
+    s->zpar ^= 1;
+}
+
+/* The fused kernel's descriptor of s for a packed launch: one part (no z
+ * split), independent wavefronts, tables within BATCH_LDS_WAVE. */
+static int packed_leap(fdtd_sim_t *s, rf_leap_t *a, int alo[3], int ahi[3], int *ade) {
+    const int zs = s->leap_zsplit, co = s->leap_coop;
+    const rf_mat_t *m;
+    s->leap_zsplit = 0;
+    s->leap_coop = 0;
+    const int rc = fill_leap(s, a, &m, alo, ahi);
+    s->leap_zsplit = zs;
+    s->leap_coop = co;
+    if (rc) return -1;
+    if (a->n_parts != 1) return -1;
+    for (int q = 0; q < 4; q++) a->tkr[q] = a->ptk[0][q];
+    a->zc = a->pzc[0];
+    a->n_items = a->pitems[0];
+    a->gz = a->gy = a->gk = 0;
+    while (a->ncache > 0 && rf_leap_lds_bytes(s->prec, a) > BATCH_LDS_WAVE) a->ncache--;
+    if (rf_leap_lds_bytes(s->prec, a) > BATCH_LDS_WAVE) return -1;
+    *ade = a->ade_split;
+    return 0;
+}
+
+static int h2d_new(void **dst, const void *src, size_t bytes, rf_stream_t *st) {
+    if (rf_malloc(dst, bytes ? bytes : 16)) return -1;
+    return bytes ? rf_h2d(*dst, src, bytes, st) : 0;
+}
+
+/* Builds the device tables from the handles' current state (phase 0). */
+static int batch_build(fdtd_batch_t *b) {
+    batch_free_tables(b);
+    b->max_freq = 0;
+    const size_t n = b->n;
+    int rc = -1;
+    rf_leap_t *tab[2] = {calloc(n, sizeof(rf_leap_t)), calloc(n, sizeof(rf_leap_t))};
+    rf_shell_job_t *jobs[2] = {calloc(n, sizeof(rf_shell_job_t)), calloc(n, sizeof(rf_shell_job_t))};
+    rf_probe_job_t *pj[2] = {calloc(n, sizeof(rf_probe_job_t)), calloc(n, sizeof(rf_probe_job_t))};
+    rf_pdft_job_t *dj = calloc(n, sizeof(rf_pdft_job_t));
+    int *gkey = calloc(n, sizeof(int));
+    unsigned short *bj = NULL;
+    int *base = NULL;
+    unsigned short *isc = NULL;
+    if (!tab[0] || !tab[1] || !jobs[0] || !jobs[1] || !pj[0] || !pj[1] || !dj || !gkey) goto out;
+    for (size_t q = 0; q < n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        bscene_t *c = &b->sc[q];
+        for (int ph = 0; ph < 2; ph++) {
+            int alo[3], ahi[3], ade = 0;
+            if (packed_leap(s, &tab[ph][q], alo, ahi, &ade)) {
+                if (ph) swap_state(s);
+                rf_set_error("batch: scene %zu: fused step not packable", q);
+                goto out;
+            }
+            if (ph == 0) {
+                c->ade = ade;
+                memcpy(c->alo, alo, sizeof alo);
+                memcpy(c->ahi, ahi, sizeof ahi);
+            }
+            rf_shell_job_t *j = &jobs[ph][q];
+            j->g = s->geom;
+            j->sh = s->shell;
+            for (int f = 0; f < 6; f++) j->fa[f] = s->f[f];
+            for (int f = 0; f < 3; f++) {
+                j->hb[f] = s->f2[3 + f];
+                j->eb[f] = s->f2[f];
+            }
+            j->cb = s->dt / (MU0 * s->dx);
+            j->dx = s->dx;
+            if (s->has_cpml) j->cp = s->cp.d;
+            if (s->zc) {
+                j->zs.zpsi = s->zpsi;
+                j->zs.zpar = s->zpar;
+                j->zs.zlen = s->cp.d.len[2];
+            }
+            j->m = s->has_mat ? s->mat.d : s->vac;
+            j->srcs = s->d_srcs;
+            j->n_srcs = s->n_srcs;
+            j->d_step = s->d_step;
+            /* the probe gather follows the swap */
+            swap_state(s);
+            rf_probe_job_t *p = &pj[ph][q];
+            for (int f = 0; f < 6; f++) p->f[f] = s->f[f];
+            p->idx = s->d_pidx;
+            p->comp = s->d_pcomp;
+            p->n = (int)s->n_probes;
+            p->ring = s->d_ring;
+            p->ring_len = s->K;
+            p->td = s->d_td;
+            p->td_cap = (long long)s->td_stride;
+            p->d_step = s->d_step;
+        }
+        /* two swaps: back at phase 0 */
+        c->expect = s->steps;
+        c->wave_gen = s->wave_gen;
+        c->parity = s->parity;
+        rf_pdft_job_t *d = &dj[q];
+        d->ring = s->d_ring;
+        d->ring_len = s->K;
+        d->n_probes = (int)s->n_probes;
+        d->freq_probe = s->d_fprobe;
+        d->n_freq = (int)s->n_freq_tot;
+        d->freq_w = s->d_fw;
+        d->dt = s->dt;
+        d->d_step = s->d_step;
+        d->acc_re = s->d_acc_re;
+        d->acc_im = s->d_acc_im;
+        d->cmp_re = s->d_cmp_re;
+        d->cmp_im = s->d_cmp_im;
+        d->comp_sum = s->comp_sum;
+        if (d->n_freq > b->max_freq) b->max_freq = d->n_freq;
+        const rf_leap_t *a = &tab[0][q];
+        gkey[q] = (a->rows << 8) | ((a->tag ? a->tag_bytes : 0) << 1) | a->zc;
+    }
+    /* leapfrog groups, one per variant, scenes in batch order */
+    b->groups = calloc(n, sizeof(bgroup_t));
+    if (!b->groups) goto out;
+    for (size_t q = 0; q < n; q++) {
+        int g = 0;
+        while (g < b->n_groups && (b->groups[g].rows << 8 | b->groups[g].tag_bytes << 1 | b->groups[g].zc) != gkey[q])
+            g++;
+        if (g == b->n_groups) {
+            bgroup_t *gr = &b->groups[b->n_groups++];
+            gr->rows = gkey[q] >> 8;
+            gr->tag_bytes = (gkey[q] >> 1) & 0x7f;
+            gr->zc = gkey[q] & 1;
+            gr->scene = calloc(n, sizeof(int));
+            if (!gr->scene) goto out;
+        }
+        b->groups[g].scene[b->groups[g].n++] = (int)q;
+    }
+    for (int g = 0; g < b->n_groups; g++) {
+        bgroup_t *gr = &b->groups[g];
+        long long items = 0;
+        int lds = 0;
+        for (int q = 0; q < gr->n; q++) {
+            items += tab[0][gr->scene[q]].n_items;
+            const int l = (int)rf_leap_lds_bytes(b->prec, &tab[0][gr->scene[q]]);
+            if (l > lds) lds = l;
+        }
+        if (items <= 0 || items > INT_MAX) {
+            rf_set_error("batch: %lld work items in one launch", items);
+            goto out;
+        }
+        free(base);
+        free(isc);
+        base = malloc(gr->n * sizeof *base);
+        isc = malloc((size_t)items * sizeof *isc);
+        rf_leap_t *gt[2] = {malloc(gr->n * sizeof(rf_leap_t)), malloc(gr->n * sizeof(rf_leap_t))};
+        if (!base || !isc || !gt[0] || !gt[1]) {
+            free(gt[0]);
+            free(gt[1]);
+            goto out;
+        }
+        long long at = 0;
+        for (int q = 0; q < gr->n; q++) {
+            base[q] = (int)at;
+            for (int i = 0; i < tab[0][gr->scene[q]].n_items; i++) isc[at + i] = (unsigned short)q;
+            at += tab[0][gr->scene[q]].n_items;
+            gt[0][q] = tab[0][gr->scene[q]];
+            gt[1][q] = tab[1][gr->scene[q]];
+        }
+        int e = h2d_new((void **)&gr->d_tab[0], gt[0], gr->n * sizeof(rf_leap_t), b->st) ||
+                h2d_new((void **)&gr->d_tab[1], gt[1], gr->n * sizeof(rf_leap_t), b->st) ||
+                h2d_new((void **)&gr->d_base, base, gr->n * sizeof *base, b->st) ||
+                h2d_new((void **)&gr->d_item_scene, isc, (size_t)items * sizeof *isc, b->st) ||
+                rf_malloc((void **)&gr->d_ctr, 16 * sizeof(unsigned)) || rf_sync(b->st);
+        free(gt[0]);
+        free(gt[1]);
+        if (e) goto out;
+        rf_leap_pack_t *pk = &gr->pk;
+        memset(pk, 0, sizeof *pk);
+        pk->item_base = gr->d_base;
+        pk->item_scene = gr->d_item_scene;
+        pk->n_items = (int)items;
+        pk->n_scenes = gr->n;
+        pk->ctr = gr->d_ctr;
+        pk->rows = gr->rows;
+        pk->tag_bytes = gr->tag_bytes;
+        pk->zc = gr->zc;
+        pk->lds_wave = lds;
+        if (rf_leap_pack_plan(b->prec, pk)) {
+            rf_set_error("batch: packed leapfrog plan");
+            goto out;
+        }
+    }
+    /* shell groups by (tag width, CPML), blocks in batch order */
+    b->shells = calloc(n, sizeof(bshell_t));
+    if (!b->shells) goto out;
+    for (size_t q = 0; q < n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        if (!s->shell.n || !s->shell.start[s->shell.n]) continue;
+        const int tb = jobs[0][q].m.tag ? jobs[0][q].m.tag_bytes : 0;
+        int g = 0;
+        while (g < b->n_shells && !(b->shells[g].tag_bytes == tb && b->shells[g].has_cpml == s->has_cpml)) g++;
+        if (g == b->n_shells) {
+            b->shells[g].tag_bytes = tb;
+            b->shells[g].has_cpml = s->has_cpml;
+            b->n_shells++;
+        }
+    }
+    for (int g = 0; g < b->n_shells; g++) {
+        bshell_t *sh = &b->shells[g];
+        rf_shell_job_t *gj[2] = {calloc(n, sizeof(rf_shell_job_t)), calloc(n, sizeof(rf_shell_job_t))};
+        if (!gj[0] || !gj[1]) {
+            free(gj[0]);
+            free(gj[1]);
+            goto out;
+        }
+        long long blocks = 0;
+        int nj = 0;
+        for (size_t q = 0; q < n; q++) {
+            fdtd_sim_t *s = b->sims[q];
+            if (!s->shell.n || !s->shell.start[s->shell.n]) continue;
+            const int tb = jobs[0][q].m.tag ? jobs[0][q].m.tag_bytes : 0;
+            if (tb != sh->tag_bytes || s->has_cpml != sh->has_cpml) continue;
+            for (int ph = 0; ph < 2; ph++) {
+                gj[ph][nj] = jobs[ph][q];
+                gj[ph][nj].block0 = (unsigned)blocks;
+            }
+            blocks += (s->shell.start[s->shell.n] + 255) / 256;
+            nj++;
+        }
+        if (blocks > INT_MAX || nj > 65535) {
+            free(gj[0]);
+            free(gj[1]);
+            rf_set_error("batch: shell launch too large");
+            goto out;
+        }
+        free(bj);
+        bj = malloc((size_t)blocks * sizeof *bj);
+        if (!bj) {
+            free(gj[0]);
+            free(gj[1]);
+            goto out;
+        }
+        for (int q = 0; q < nj; q++) {
+            const long long end = q + 1 < nj ? gj[0][q + 1].block0 : blocks;
+            for (long long k = gj[0][q].block0; k < end; k++) bj[k] = (unsigned short)q;
+        }
+        sh->n_blocks = (int)blocks;
+        int e = h2d_new((void **)&sh->d_jobs[0], gj[0], nj * sizeof(rf_shell_job_t), b->st) ||
+                h2d_new((void **)&sh->d_jobs[1], gj[1], nj * sizeof(rf_shell_job_t), b->st) ||
+                h2d_new((void **)&sh->d_block_job, bj, (size_t)blocks * sizeof *bj, b->st) || rf_sync(b->st);
+        free(gj[0]);
+        free(gj[1]);
+        if (e) goto out;
+    }
+    if (h2d_new((void **)&b->d_probe[0], pj[0], n * sizeof(rf_probe_job_t), b->st) ||
+        h2d_new((void **)&b->d_probe[1], pj[1], n * sizeof(rf_probe_job_t), b->st) ||
+        h2d_new((void **)&b->d_pdft, dj, n * sizeof(rf_pdft_job_t), b->st) || rf_sync(b->st))
+        goto out;
+    b->phase = 0;
+    b->built = 1;
+    rc = 0;
+out:
+    free(tab[0]);
+    free(tab[1]);
+    free(jobs[0]);
+    free(jobs[1]);
+    free(pj[0]);
+    free(pj[1]);
+    free(dj);
+    free(gkey);
+    free(bj);
+    free(base);
+    free(isc);
+    if (rc) batch_free_tables(b);
+    return rc;
+}
+
+/* The launches of one step of every scene at the batch's phase; the
+ * handles' host state advances by one step (buffer swap). */
+static int batch_enqueue_step(fdtd_batch_t *b) {
+    const int ph = b->phase;
+    for (int g = 0; g < b->n_groups; g++) {
+        rf_leap_pack_t pk = b->groups[g].pk;
+        pk.tab = b->groups[g].d_tab[ph];
+        pk.qpar = ph;
+        if (rf_launch_leapfrog_packed(b->prec, &pk, b->st)) return -1;
+    }
+    for (size_t q = 0; q < b->n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        const bscene_t *c = &b->sc[q];
+        if (c->ade && rf_launch_ade_pass(s->prec, &s->geom, c->alo, c->ahi, s->f, s->f2, &s->mat.d, NULL, s->K,
+                                         s->d_step, b->st))
+            return -1;
+    }
+    for (int g = 0; g < b->n_shells; g++) {
+        const bshell_t *sh = &b->shells[g];
+        if (rf_launch_shell_h_packed(b->prec, sh->d_jobs[ph], sh->d_block_job, sh->n_blocks, sh->has_cpml, b->st))
+            return -1;
+    }
+    for (int g = 0; g < b->n_shells; g++) {
+        const bshell_t *sh = &b->shells[g];
+        if (rf_launch_shell_e_packed(b->prec, sh->d_jobs[ph], sh->d_block_job, sh->n_blocks, sh->tag_bytes,
+                                     sh->has_cpml, b->st))
+            return -1;
+    }
+    for (size_t q = 0; q < b->n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        swap_state(s);
+        if (s->has_mat && s->mat.n_tensor &&
+            rf_launch_tensor(s->prec, &s->geom, s->f, s->mat.tensor_edges, s->mat.n_tensor, b->st))
+            return -1;
+        for (size_t p = 0; p < s->n_planes && !s->post_in_kernel; p++)
+            if (s->planes[p].after_e &&
+                rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, b->st))
+                return -1;
+        for (int f = 0; f < s->n_flux; f++) {
+            const size_t nn[3] = {s->nx, s->ny, s->nz};
+            const int a = s->flux[f][0];
+            const size_t cells = nn[(a + 1) % 3] * nn[(a + 2) % 3];
+            const int blocks = (int)((cells + 255) / 256 < (size_t)s->n_partials ? (cells + 255) / 256
+                                                                                  : (size_t)s->n_partials);
+            if (rf_launch_flux(s->prec, &s->geom, a, s->flux[f][1], s->f2, s->f, s->d_aux, blocks, s->d_flux + f,
+                               b->st))
+                return -1;
+        }
+        rf_caps_t cs;
+        memset(&cs, 0, sizeof cs);
+        for (size_t m = 0; m < s->n_mons; m++) {
+            smon_t *mo = &s->mons[m];
+            for (int w = 0; w < 2; w++) {
+                rf_cap_t *c = &cs.c[cs.n++];
+                c->f = s->f[mo->comp[w]];
+                c->buf = mo->ring[w];
+                c->axis = mo->axis;
+                c->index = mo->index;
+                c->na = mo->na;
+                c->nb = mo->nb;
+                if (cs.n == RF_MAX_CAPS || (m + 1 == s->n_mons && w == 1)) {
+                    if (rf_launch_plane_captures(s->prec, &s->geom, &cs, s->K, s->d_step, b->st)) return -1;
+                    cs.n = 0;
+                }
+            }
+        }
+    }
+    if (rf_launch_probe_record_packed(b->prec, b->d_probe[ph], (int)b->n, b->st)) return -1;
+    b->phase ^= 1;
+    return 0;
+}
+
+static int batch_enqueue_flush(fdtd_batch_t *b, int count) {
+    if (count <= 0) return 0;
+    if (rf_launch_probe_dft_packed(b->d_pdft, (int)b->n, b->max_freq, count, b->st)) return -1;
+    for (size_t q = 0; q < b->n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        for (size_t m = 0; m < s->n_mons; m++) {
+            smon_t *mo = &s->mons[m];
+            const long long cells = (long long)mo->na * mo->nb;
+            for (int w = 0; w < 2; w++)
+                if (rf_launch_plane_dft(mo->ring[w], s->prec, cells, s->K, mo->nf, mo->d_w, s->dt, s->d_step, count,
+                                        s->d_tw, mo->acc_re[w], mo->acc_im[w], NULL, NULL, 0, b->st))
+                    return -1;
+        }
+    }
+    return 0;
+}
+
+fdtd_batch_t *fdtd_batch_create(fdtd_sim_t *const *sims, size_t n_sims) {
+    if (!sims || n_sims == 0 || n_sims > 65535) {
+        rf_set_error("fdtd_batch_create: 1 to 65535 handles");
+        return NULL;
+    }
+    for (size_t q = 0; q < n_sims; q++) {
+        if (!sims[q]) {
+            rf_set_error("fdtd_batch_create: NULL handle");
+            return NULL;
+        }
+        for (size_t r = 0; r < q; r++)
+            if (sims[r] == sims[q]) {
+                rf_set_error("fdtd_batch_create: a handle appears twice");
+                return NULL;
+            }
+        if (sims[q]->prec != sims[0]->prec || sims[q]->K != sims[0]->K) {
+            rf_set_error("fdtd_batch_create: the handles differ in precision or dft_block");
+            return NULL;
+        }
+    }
+    fdtd_batch_t *b = calloc(1, sizeof *b);
+    if (!b) return NULL;
+    b->sims = malloc(n_sims * sizeof *b->sims);
+    b->sc = calloc(n_sims, sizeof *b->sc);
+    b->st = rf_stream_create();
+    if (!b->sims || !b->sc || !b->st) {
+        fdtd_batch_destroy(b);
+        return NULL;
+    }
+    memcpy(b->sims, sims, n_sims * sizeof *b->sims);
+    b->n = n_sims;
+    b->prec = sims[0]->prec;
+    b->K = sims[0]->K;
+    return b;
+}
+
+/* The scenes must be ready for packed steps: set up, on the fused step
+ * (K is even there), not profiling, on a single stream.  -2 when a scene
+ * cannot be packed, before any step is enqueued. */
+static int batch_prepare(fdtd_batch_t *b, size_t n_steps) {
+    for (size_t q = 0; q < b->n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        if (run_prepare(s, n_steps)) return -1;
+        if (!s->fused || (s->K & 1) || s->profiling) {
+            rf_set_error("fdtd_batch_run: scene %zu is not on the fused step or is profiling", q);
+            return -2;
+        }
+    }
+    for (size_t q = 0; q < b->n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        if (rf_sync(s->st)) return -1;
+        if (s->pending && (enqueue_flush(s, s->pending) || rf_sync(s->st))) return -1;
+        s->pending = 0;
+        if (s->shell_overlap) {
+            s->shell_overlap = 0;
+            for (int p = 0; p < 2; p++) {
+                rf_graph_destroy(s->graphs[p]);
+                s->graphs[p] = NULL;
+            }
+        }
+    }
+    /* tables built from other buffers or wave tables, or a handle stepped
+     * on its own since: build them again */
+    int stale = !b->built;
+    for (size_t q = 0; q < b->n && !stale; q++) {
+        const fdtd_sim_t *s = b->sims[q];
+        const bscene_t *c = &b->sc[q];
+        stale = s->wave_gen != c->wave_gen || s->parity != (c->parity ^ b->phase) || s->steps != c->expect;
+    }
+    if (stale && batch_build(b)) return b->built ? -1 : -2;
+    return 0;
+}
+
+static int batch_run(fdtd_batch_t *b, size_t n_steps) {
+    const int pr = batch_prepare(b, n_steps);
+    if (pr) return pr;
+    size_t left = n_steps;
+    int pending = 0;
+    while (left) {
+        if (pending == 0 && left >= (size_t)b->K) {
+            rf_graph_t **g = &b->graphs[b->phase];
+            if (!*g) {
+                const int ph = b->phase;
+                if (rf_graph_begin(b->st)) return -1;
+                int rc = 0;
+                for (int q = 0; q < b->K && !rc; q++) rc = batch_enqueue_step(b);
+                if (!rc) rc = batch_enqueue_flush(b, b->K);
+                const int rc2 = rf_graph_end(b->st, g);
+                /* capture ran no kernel: K (even) swaps left every handle's
+                 * state where it was */
+                b->phase = ph;
+                if (rc || rc2) {
+                    rf_graph_destroy(*g);
+                    *g = NULL;
+                    rf_set_error("batch graph capture: %s", rf_error_string());
+                    return -1;
+                }
+            }
+            if (rf_graph_launch(*g, b->st)) return -1;
+            for (size_t q = 0; q < b->n; q++) b->sims[q]->steps += (size_t)b->K;
+            left -= (size_t)b->K;
+            continue;
+        }
+        if (batch_enqueue_step(b)) return -1;
+        for (size_t q = 0; q < b->n; q++) b->sims[q]->steps++;
+        left--;
+        if (++pending == b->K) {
+            if (batch_enqueue_flush(b, b->K)) return -1;
+            pending = 0;
+        }
+    }
+    if (pending && batch_enqueue_flush(b, pending)) return -1;
+    if (rf_sync(b->st)) {
+        rf_set_error("fdtd_batch_run: %s", rf_error_string());
+        return -1;
+    }
+    for (size_t q = 0; q < b->n; q++) b->sc[q].expect = b->sims[q]->steps;
+    return 0;
+}
+
+int fdtd_batch_run(fdtd_batch_t *b, size_t n_steps) {
+    if (!b) return -1;
+    return batch_run(b, n_steps) ? -1 : 0;
+}
+
+size_t fdtd_batch_size(const fdtd_batch_t *b) { return b ? b->n : 0; }
+
+int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell) {
+    if (!b || !b->built) return -1;
+    if (leapfrog) *leapfrog = b->n_groups;
+    if (shell) *shell = 2 * b->n_shells;
+    return 0;
 }
 
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }

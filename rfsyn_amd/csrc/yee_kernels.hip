@@ -317,6 +317,23 @@ k_probe_record(const T *ex, const T *ey, const T *ez, const T *hx, const T *hy, 
     if (threadIdx.x == 0) *d_step = step + 1;
 }
 
+// Probe gathers of many scenes: block b records the probes of job b and
+// advances its step counter (k_probe_record per job).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_probe_record_packed(const rf_probe_job_t *__restrict__ jobs) {
+    const rf_probe_job_t &jb = jobs[blockIdx.x];
+    const long long step = *jb.d_step;
+    const T *const *F = reinterpret_cast<const T *const *>(jb.f);
+    for (int p = threadIdx.x; p < jb.n; p += blockDim.x) {
+        const double v = static_cast<double>(F[jb.comp[p]][jb.idx[p]]);
+        jb.ring[(step % jb.ring_len) * jb.n + p] = v;
+        if (step < jb.td_cap) jb.td[(long long)p * jb.td_cap + step] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *jb.d_step = step + 1;
+}
+
 __device__ __forceinline__ void neumaier(double &s, double &c, double x) {
     const double t = s + x;
     if (fabs(s) >= fabs(x)) c += (s - t) + x;
@@ -357,6 +374,42 @@ k_probe_dft(const double *ring, int ring_len, int n_probes, const int *freq_prob
     if (comp_sum) {
         cmp_re[t] = cr;
         cmp_im[t] = ci;
+    }
+}
+
+// Point-probe DFTs of many scenes: blocks (x, y) take the pairs
+// 256 x .. 256 x + 255 of job y, each summed as k_probe_dft sums it.
+__global__ void __launch_bounds__(256)
+k_probe_dft_packed(const rf_pdft_job_t *__restrict__ jobs, int count) {
+    const rf_pdft_job_t &jb = jobs[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= jb.n_freq) return;
+    const int p = jb.freq_probe[t];
+    const double w = jb.freq_w[t];
+    const double dt = jb.dt;
+    const long long end = *jb.d_step;
+    double re = jb.acc_re[t], im = jb.acc_im[t];
+    double cr = jb.comp_sum ? jb.cmp_re[t] : 0.0, ci = jb.comp_sum ? jb.cmp_im[t] : 0.0;
+    for (long long n = end - count; n < end; n++) {
+        const double v = jb.ring[(n % jb.ring_len) * jb.n_probes + p];
+        const double tt = static_cast<double>(n + 1) * dt;
+        double sn, cs;
+        sincos(w * tt, &sn, &cs);
+        const double tr = v * cs * dt;
+        const double ti = v * sn * dt;
+        if (jb.comp_sum) {
+            neumaier(re, cr, tr);
+            neumaier(im, ci, -ti);
+        } else {
+            re += tr;
+            im -= ti;
+        }
+    }
+    jb.acc_re[t] = re;
+    jb.acc_im[t] = im;
+    if (jb.comp_sum) {
+        jb.cmp_re[t] = cr;
+        jb.cmp_im[t] = ci;
     }
 }
 
@@ -773,6 +826,23 @@ int rf_launch_probe_dft(const double *ring, int ring_len, int n_probes, const in
     hipLaunchKernelGGL(k_probe_dft, dim3((n_freq + 255) / 256), dim3(256), 0, hs(s), ring, ring_len,
                        n_probes, freq_probe, n_freq, freq_w, dt, d_step, count, acc_re, acc_im,
                        cmp_re, cmp_im, comp_sum);
+    return launched();
+}
+
+int rf_launch_probe_record_packed(int prec, const rf_probe_job_t *jobs, int n_jobs, rf_stream_t *s) {
+    if (n_jobs <= 0) return 0;
+    if (!jobs) return -1;
+    if (prec == 0)
+        hipLaunchKernelGGL(k_probe_record_packed<double>, dim3(n_jobs), dim3(256), 0, hs(s), jobs);
+    else
+        hipLaunchKernelGGL(k_probe_record_packed<float>, dim3(n_jobs), dim3(256), 0, hs(s), jobs);
+    return launched();
+}
+
+int rf_launch_probe_dft_packed(const rf_pdft_job_t *jobs, int n_jobs, int max_freq, int count, rf_stream_t *s) {
+    if (n_jobs <= 0 || max_freq <= 0 || count <= 0) return 0;
+    if (!jobs || n_jobs > 65535) return -1;
+    hipLaunchKernelGGL(k_probe_dft_packed, dim3((max_freq + 255) / 256, n_jobs), dim3(256), 0, hs(s), jobs, count);
     return launched();
 }
 

@@ -460,6 +460,31 @@ def run_many(lib, runs, steps):
     assert lib.fdtd_sim_run_many(arr, len(runs), steps) == 0, lib.fdtd_last_error()
 
 
+class Batch:
+    """fdtd_batch_t over NativeRun handles (packed launches); close it
+    before the handles."""
+
+    def __init__(self, lib, runs):
+        self.lib = lib
+        arr = (C.c_void_p * len(runs))(*[r.sim for r in runs])
+        self.b = lib.fdtd_batch_create(arr, len(runs))
+        assert self.b, lib.fdtd_last_error()
+
+    def run(self, steps):
+        assert self.lib.fdtd_batch_run(self.b, steps) == 0, self.lib.fdtd_last_error()
+
+    def launches(self):
+        """(packed leapfrog launches, shell launches) per step."""
+        lf, sh = C.c_int(), C.c_int()
+        assert self.lib.fdtd_batch_launches(self.b, C.byref(lf), C.byref(sh)) == 0
+        return lf.value, sh.value
+
+    def close(self):
+        if self.b:
+            self.lib.fdtd_batch_destroy(self.b)
+            self.b = None
+
+
 class Surrogate:
     """ctypes wrapper of an fdtd_sur_t (include/fdtd_surrogate.h)."""
 

@@ -1,10 +1,13 @@
-"""Batch of independent scenes (fdtd_sim_run_many): every handle ends in
-the state its own fdtd_sim_run would give it, bit for bit."""
+"""Batches of independent scenes (fdtd_batch_*, fdtd_sim_run_many): every
+handle ends in the state its own fdtd_sim_run would give it, bit for bit,
+and scenes of the 96-scene workload match the oracle."""
+import os
+
 import numpy as np
 import pytest
 
 from rfsyn_amd import capi
-from rfsyn_amd.driver import build_c4_scene, c4_scene, run_many, same_values
+from rfsyn_amd.driver import Batch, build_c3, build_c4_scene, c4_scene, run_many, same_values
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +88,104 @@ def test_c4_scenes_fp64_vs_oracle(gpu, oracle):
         for k in f_o:
             assert same_values(f[k], f_o[k]), k
         assert np.max(np.abs(dft - dft_o)) <= 1e-12 * np.max(np.abs(dft_o))
+
+
+C3_SMALL = (40, 36, 256)
+
+
+def _c3_small(lib, precision, td_cap):
+    """The dispersive periodic stack at 40 x 36 x 256 (split ADE pass, an
+    Ex plane source, two z-plane monitors at 128 frequencies, point
+    probes)."""
+    run, pids, mons, _ = build_c3(lib, precision, td_cap, n=C3_SMALL, npml=16)
+    return run, pids, mons
+
+
+def _mixed(lib, precision, td_cap):
+    """Three c4 scenes (CPML on every face, lossy boxes, a dipole) and the
+    small dispersive stack: (run, probes, monitors) each."""
+    out = [build_c4_scene(lib, sp, precision, td_cap, npml=6) + ([],) for sp in _specs(3)]
+    out.append(_c3_small(lib, precision, td_cap))
+    return out
+
+
+def _all_outputs(run, pids, mons, steps):
+    td, dft, f = _outputs(run, pids, steps)
+    planes = [run.plane_dft(m, w, (128, C3_SMALL[0], C3_SMALL[1])) for m in mons for w in range(2)]
+    return td, dft, f, planes
+
+
+@pytest.mark.parametrize("precision", [capi.FDTD_FP32, capi.FDTD_FP64])
+def test_packed_batch_equals_separate_runs(gpu, precision):
+    """A packed batch of uneven scenes with different features, run as 64
+    steps (two captured blocks), then 10 steps of one handle on its own,
+    then 86 more steps (graph blocks and single steps): every handle
+    bitwise equal to the same steps run alone."""
+    counts = [160, 150, 150, 150]
+    alone = []
+    for q, (run, pids, mons) in enumerate(_mixed(gpu, precision, 200)):
+        run.run(counts[q])
+        alone.append(_all_outputs(run, pids, mons, counts[q]))
+        run.close()
+    scenes = _mixed(gpu, precision, 200)
+    b = Batch(gpu, [r for r, _, _ in scenes])
+    b.run(64)
+    lf, sh = b.launches()
+    assert lf == 1 and sh == 2
+    scenes[0][0].run(10)
+    b.run(86)
+    b.close()
+    for q, ((run, pids, mons), ref) in enumerate(zip(scenes, alone)):
+        got = _all_outputs(run, pids, mons, counts[q])
+        assert all(same_values(a, c) for a, c in zip(ref[0], got[0])), q
+        assert same_values(ref[1], got[1]), q
+        for k in ref[2]:
+            assert same_values(ref[2][k], got[2][k]), (q, k)
+        for a, c in zip(ref[3], got[3]):
+            assert same_values(a, c), q
+        run.close()
+
+
+def test_packed_batch_groups_tiling_variants(gpu):
+    """Scenes planned with different rows per tile take one packed
+    leapfrog launch per variant, with the same bits as alone."""
+    steps = 70
+    specs = _specs(3)
+    alone = []
+    rows = [4, 8, 4]
+    for sp, r in zip(specs, rows):
+        os.environ["RFSYN_LEAP_ROWS"] = str(r)
+        run, pids = build_c4_scene(gpu, sp, capi.FDTD_FP32, steps, npml=6)
+        run.run(steps)
+        alone.append(_outputs(run, pids, steps))
+        run.close()
+    runs = []
+    for sp, r in zip(specs, rows):
+        os.environ["RFSYN_LEAP_ROWS"] = str(r)
+        run, pids = build_c4_scene(gpu, sp, capi.FDTD_FP32, steps, npml=6)
+        run.run(0)   # set-up reads the tiling request
+        runs.append((run, pids))
+    os.environ.pop("RFSYN_LEAP_ROWS", None)
+    b = Batch(gpu, [r for r, _ in runs])
+    b.run(steps)
+    assert b.launches()[0] == 2
+    b.close()
+    for (run, pids), (td, dft, f) in zip(runs, alone):
+        td2, dft2, f2 = _outputs(run, pids, steps)
+        assert all(same_values(a, c) for a, c in zip(td, td2))
+        assert same_values(dft, dft2)
+        for k in f:
+            assert same_values(f[k], f2[k]), k
+        run.close()
+
+
+def test_batch_rejects_mixed_precision(gpu):
+    import ctypes as C
+    a, _ = build_c4_scene(gpu, _specs(1)[0], capi.FDTD_FP32, 10, npml=6)
+    c, _ = build_c4_scene(gpu, _specs(1)[0], capi.FDTD_FP64, 10, npml=6)
+    arr = (C.c_void_p * 2)(a.sim, c.sim)
+    assert not gpu.fdtd_batch_create(arr, 2)
+    # run_many falls back to the streams of the handles
+    assert gpu.fdtd_sim_run_many(arr, 2, 5) == 0
+    a.close()
+    c.close()
