@@ -46,6 +46,7 @@ using namespace rfk;
 
 constexpr int WL = 64;       // lanes per wavefront
 constexpr int KW = WL - 2;   // columns owned per wavefront
+static_assert(KW == RF_LEAP_COLS, "the host plans z tiles of RF_LEAP_COLS columns");
 constexpr int WPB = 4;       // wavefronts per block (independent mode)
 constexpr int WPB_MAX = 4;   // wavefronts per block (cooperative mode)
 constexpr int N_XCD = 8;     // blocks are dealt round-robin to the XCDs
@@ -1018,17 +1019,35 @@ k_shell_h_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *
                           static_cast<T>(jb.dx), jb.cp, jb.zs);
 }
 
+// The first SHELL_LDS_ENTRIES entries of the material table, staged in LDS
+// by the block (every thread takes part, before any returns); returns how
+// many the block holds.
+constexpr int SHELL_LDS_ENTRIES = 64;
+template <typename T, typename TagT>
+__device__ __forceinline__ int shell_stage_entries(const rf_mat_t &m, typename EntryOf<T>::type *s_ent) {
+    if constexpr (std::is_same<TagT, NoTag>::value) {
+        return 0;
+    } else {
+        using Entry = typename EntryOf<T>::type;
+        const int n = m.n_entries < SHELL_LDS_ENTRIES ? m.n_entries : SHELL_LDS_ENTRIES;
+        const Entry *ents = static_cast<const Entry *>(m.entries);
+        for (int e = threadIdx.x; e < n; e += blockDim.x) s_ent[e] = ents[e];
+        __syncthreads();
+        return n;
+    }
+}
+
 // E update of one component on a shell node (e_component of the
 // two-kernel path, returning the value).
 template <typename T, typename TagT>
 __device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_mat_t &m, unsigned tag, int i,
-                                           int j, int k) {
+                                           int j, int k, const typename EntryOf<T>::type *s_ent, int n_lds) {
     using Entry = typename EntryOf<T>::type;
     using Ade = typename EntryOf<T>::ade;
     if constexpr (std::is_same<TagT, NoTag>::value) {
         return T(1) * e_old + static_cast<T>(m.vac_ca) * (dA - dB);
     } else {
-        const Entry &en = static_cast<const Entry *>(m.entries)[tag];
+        const Entry &en = static_cast<int>(tag) < n_lds ? s_ent[tag] : static_cast<const Entry *>(m.entries)[tag];
         const uint32_t fl = en.flags[c];
         if (fl & RF_EDGE_TENSOR) return e_old;
         if (fl & RF_EDGE_ADE) {
@@ -1048,7 +1067,8 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
                                              const T *__restrict__ hy, const T *__restrict__ hz,
                                              T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx,
                                              const rf_cpml_t &cp, const rf_zpsi_t &zs, const rf_mat_t &m,
-                                             const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
+                                             const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
+                                             const typename EntryOf<T>::type *s_ent, int n_lds) {
     const Geo g = geo_of(gg);
     int i, j, k;
     if (!shell_node(sh, t, i, j, k)) return;
@@ -1081,7 +1101,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
             else
                 dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
         }
-        e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k);
+        e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k, s_ent, n_lds);
     }
     if (do_y) {   // materials.c:1015-1040
         T dA = hx[idx] - hx[idx + dk];
@@ -1093,7 +1113,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
                 dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
             dB = stretch<T>(dB, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[1][1]), i, j, k, dx);
         }
-        e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k);
+        e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k, s_ent, n_lds);
     }
     if (do_z) {   // materials.c:1050-1058
         T dA = hy[idx] - hy[idx + di];
@@ -1102,7 +1122,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
             dA = stretch<T>(dA, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[2][0]), i, j, k, dx);
             dB = stretch<T>(dB, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[2][1]), i, j, k, dx);
         }
-        e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k);
+        e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k, s_ent, n_lds);
     }
     if (n_srcs) {
         e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 1);
@@ -1120,8 +1140,10 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
           const T *__restrict__ eza, const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
           T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_zpsi_t zs,
           rf_mat_t m, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
+    __shared__ typename EntryOf<T>::type s_ent[SHELL_LDS_ENTRIES];
+    const int n_lds = shell_stage_entries<T, TagT>(m, s_ent);
     shell_e_node<T, TagT, CPML>(gg, sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, exa, eya, eza, hx, hy, hz,
-                                exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step);
+                                exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step, s_ent, n_lds);
 }
 
 // The shell E updates of many scenes (block b: job block_job[b]).
@@ -1132,9 +1154,11 @@ k_shell_e_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *
     const T *const *fa = reinterpret_cast<const T *const *>(jb.fa);
     const T *const *hb = reinterpret_cast<const T *const *>(jb.hb);
     T *const *eb = reinterpret_cast<T *const *>(jb.eb);
+    __shared__ typename EntryOf<T>::type s_ent[SHELL_LDS_ENTRIES];
+    const int n_lds = shell_stage_entries<T, TagT>(jb.m, s_ent);
     shell_e_node<T, TagT, CPML>(jb.g, jb.sh, (long long)(blockIdx.x - jb.block0) * blockDim.x + threadIdx.x, fa[0],
                                 fa[1], fa[2], hb[0], hb[1], hb[2], eb[0], eb[1], eb[2], static_cast<T>(jb.dx), jb.cp,
-                                jb.zs, jb.m, jb.srcs, jb.n_srcs, jb.d_step);
+                                jb.zs, jb.m, jb.srcs, jb.n_srcs, jb.d_step, s_ent, n_lds);
 }
 
 template <typename T>

@@ -191,6 +191,10 @@ typedef struct {
     void *J, *P;
 } rf_leap_cold_t;
 
+/* z columns a wavefront of the fused step owns (its 64 lanes minus the
+ * two halo lanes). */
+#define RF_LEAP_COLS 62
+
 /* Fused leapfrog step: one kernel computes H^{n+1/2} and then E^{n+1}
  * from the "A" buffers (E^n, H^{n-1/2}) into the "B" buffers for the nodes
  * of the box [ib0, ib1), which must lie inside the CPML plain ranges with

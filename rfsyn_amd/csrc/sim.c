@@ -442,8 +442,15 @@ static int plan_regions(fdtd_sim_t *s) {
     memset(sh, 0, sizeof *sh);
     s->kbrk = s->ib1[2];
     if (s->zc) {
+        /* z tiles restart at the upper z slab, so the tiles below it have
+         * no CPML lane, unless that takes more tiles than a plain split of
+         * the z range (short z ranges) */
         int hz = s->cp.d.hi[2];
         s->kbrk = hz < s->ib0[2] ? s->ib0[2] : (hz > s->ib1[2] ? s->ib1[2] : hz);
+        const int kw = RF_LEAP_COLS;
+        const int plain = (s->ib1[2] - s->ib0[2] + kw - 1) / kw;
+        const int split = (s->kbrk - s->ib0[2] + kw - 1) / kw + (s->ib1[2] - s->kbrk + kw - 1) / kw;
+        if (split > plain) s->kbrk = s->ib1[2];
     }
     const int *b0 = s->ib0, *b1 = s->ib1;
     int box[6][2][3] = {
@@ -1375,6 +1382,10 @@ struct fdtd_batch {
     int built;
     int phase;                  /* steps through the batch since the tables were built, mod 2 */
     rf_graph_t *graphs[2];      /* captured block per phase */
+    /* the shells' H update on a second stream beside the leapfrog launches
+     * (it reads state n only and writes nodes outside the fused boxes) */
+    rf_stream_t *st2;
+    rf_event_t *fork, *join;
 };
 
 static void batch_drop_graphs(fdtd_batch_t *b) {
@@ -1420,6 +1431,12 @@ void fdtd_batch_destroy(fdtd_batch_t *b) {
     batch_free_tables(b);
     free(b->sc);
     free(b->sims);
+    if (b->st2) {
+        rf_sync(b->st2);
+        rf_stream_destroy(b->st2);
+    }
+    if (b->fork) rf_event_destroy(b->fork);
+    if (b->join) rf_event_destroy(b->join);
     rf_stream_destroy(b->st);
     free(b);
 }
@@ -1711,6 +1728,17 @@ out:
  * handles' host state advances by one step (buffer swap). */
 static int batch_enqueue_step(fdtd_batch_t *b) {
     const int ph = b->phase;
+    const int overlap = b->st2 && b->n_shells;
+    if (overlap) {
+        if (rf_event_record(b->fork, b->st) || rf_stream_wait(b->st2, b->fork)) return -1;
+        for (int g = 0; g < b->n_shells; g++) {
+            const bshell_t *sh = &b->shells[g];
+            if (rf_launch_shell_h_packed(b->prec, sh->d_jobs[ph], sh->d_block_job, sh->n_blocks, sh->has_cpml,
+                                         b->st2))
+                return -1;
+        }
+        if (rf_event_record(b->join, b->st2)) return -1;
+    }
     for (int g = 0; g < b->n_groups; g++) {
         rf_leap_pack_t pk = b->groups[g].pk;
         pk.tab = b->groups[g].d_tab[ph];
@@ -1724,11 +1752,12 @@ static int batch_enqueue_step(fdtd_batch_t *b) {
                                          s->d_step, b->st))
             return -1;
     }
-    for (int g = 0; g < b->n_shells; g++) {
+    for (int g = 0; g < b->n_shells && !overlap; g++) {
         const bshell_t *sh = &b->shells[g];
         if (rf_launch_shell_h_packed(b->prec, sh->d_jobs[ph], sh->d_block_job, sh->n_blocks, sh->has_cpml, b->st))
             return -1;
     }
+    if (overlap && rf_stream_wait(b->st, b->join)) return -1;
     for (int g = 0; g < b->n_shells; g++) {
         const bshell_t *sh = &b->shells[g];
         if (rf_launch_shell_e_packed(b->prec, sh->d_jobs[ph], sh->d_block_job, sh->n_blocks, sh->tag_bytes,
@@ -1826,6 +1855,16 @@ fdtd_batch_t *fdtd_batch_create(fdtd_sim_t *const *sims, size_t n_sims) {
         return NULL;
     }
     memcpy(b->sims, sims, n_sims * sizeof *b->sims);
+    const char *e = getenv("RFSYN_SHELL_OVERLAP");
+    if (!(e && atoi(e) == 0)) {
+        b->st2 = rf_stream_create();
+        b->fork = rf_event_create_sync();
+        b->join = rf_event_create_sync();
+        if (!b->st2 || !b->fork || !b->join) {
+            fdtd_batch_destroy(b);
+            return NULL;
+        }
+    }
     b->n = n_sims;
     b->prec = sims[0]->prec;
     b->K = sims[0]->K;
