@@ -187,6 +187,7 @@ class C3:
     n = (512, 512, 256)
     npml = (0, 0, 16)
     tag_bytes = 1
+    mon_freqs = 128
 
     def __init__(self):
         from rfsyn_amd.driver import c3_layers
@@ -393,6 +394,17 @@ def bench_c4(args, lib):
     print(json.dumps(out))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(wl, budget_s=15.0):
     ref_so = os.path.join(REPO, "oracle", "_ref", "libref_fdtd.so")
     port_so = os.path.join(REPO, "oracle", "liboracle_fdtd.so")
@@ -415,6 +427,7 @@ def cpu_baseline(wl, budget_s=15.0):
     if path == port_so:
         kind = "port"
     return {"value": round(cells * steps / t / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": kind,
+            "cpu_model": cpu_model(),
             "sample": f"{wl.config['workload'].split(':')[0]} scene, {steps} steps on 1 pinned core "
                       f"(host has {os.cpu_count()} cpus)"}
 
@@ -515,8 +528,12 @@ def main():
         if total > timed_steps:
             run.run(total - timed_steps)
         dft = np.stack([run.dft(p) for p in pids])
+        # the monitors' DFT of the source's component (Ex, the first tangential one)
+        planes = [run.plane_dft(m, 0, (wl.mon_freqs, wl.n[0], wl.n[1])) for m in mons]
+        block = lib.fdtd_sim_dft_block(run.sim)
         run.close()
         res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, sh_ms=sh_ms / nk, dft=dft, fused=fused, box=box,
+                         planes=planes, block=block,
                          tiling=None if args.no_autotune or not fused else
                          {"rows": chosen[0], "planes_per_item": chosen[1], "z_split": chosen[2], "coop": chosen[3]})
 
@@ -549,11 +566,39 @@ def main():
                   "e_bytes_per_launch": eb, "e_avg_launch_us": round(r["e_ms"] * 1e3, 3)}
     achieved = byt / (dur * 1e-3) / 1e9
     full = step_bytes(cells, slab_cells(n, wl.npml), s, wl.tag_bytes, drude, lorentz)
+    mon = None
+    if r["planes"]:
+        # per plane per step: 2 tangential components read, buffered and read
+        # back (6 s per cell); per flush of K steps their FP64 complex
+        # accumulators read and written (64 F per cell) over K steps
+        n_plane = n[0] * n[1]
+        mon_b = len(r["planes"]) * (6 * s * n_plane + 64 * wl.mon_freqs * n_plane // r["block"])
+        full += mon_b
+        mon = {"planes": len(r["planes"]), "freqs": wl.mon_freqs, "dft_block": r["block"], "bytes_per_step": mon_b}
     step_gbs = full / (r["wall"] / args.steps) / 1e9
     d64, d32 = res[capi.FDTD_FP64]["dft"], res[capi.FDTD_FP32]["dft"]
     peak = np.max(np.abs(d64), axis=1, keepdims=True)
     # a short run whose probes have not seen the pulse yet has no error to report
     err = float(np.max(np.abs(d32 - d64) / peak)) if np.all(peak > 0) else None
+    err_detail = None
+    if err is not None:
+        e = np.abs(d32 - d64) / peak   # [probe, frequency], normalised by the probe's in-band peak
+        sig = lambda a: [float(f"{x:.3g}") for x in a]
+        err_detail = {"normalisation": "|X32 - X64| / max_f |X64(probe, f)|",
+                      "per_probe_max": sig(e.max(axis=1)), "per_freq_max_over_probes": sig(e.max(axis=0))}
+        if r["planes"]:
+            mon_err = []
+            for p64, p32 in zip(res[capi.FDTD_FP64]["planes"], res[capi.FDTD_FP32]["planes"]):
+                num = np.sqrt(np.sum(np.abs(p32 - p64) ** 2, axis=(1, 2)))
+                den = np.sqrt(np.sum(np.abs(p64) ** 2, axis=(1, 2)))
+                rel = num / np.where(den > 0, den, 1.0)
+                mon_err.append({"peak_normalised_max": float(f"{(num / den.max()).max():.3g}"),
+                                "max": float(f"{rel.max():.3g}"), "per_freq": sig(rel)})
+            err_detail["plane_monitors_rel_l2_per_freq"] = {
+                "normalisation": "||X32 - X64||_2 / ||X64||_2 over the plane at each frequency, Ex "
+                                 "(peak_normalised_max: divided by the largest ||X64||_2 over frequency instead, the "
+                                 "probes' convention; the per-frequency ratio has no bound at a spectral null)",
+                "R": mon_err[0], "T": mon_err[1] if len(mon_err) > 1 else None}
     other = res[capi.FDTD_FP32 if wl.headline == capi.FDTD_FP64 else capi.FDTD_FP64]
     out = {
         "metric": METRIC,
@@ -575,10 +620,11 @@ def main():
                      if r["fused"] else None,
                      "bytes_per_launch": byt, "avg_launch_us": round(dur * 1e3, 3), **detail},
         "step": {"bytes_per_step": full, "achieved_gbs": round(step_gbs, 2),
-                 "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4), "monitors": mon},
         "other_precision": {"dtype": "f32" if wl.headline == capi.FDTD_FP64 else "f64",
                             "value": round(cells * args.steps / other["wall"] / 1e6, 3)},
         "fp32_vs_fp64_probe_dft_err": err,
+        "fp32_vs_fp64_err_detail": err_detail,
         "tiling": r["tiling"],
         "step_path": "fused" if r["fused"] else "two-kernel",
         "accuracy_steps": total,

@@ -43,7 +43,9 @@ typedef struct {
                               DFT sums, 0 = plain sums in the reference
                               order; plane monitors always use plain
                               double sums                               */
-    size_t dft_block;      /* steps buffered between DFT flushes (0: 32) */
+    size_t dft_block;      /* steps buffered between DFT flushes, at most
+                              128 (0: 32, or 64 when the handle has plane
+                              monitors); the results do not depend on it */
 } fdtd_sim_options_t;
 
 fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml,
@@ -144,6 +146,8 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]);
  * receives 1 when the two-kernel step was kept.  0 on success. */
 int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
+/* Steps per DFT block the handle runs with (see dft_block); -1 for NULL. */
+int fdtd_sim_dft_block(const fdtd_sim_t *s);
 /* Diagnostics: streams the handle's next step uses (1, or 2 when the
  * CPML shell's H update runs on a second stream beside the fused kernel;
  * handles advanced together by fdtd_sim_run_many use one). */

@@ -520,9 +520,10 @@ int rf_launch_paint_classify(const rf_paint_shape_t *sh, int ns, const long long
                              const int n[3], unsigned per, int ss, long long keep, rf_paint_rec_t *recs,
                              rf_stream_t *s);
 
-/* Plane monitor DFT flush over `count` (<= 64) buffered steps ending at
- * step *d_step - 1; accumulators [f][cell]; tw_scratch holds
- * 2 * nf * count doubles. */
+/* Plane monitor DFT flush over `count` (<= RF_PDFT_TWS) buffered steps
+ * ending at step *d_step - 1; accumulators [f][cell]; tw_scratch holds
+ * 2 * nf * RF_PDFT_TWS doubles. */
+#define RF_PDFT_TWS 128
 int rf_launch_plane_dft(const void *buf, int buf_prec, long long cells, int ring_len, int nf,
                         const double *freq_w, double dt, const long long *d_step, int count,
                         double *tw_scratch, double *acc_re, double *acc_im, double *cmp_re,

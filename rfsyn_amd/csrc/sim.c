@@ -190,8 +190,9 @@ fdtd_sim_t *fdtd_sim_create(const fdtd_grid_t *g, const cpml_t *pml, const mater
     s->geom = rf_geom_of(g);
     s->cells = rf_cells(g);
     s->comp_sum = o.dft_compensated ? 1 : 0;
-    s->K = o.dft_block ? (int)o.dft_block : 32;
-    if (s->K > 64) s->K = 64;
+    /* 0: chosen at the first run (dft_block_of) */
+    s->K = o.dft_block ? (int)o.dft_block : 0;
+    if (s->K > RF_PDFT_TWS) s->K = RF_PDFT_TWS;
     s->st = rf_stream_create();
     if (!s->st) goto fail;
     if (dalloc(s, &s->blk, 6 * s->cells * s->es)) goto fail;
@@ -357,10 +358,6 @@ int fdtd_sim_add_plane_monitor(fdtd_sim_t *s, int axis, size_t index, const doub
     if (s->started) { rf_set_error("monitors must be added before the first run"); return -1; }
     size_t n[3] = {s->nx, s->ny, s->nz};
     if (index > n[axis]) { rf_set_error("monitor plane outside the grid"); return -1; }
-    if ((size_t)s->K * n_freqs * 16 > 64 * 1024) {
-        rf_set_error("dft_block * n_freqs too large for the plane DFT twiddle block (<= 4096)");
-        return -1;
-    }
     smon_t *nm = realloc(s->mons, (s->n_mons + 1) * sizeof *nm);
     if (!nm) return -1;
     s->mons = nm;
@@ -482,15 +479,24 @@ static int plan_regions(fdtd_sim_t *s) {
 }
 
 /* Device arrays for probes and monitors, allocated once at the first run. */
+/* The DFT block the handle runs with: the requested dft_block, else 32
+ * steps, or 64 with plane monitors (their accumulators are read and
+ * written once per block); even on the fused step. */
+static int dft_block_of(const fdtd_sim_t *s, int fused) {
+    int k = s->K ? s->K : (s->n_mons ? 64 : 32);
+    if (fused && (k & 1)) k++;   /* a captured block must return to its buffer parity */
+    return k;
+}
+
 static int finalize(fdtd_sim_t *s) {
     s->fused = want_fused(s);
+    s->K = dft_block_of(s, s->fused);
     if (s->fused) {
         /* B starts as a copy of A so nodes no kernel writes agree */
         if (dalloc(s, &s->blk2, 6 * s->cells * s->es) || rf_d2d(s->blk2, s->blk, 6 * s->cells * s->es, s->st))
             return -1;
         for (int q = 0; q < 6; q++) s->f2[q] = (char *)s->blk2 + q * s->cells * s->es;
         if (rf_sync(s->st)) return -1;
-        if (s->K & 1) s->K++;   /* a captured block must return to its buffer parity */
         s->post_in_kernel = !(s->has_mat && s->mat.n_tensor);
         rf_leap_cold_t cold;
         memset(&cold, 0, sizeof cold);
@@ -603,7 +609,7 @@ static int finalize(fdtd_sim_t *s) {
                 return -1;
         }
     }
-    if (max_nf && dalloc(s, (void **)&s->d_tw, (size_t)2 * max_nf * s->K * sizeof(double))) return -1;
+    if (max_nf && dalloc(s, (void **)&s->d_tw, (size_t)2 * max_nf * RF_PDFT_TWS * sizeof(double))) return -1;
     s->started = 1;
     return 0;
 }
@@ -1840,7 +1846,9 @@ fdtd_batch_t *fdtd_batch_create(fdtd_sim_t *const *sims, size_t n_sims) {
                 rf_set_error("fdtd_batch_create: a handle appears twice");
                 return NULL;
             }
-        if (sims[q]->prec != sims[0]->prec || sims[q]->K != sims[0]->K) {
+        if (sims[q]->prec != sims[0]->prec ||
+            (sims[q]->started ? sims[q]->K : dft_block_of(sims[q], 1)) !=
+                (sims[0]->started ? sims[0]->K : dft_block_of(sims[0], 1))) {
             rf_set_error("fdtd_batch_create: the handles differ in precision or dft_block");
             return NULL;
         }
@@ -1878,11 +1886,13 @@ static int batch_prepare(fdtd_batch_t *b, size_t n_steps) {
     for (size_t q = 0; q < b->n; q++) {
         fdtd_sim_t *s = b->sims[q];
         if (run_prepare(s, n_steps)) return -1;
-        if (!s->fused || (s->K & 1) || s->profiling) {
-            rf_set_error("fdtd_batch_run: scene %zu is not on the fused step or is profiling", q);
+        if (!s->fused || (s->K & 1) || s->profiling || s->K != b->sims[0]->K) {
+            rf_set_error("fdtd_batch_run: scene %zu is not on the fused step, is profiling or has another "
+                         "dft_block", q);
             return -2;
         }
     }
+    b->K = b->sims[0]->K;
     for (size_t q = 0; q < b->n; q++) {
         fdtd_sim_t *s = b->sims[q];
         if (rf_sync(s->st)) return -1;
@@ -1970,6 +1980,11 @@ int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell) {
 }
 
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s) { return s ? s->steps : 0; }
+
+int fdtd_sim_dft_block(const fdtd_sim_t *s) {
+    if (!s) return -1;
+    return s->started ? s->K : dft_block_of(s, want_fused(s));
+}
 
 int fdtd_sim_streams(const fdtd_sim_t *s) {
     if (!s) return -1;

@@ -427,55 +427,90 @@ k_plane_captures(rf_geom_t gg, rf_caps_t cs, int ring_len, const long long *d_st
     plane_capture_elem<T>(gg, cs, t, ring_len, *d_step);
 }
 
-// Twiddles for the buffered block, scaled by dt:
-// tw[s][f] = dt (cos, sin)(w_f (n_s+1) dt).
+// Twiddles for the buffered block, scaled by dt, frequency-major:
+// tw[f][s] = dt (cos, sin)(w_f (n_s+1) dt), rows of RF_PDFT_TWS steps.
 __global__ void k_twiddles(int nf, const double *freq_w, double dt, const long long *d_step,
                            int count, double2 *tw) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nf * count) return;
-    const int s = t / nf, f = t % nf;
+    const int s = t % count, f = t / count;
     const long long n = *d_step - count + s;
     const double tt = static_cast<double>(n + 1) * dt;
     double sn, cs;
     sincos(freq_w[f] * tt, &sn, &cs);
-    tw[t] = make_double2(cs * dt, sn * dt);
+    tw[(long long)f * RF_PDFT_TWS + s] = make_double2(cs * dt, sn * dt);
 }
 
-// Time-blocked plane DFT: one thread per plane cell keeps its KC buffered
-// samples in registers and, for every frequency, adds them to the double
-// accumulators [f][cell] in time order (one read and one write of each
-// accumulator per block of steps instead of per step).  Twiddles come from
-// LDS; the samples past `count` contribute nothing.
-template <typename B, int KC>
+// Time-blocked plane DFT as a register-tiled product: a block takes 64
+// cells of the plane and, frequency tile by frequency tile (32 at a time),
+// adds the buffered samples off .. off + cnt - 1 of the block of `count`
+// steps to the FP64 accumulators [f][cell], which are read and written
+// once per pass.  The samples of the block's cells for the pass and the
+// tile's twiddles tw[f][s] = dt (cos, sin)(w_f t_s) are staged in LDS;
+// thread (cg, fg) holds the accumulators of cells cg + 16 i (i < 4) and
+// frequencies fg + 16 j (j < 2), and adds the terms of each output in
+// time order with one fused multiply-add each, so every accumulator sees
+// the same sequence of operations whatever the block length.
+constexpr int PD_CELLS = 64, PD_FT = 32;
+template <typename B, int KP>
 __global__ void __launch_bounds__(256)
-k_plane_dft(const B *buf, long long cells, int ring_len, int nf, const double2 *tw,
-            const long long *d_step, int count, double *acc_re, double *acc_im) {
-    extern __shared__ double2 s_tw[];
-    for (int t = threadIdx.x; t < nf * count; t += blockDim.x) s_tw[t] = tw[t];
-    __syncthreads();
-    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= cells) return;
-    const long long end = *d_step;
-    B v[KC];
-#pragma unroll
-    for (int s = 0; s < KC; s++) {
-        const long long n = end - count + s;
-        v[s] = s < count ? buf[(n % ring_len) * cells + cell] : B(0);
+k_plane_dft(const B *__restrict__ buf, long long cells, int ring_len, int nf, const double2 *__restrict__ tw,
+            const long long *__restrict__ d_step, int count, int off, int cnt, double *__restrict__ acc_re,
+            double *__restrict__ acc_im) {
+    __shared__ B s_x[PD_CELLS][KP + 1];          // samples, padded rows
+    __shared__ double2 s_w[KP][PD_FT];           // twiddles of the tile
+    const int t = threadIdx.x;
+    const long long c0 = (long long)blockIdx.x * PD_CELLS;
+    const long long first = *d_step - count + off;
+    for (int e = t; e < PD_CELLS * KP; e += 256) {
+        const int c = e % PD_CELLS, q = e / PD_CELLS;
+        const long long cell = c0 + c;
+        s_x[c][q] = q < cnt && cell < cells ? buf[((first + q) % ring_len) * cells + cell] : B(0);
     }
-    for (int f = 0; f < nf; f++) {
-        const long long o = (long long)f * cells + cell;
-        double re = acc_re[o], im = acc_im[o];
+    const int cg = t % 16, fg = t / 16;
+    for (int ft = 0; ft < nf; ft += PD_FT) {
+        // the tile's accumulators, in flight while its twiddles are staged
+        double re[4][2], im[4][2];
 #pragma unroll
-        for (int s = 0; s < KC; s++) {
-            if (s < count) {
-                const double2 w = s_tw[s * nf + f];
-                const double x = static_cast<double>(v[s]);
-                re += x * w.x;
-                im -= x * w.y;
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const long long cell = c0 + cg + 16 * i;
+                const int f = ft + fg + 16 * j;
+                const bool ok = cell < cells && f < nf;
+                re[i][j] = ok ? acc_re[(long long)f * cells + cell] : 0.0;
+                im[i][j] = ok ? acc_im[(long long)f * cells + cell] : 0.0;
+            }
+        __syncthreads();   // the previous tile's twiddles are read
+        for (int e = t; e < KP * PD_FT; e += 256) {
+            const int q = e % KP, f = e / KP;
+            s_w[q][f] = q < cnt && ft + f < nf ? tw[(long long)(ft + f) * RF_PDFT_TWS + off + q] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int q = 0; q < cnt; q++) {
+            double x[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) x[i] = static_cast<double>(s_x[cg + 16 * i][q]);
+            const double2 w0 = s_w[q][fg], w1 = s_w[q][fg + 16];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                re[i][0] = __builtin_fma(x[i], w0.x, re[i][0]);
+                im[i][0] = __builtin_fma(-x[i], w0.y, im[i][0]);
+                re[i][1] = __builtin_fma(x[i], w1.x, re[i][1]);
+                im[i][1] = __builtin_fma(-x[i], w1.y, im[i][1]);
             }
         }
-        acc_re[o] = re;
-        acc_im[o] = im;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const long long cell = c0 + cg + 16 * i;
+                const int f = ft + fg + 16 * j;
+                if (cell < cells && f < nf) {
+                    acc_re[(long long)f * cells + cell] = re[i][j];
+                    acc_im[(long long)f * cells + cell] = im[i][j];
+                }
+            }
     }
 }
 
@@ -872,29 +907,27 @@ int rf_launch_plane_dft(const void *buf, int buf_prec, long long cells, int ring
                         double *tw_scratch, double *acc_re, double *acc_im, double *cmp_re,
                         double *cmp_im, int comp_sum, rf_stream_t *s) {
     if (count <= 0) return 0;
-    if (count > 64 || (size_t)nf * count * sizeof(double2) > 64 * 1024) return -1;
+    if (count > RF_PDFT_TWS || nf <= 0) return -1;
     double2 *tw = reinterpret_cast<double2 *>(tw_scratch);
     hipLaunchKernelGGL(k_twiddles, dim3((nf * count + 255) / 256), dim3(256), 0, hs(s), nf, freq_w,
                        dt, d_step, count, tw);
     (void)cmp_re;
     (void)cmp_im;
     if (comp_sum) return -1;   /* plane monitors use plain double sums */
-    const size_t lds = (size_t)nf * count * sizeof(double2);
-    dim3 grd((unsigned)((cells + 255) / 256));
-#define RF_PLANE_DFT(BT, KC)                                                                        \
-    hipLaunchKernelGGL((k_plane_dft<BT, KC>), grd, dim3(256), lds, hs(s), static_cast<const BT *>(buf), \
-                       cells, ring_len, nf, tw, d_step, count, acc_re, acc_im)
-    if (buf_prec == 0) {
-        if (count <= 16) RF_PLANE_DFT(double, 16);
-        else if (count <= 32) RF_PLANE_DFT(double, 32);
-        else RF_PLANE_DFT(double, 64);
-    } else {
-        if (count <= 16) RF_PLANE_DFT(float, 16);
-        else if (count <= 32) RF_PLANE_DFT(float, 32);
-        else RF_PLANE_DFT(float, 64);
+    dim3 grd((unsigned)((cells + PD_CELLS - 1) / PD_CELLS));
+    // samples per pass: 64 (float) or 32 (double) staged in LDS
+    const int pass = buf_prec == 0 ? 32 : 64;
+    for (int off = 0; off < count; off += pass) {
+        const int cnt = count - off < pass ? count - off : pass;
+        if (buf_prec == 0)
+            hipLaunchKernelGGL((k_plane_dft<double, 32>), grd, dim3(256), 0, hs(s), static_cast<const double *>(buf),
+                               cells, ring_len, nf, tw, d_step, count, off, cnt, acc_re, acc_im);
+        else
+            hipLaunchKernelGGL((k_plane_dft<float, 64>), grd, dim3(256), 0, hs(s), static_cast<const float *>(buf),
+                               cells, ring_len, nf, tw, d_step, count, off, cnt, acc_re, acc_im);
+        if (launched()) return -1;
     }
-#undef RF_PLANE_DFT
-    return launched();
+    return 0;
 }
 
 int rf_launch_energy(int prec, const rf_geom_t *g, void *const f[6], const void *tag, int tag_bytes,

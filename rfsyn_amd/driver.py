@@ -327,14 +327,14 @@ def c3_probes(n, layers, count=8, seed=5):
     return out
 
 
-def build_c3(lib, precision, td_cap, n=(512, 512, 256), npml=16, seed=3, monitors=True):
+def build_c3(lib, precision, td_cap, n=(512, 512, 256), npml=16, seed=3, monitors=True, block=0):
     """512x512x256, dx = 0.5 mm, periodic x/y, 16-cell CPML on +-z, Ex soft
     plane source at k = npml + 8 (40 GHz, 30 GHz -20 dB bandwidth), Drude
     grating / 2-pole Lorentz / eps=4 stack, two full-plane monitors at 128
     frequencies over 25-55 GHz, plus 8 Ex point probes with DFTs."""
     layers = c3_layers(seed, n[2], npml)
     run = NativeRun(lib, n, 0.5e-3, 0.99, periodic=3, cpml=capi.CpmlParams(npml, 3.0, 1.0, 1.0, 0.05, 1e-8),
-                    precision=precision, compensated=1,
+                    precision=precision, compensated=1, block=block,
                     material_setup=lambda lb, g, m: c3_materials(lb, g, m, layers))
     run.add_plane_source(capi.FIELD_EX, 2, npml + 8, "source_waveform_modulated_gaussian",
                          modgauss_params(40e9, 30e9))

@@ -96,8 +96,9 @@ C3_SMALL = (40, 36, 256)
 def _c3_small(lib, precision, td_cap):
     """The dispersive periodic stack at 40 x 36 x 256 (split ADE pass, an
     Ex plane source, two z-plane monitors at 128 frequencies, point
-    probes)."""
-    run, pids, mons, _ = build_c3(lib, precision, td_cap, n=C3_SMALL, npml=16)
+    probes), with the c4 scenes' DFT block of 32 steps (a batch holds one
+    block length)."""
+    run, pids, mons, _ = build_c3(lib, precision, td_cap, n=C3_SMALL, npml=16, block=32)
     return run, pids, mons
 
 
