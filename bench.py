@@ -272,6 +272,24 @@ class C2:
 WORKLOADS = {"c1": C1, "c2": C2, "c3": C3, "c4": None, "c5": None}
 
 
+def sur_gemm_flops(n, grid=64, chans=(3, 16, 32, 64, 128), width=512, queries=8, outputs=128, coords=78):
+    """(forward, training step) FLOPs of the surrogate's matrix products
+    for a batch of n scenes, 2 M N K per product as the network issues
+    them (rfsyn_amd/csrc/surrogate.c: forward, backward)."""
+    fwd = bwd = 0
+    S = [grid >> l for l in range(len(chans))]
+    for l in range(len(chans) - 1):
+        rows, K, N = n * S[l + 1] ** 3, 27 * chans[l], chans[l + 1]
+        fwd += 2 * rows * K * N
+        bwd += 2 * K * N * rows + 2 * rows * N + (2 * rows * K * N if l > 0 else 0)
+    F = S[-1] ** 3 * chans[-1]
+    q, W, O = n * queries, width, outputs
+    fwd += 2 * n * F * W + 2 * q * coords * W + 2 * q * W * W + 2 * q * W * O
+    bwd += (2 * W * O * q + 2 * q * O + 2 * q * O * W + 2 * W * W * q + 2 * q * W + 2 * q * W * W
+            + 2 * coords * W * q + 2 * q * W + 2 * F * W * n + 2 * n * F * W)
+    return fwd, fwd + bwd
+
+
 def bench_c5(args, lib):
     """Learned surrogate plus FDTD judge on 64^3 scenes of the periodic
     dispersive stack distribution: label train and test scenes with the
@@ -283,8 +301,9 @@ def bench_c5(args, lib):
     n_tr, n_te = args.c5_train, args.c5_test
     log = lambda m: print(f"[c5] {m}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    mtr, ctr, ltr, fdtd_tr = sp.label_scenes(lib, list(range(n_tr)), steps)
-    mte, cte, lte, fdtd_te = sp.label_scenes(lib, list(range(n_tr, n_tr + n_te)), steps)
+    mtr, ctr, ltr, fdtd_tr = sp.label_scenes(lib, list(range(n_tr)), steps, batch=args.c5_label_batch, log=log)
+    mte, cte, lte, fdtd_te = sp.label_scenes(lib, list(range(n_tr, n_tr + n_te)), steps, batch=args.c5_label_batch,
+                                             log=log)
     label_wall = time.perf_counter() - t0
     log(f"labelled {n_tr + n_te} scenes: {label_wall:.1f} s wall, {fdtd_tr + fdtd_te:.1f} s of FDTD")
     # the plain single-scene solver on the same scenes: one handle at a time
@@ -298,6 +317,8 @@ def bench_c5(args, lib):
     losses = sp.train(net, mtr, ctr, ltr / scale, args.c5_epochs, lr=args.c5_lr, batch=bt,
                       log=log, schedule="cos")
     train_s = time.perf_counter() - t1
+    n_steps_tr = args.c5_epochs * max(1, n_tr // bt)
+    step_flops = sur_gemm_flops(bt)[1]
     # accuracy on the held-out scenes, and the predictor without the network
     pred = sp.predict(net, mte, cte, 1024) * scale
     err = sp.rel_l2(pred, lte)
@@ -319,6 +340,13 @@ def bench_c5(args, lib):
                                f"8 probes x 64-freq transfer functions, FDTD {steps} steps",
                    "epochs": args.c5_epochs, "lr": args.c5_lr, "lr_schedule": "cosine", "batch_train": bt,
                    "batch_infer": 1024},
+        "roofline": {"bound": "mfma", "kernel": "training step (all matrix products, BF16 operands, FP32 accumulation)",
+                     "achieved": round(step_flops * n_steps_tr / train_s / 1e12, 3), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(step_flops * n_steps_tr / train_s / 1e12 / 2500.0, 5),
+                     "traffic": None, "flops_per_step": step_flops,
+                     "ms_per_train_step": round(train_s / n_steps_tr * 1e3, 3),
+                     "note": "wall time of whole Adam steps (products, im2col, GELU, loss, Adam, host uploads); "
+                             "the products' own share is in the rocprof summary"},
         "fdtd_scenes_per_s": round(fdtd_rate, 3),
         "fdtd_single_scene_per_s": round(n_one / fdtd_one, 3),
         "batching_gain": round(fdtd_rate / (n_one / fdtd_one), 3),
@@ -386,7 +414,8 @@ def bench_c4(args, lib):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(byt / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": byt},
-        "packed_launches_per_step": {"leapfrog": launches[0], "shell": launches[1], "probes": 1},
+        "packed_launches_per_step": {"leapfrog": launches[0], "shell": launches[1], "ade_pass": launches[2],
+                                     "probes": 1},
         "streams_per_scene_mcell_s": round(cells * args.steps / wall_streams / 1e6, 3),
         "sequential_8_scenes_mcell_s": round(cells8 * args.steps / one / 1e6, 3),
         "autotuned": not args.no_autotune,
@@ -470,11 +499,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip running to the full step count for the FP32-vs-FP64 comparison")
-    ap.add_argument("--c5-train", type=int, default=4096, help="c5: training scenes")
-    ap.add_argument("--c5-test", type=int, default=1024, help="c5: held-out scenes")
-    ap.add_argument("--c5-epochs", type=int, default=240, help="c5: training epochs")
-    ap.add_argument("--c5-lr", type=float, default=1e-3, help="c5: Adam learning rate (cosine-annealed)")
-    ap.add_argument("--c5-batch", type=int, default=64, help="c5: training batch")
+    ap.add_argument("--c5-train", type=int, default=20000, help="c5: training scenes")
+    ap.add_argument("--c5-test", type=int, default=4096, help="c5: held-out scenes")
+    ap.add_argument("--c5-epochs", type=int, default=150, help="c5: training epochs")
+    ap.add_argument("--c5-lr", type=float, default=3e-4, help="c5: Adam learning rate (cosine-annealed)")
+    ap.add_argument("--c5-batch", type=int, default=256, help="c5: training batch")
+    ap.add_argument("--c5-label-batch", type=int, default=256, help="c5: scenes per packed FDTD batch")
     ap.add_argument("--c5-fdtd-steps", type=int, default=1024, help="c5: FDTD steps per label")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiling of the fused step instead of timing the candidates first")

@@ -102,9 +102,10 @@ int fdtd_sim_run_many(fdtd_sim_t *const *sims, size_t n_sims, size_t n_steps);
  * one scene's buffers, CPML, materials and sources.  The CPML shells of
  * all scenes take one launch for H and one for E, all probe gathers one,
  * and per block of dft_block steps all point-probe DFTs one.  A block of
- * steps of the whole batch is one captured graph.  A scene's other
- * launches (split ADE pass, tensor edges, post-E plane sources, flux
- * planes, plane monitors) keep their place in its step.  Every handle
+ * steps of the whole batch is one captured graph.  The split ADE passes
+ * of all scenes take one launch per (tag width, pole count) variant.  A
+ * scene's other launches (tensor edges, post-E plane sources, flux planes,
+ * plane monitors) keep their place in its step.  Every handle
  * ends in the state fdtd_sim_run would give it, bit for bit.
  * fdtd_batch_create: 1..65535 distinct handles with equal precision and
  * dft_block (NULL otherwise); the handles must outlive the batch.
@@ -117,9 +118,9 @@ int fdtd_batch_run(fdtd_batch_t *b, size_t n_steps);
 void fdtd_batch_destroy(fdtd_batch_t *b);
 size_t fdtd_batch_size(const fdtd_batch_t *b);
 /* Diagnostics after a run: packed leapfrog launches per step (one per
- * tiling variant) and shell launches per step.  0, or -1 before the
- * first run. */
-int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell);
+ * tiling variant), shell launches and split-ADE-pass launches per step
+ * (any pointer may be NULL).  0, or -1 before the first run. */
+int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell, int *ade);
 
 /* Tiling autotuner (extension): before the first step, times the fused
  * step's kernel for each candidate tiling (rows per tile, planes per work

@@ -237,7 +237,7 @@ EXTENSION_API = {
     "fdtd_batch_run": (C.c_int, [_vp, _sz]),
     "fdtd_batch_destroy": (None, [_vp]),
     "fdtd_batch_size": (_sz, [_vp]),
-    "fdtd_batch_launches": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fdtd_batch_launches": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fdtd_sim_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "fdtd_sim_select_path": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "fdtd_sur_create": (_vp, [C.POINTER(SurConfig), C.c_uint64]),

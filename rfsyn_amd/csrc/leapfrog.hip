@@ -1255,21 +1255,16 @@ __device__ __forceinline__ T ade_finish(const typename EntryOf<T>::ade &a, T t, 
     return e_new;
 }
 
+// Block blk of the ADE pass over the box of count nodes at (lo0, lo1, lo2)
+// with j, k extents nj, nk (the whole block takes part: it stages LDS).
 template <typename T, typename TagT, int NS>
-__global__ void __launch_bounds__(256)
-k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, unsigned count,
-           const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
-           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, unsigned ade_blocks, rf_caps_t cs,
-           int cap_ring, const long long *d_step) {
+__device__ __forceinline__ void ade_pass_block(const rf_geom_t &gg, int lo0, int lo1, int lo2, unsigned nj,
+                                               unsigned nk, unsigned count, const T *__restrict__ exa,
+                                               const T *__restrict__ eya, const T *__restrict__ eza,
+                                               T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb,
+                                               const rf_mat_t &m, int ncache, unsigned blk, unsigned n_blocks) {
     using Entry = typename EntryOf<T>::type;
     using Ade = typename EntryOf<T>::ade;
-    // the blocks past the ADE range capture the plane monitors of the step
-    // (their planes lie outside the ADE box, so their values are final)
-    if (blockIdx.x >= ade_blocks) {
-        plane_capture_elem<T>(gg, cs, (long long)(blockIdx.x - ade_blocks) * blockDim.x + threadIdx.x, cap_ring,
-                              *d_step);
-        return;
-    }
     // every ADE set and, for the first ncache entries of the material
     // table, the flags and ADE set ids of the three components (one byte
     // each), staged in LDS
@@ -1293,8 +1288,8 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
     // their dispersive edges, then they are updated, so each round trip
     // carries several nodes' loads
     const Geo g = geo_of(gg);
-    const unsigned stride = gridDim.x * blockDim.x;
-    const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned stride = n_blocks * blockDim.x;
+    const unsigned t0 = blk * blockDim.x + threadIdx.x;
     const T *__restrict__ J = static_cast<const T *>(m.J);
     const T *__restrict__ P = static_cast<const T *>(m.P);
     long long idx[ADE_NPT];
@@ -1386,6 +1381,37 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
                 if (ad.has_p[s]) Pw[o] = np_[s];
             }
         }
+}
+
+template <typename T, typename TagT, int NS>
+__global__ void __launch_bounds__(256)
+k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, unsigned count,
+           const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
+           T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, unsigned ade_blocks, rf_caps_t cs,
+           int cap_ring, const long long *d_step) {
+    // the blocks past the ADE range capture the plane monitors of the step
+    // (their planes lie outside the ADE box, so their values are final)
+    if (blockIdx.x >= ade_blocks) {
+        plane_capture_elem<T>(gg, cs, (long long)(blockIdx.x - ade_blocks) * blockDim.x + threadIdx.x, cap_ring,
+                              *d_step);
+        return;
+    }
+    ade_pass_block<T, TagT, NS>(gg, lo0, lo1, lo2, nj, nk, count, exa, eya, eza, exb, eyb, ezb, m, ncache, blockIdx.x,
+                                ade_blocks);
+}
+
+// The ADE passes of many scenes: block b runs block b - block0 of the
+// job block_job[b].
+template <typename T, typename TagT, int NS>
+__global__ void __launch_bounds__(256)
+k_ade_pass_packed(const rf_ade_job_t *__restrict__ jobs, const unsigned short *__restrict__ block_job) {
+    const rf_ade_job_t &jb = jobs[block_job[blockIdx.x]];
+    const unsigned nb = (jb.count + 255u) / 256u;
+    ade_pass_block<T, TagT, NS>(jb.g, jb.lo[0], jb.lo[1], jb.lo[2], jb.nj, jb.nk, jb.count,
+                                static_cast<const T *>(jb.ea[0]), static_cast<const T *>(jb.ea[1]),
+                                static_cast<const T *>(jb.ea[2]), static_cast<T *>(jb.eb[0]),
+                                static_cast<T *>(jb.eb[1]), static_cast<T *>(jb.eb[2]), jb.m, jb.ncache,
+                                blockIdx.x - jb.block0, nb);
 }
 
 template <typename T, typename TagT>
@@ -1839,4 +1865,41 @@ extern "C" int rf_launch_shell_e_packed(int prec, const rf_shell_job_t *jobs, co
                         : shell_e_packed_tag<double, false>(jobs, block_job, n_blocks, tag_bytes, st);
     return has_cpml ? shell_e_packed_tag<float, true>(jobs, block_job, n_blocks, tag_bytes, st)
                     : shell_e_packed_tag<float, false>(jobs, block_job, n_blocks, tag_bytes, st);
+}
+
+// Pole-count class of the ADE pass variant: 1, 2, 4 or RF_MAX_SLOTS.
+extern "C" int rf_ade_pass_class(int n_slots) {
+    return n_slots <= 1 ? 1 : n_slots == 2 ? 2 : n_slots <= 4 ? 4 : RF_MAX_SLOTS;
+}
+
+extern "C" int rf_launch_ade_pass_packed(int prec, const rf_ade_job_t *jobs, const unsigned short *block_job,
+                                         int n_blocks, int tag_bytes, int ns_class, size_t lds, rf_stream_t *s) {
+    if (n_blocks <= 0) return 0;
+    if (!jobs || !block_job) return -1;
+    hipStream_t st = rf_raw_stream(s);
+#define RF_ADE_PACKED(T, TT, NS) \
+    hipLaunchKernelGGL((k_ade_pass_packed<T, TT, NS>), dim3(n_blocks), dim3(256), lds, st, jobs, block_job)
+#define RF_ADE_PACKED_NS(T, TT)                  \
+    switch (ns_class) {                          \
+    case 1: RF_ADE_PACKED(T, TT, 1); break;      \
+    case 2: RF_ADE_PACKED(T, TT, 2); break;      \
+    case 4: RF_ADE_PACKED(T, TT, 4); break;      \
+    default: RF_ADE_PACKED(T, TT, RF_MAX_SLOTS); \
+    }
+#define RF_ADE_PACKED_TAG(T)                                 \
+    switch (tag_bytes) {                                     \
+    case 1: RF_ADE_PACKED_NS(T, uint8_t); break;             \
+    case 2: RF_ADE_PACKED_NS(T, uint16_t); break;            \
+    case 4: RF_ADE_PACKED_NS(T, uint32_t); break;            \
+    default: return -1;                                      \
+    }
+    if (prec == 0) {
+        RF_ADE_PACKED_TAG(double)
+    } else {
+        RF_ADE_PACKED_TAG(float)
+    }
+#undef RF_ADE_PACKED_TAG
+#undef RF_ADE_PACKED_NS
+#undef RF_ADE_PACKED
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -372,6 +372,27 @@ int rf_launch_shell_h_packed(int prec, const rf_shell_job_t *jobs, const unsigne
 int rf_launch_shell_e_packed(int prec, const rf_shell_job_t *jobs, const unsigned short *block_job, int n_blocks,
                              int tag_bytes, int has_cpml, rf_stream_t *s);
 
+/* The split ADE passes of many scenes: per job the arguments of
+ * rf_launch_ade_pass (box at lo with j, k extents nj, nk and count nodes,
+ * E^n in ea, the plain part and then E^{n+1} in eb; no captures), ncache
+ * = min(n_entries, 4096) entries staged in LDS, and the first block of
+ * its ceil(count / 256) blocks in the launch.  Every job uses the variant
+ * (tag_bytes, ns_class = rf_ade_pass_class(n_slots)); lds = the largest
+ * n_ade * sizeof(rf_ade*_t) + 8 ncache of the jobs. */
+typedef struct {
+    rf_geom_t g;
+    int lo[3];
+    unsigned nj, nk, count;
+    const void *ea[3];
+    void *eb[3];
+    rf_mat_t m;
+    int ncache;
+    unsigned block0;
+} rf_ade_job_t;
+int rf_ade_pass_class(int n_slots);
+int rf_launch_ade_pass_packed(int prec, const rf_ade_job_t *jobs, const unsigned short *block_job, int n_blocks,
+                              int tag_bytes, int ns_class, size_t lds, rf_stream_t *s);
+
 /* Probe gathers of many scenes (rf_launch_probe_record per job, one block
  * each; every job's step counter advances). */
 typedef struct {

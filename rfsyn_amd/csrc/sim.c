@@ -1365,6 +1365,14 @@ typedef struct {
 } bshell_t;
 
 typedef struct {
+    int tag_bytes, ns_class;
+    int n_blocks;
+    size_t lds;
+    rf_ade_job_t *d_jobs[2];
+    unsigned short *d_block_job;
+} bade_t;
+
+typedef struct {
     int ade;                    /* split ADE pass box */
     int alo[3], ahi[3];
     size_t expect;              /* the handle's steps after the batch's last run */
@@ -1382,6 +1390,8 @@ struct fdtd_batch {
     int n_groups;
     bshell_t *shells;
     int n_shells;
+    bade_t *ades;
+    int n_ades;
     rf_probe_job_t *d_probe[2];
     rf_pdft_job_t *d_pdft;
     int max_freq;
@@ -1423,6 +1433,14 @@ static void batch_free_tables(fdtd_batch_t *b) {
     free(b->shells);
     b->shells = NULL;
     b->n_shells = 0;
+    for (int q = 0; q < b->n_ades; q++) {
+        rf_free(b->ades[q].d_jobs[0]);
+        rf_free(b->ades[q].d_jobs[1]);
+        rf_free(b->ades[q].d_block_job);
+    }
+    free(b->ades);
+    b->ades = NULL;
+    b->n_ades = 0;
     rf_free(b->d_probe[0]);
     rf_free(b->d_probe[1]);
     rf_free(b->d_pdft);
@@ -1707,6 +1725,85 @@ static int batch_build(fdtd_batch_t *b) {
         free(gj[1]);
         if (e) goto out;
     }
+    /* split ADE passes by (tag width, pole class), blocks in batch order */
+    b->ades = calloc(n, sizeof(bade_t));
+    if (!b->ades) goto out;
+    for (size_t q = 0; q < n; q++) {
+        fdtd_sim_t *s = b->sims[q];
+        if (!b->sc[q].ade) continue;
+        const int tb = s->mat.d.tag_bytes, nc = rf_ade_pass_class(s->mat.d.n_slots);
+        int g = 0;
+        while (g < b->n_ades && !(b->ades[g].tag_bytes == tb && b->ades[g].ns_class == nc)) g++;
+        if (g == b->n_ades) {
+            b->ades[g].tag_bytes = tb;
+            b->ades[g].ns_class = nc;
+            b->n_ades++;
+        }
+    }
+    for (int g = 0; g < b->n_ades; g++) {
+        bade_t *ad = &b->ades[g];
+        rf_ade_job_t *aj[2] = {calloc(n, sizeof(rf_ade_job_t)), calloc(n, sizeof(rf_ade_job_t))};
+        if (!aj[0] || !aj[1]) {
+            free(aj[0]);
+            free(aj[1]);
+            goto out;
+        }
+        long long blocks = 0;
+        int nj = 0;
+        for (size_t q = 0; q < n; q++) {
+            fdtd_sim_t *s = b->sims[q];
+            const bscene_t *c = &b->sc[q];
+            if (!c->ade || s->mat.d.tag_bytes != ad->tag_bytes || rf_ade_pass_class(s->mat.d.n_slots) != ad->ns_class)
+                continue;
+            const long long cnt = (long long)(c->ahi[0] - c->alo[0]) * (c->ahi[1] - c->alo[1]) * (c->ahi[2] - c->alo[2]);
+            if (cnt <= 0 || cnt >= (1LL << 31)) continue;
+            for (int ph = 0; ph < 2; ph++) {
+                rf_ade_job_t *j = &aj[ph][nj];
+                j->g = s->geom;
+                for (int a = 0; a < 3; a++) j->lo[a] = c->alo[a];
+                j->nj = (unsigned)(c->ahi[1] - c->alo[1]);
+                j->nk = (unsigned)(c->ahi[2] - c->alo[2]);
+                j->count = (unsigned)cnt;
+                for (int f = 0; f < 3; f++) {
+                    j->ea[f] = s->f[f];
+                    j->eb[f] = s->f2[f];
+                }
+                j->m = s->mat.d;
+                j->ncache = s->mat.d.n_entries < 4096 ? s->mat.d.n_entries : 4096;
+                j->block0 = (unsigned)blocks;
+                swap_state(s);
+            }
+            const size_t ade_sz = s->prec == 0 ? sizeof(rf_ade64_t) : sizeof(rf_ade32_t);
+            const size_t lds = (size_t)s->mat.d.n_ade * ade_sz + (size_t)aj[0][nj].ncache * 8;
+            if (lds > ad->lds) ad->lds = lds;
+            blocks += (cnt + 255) / 256;
+            nj++;
+        }
+        if (blocks > INT_MAX || nj > 65535) {
+            free(aj[0]);
+            free(aj[1]);
+            rf_set_error("batch: ADE launch too large");
+            goto out;
+        }
+        free(bj);
+        bj = malloc((size_t)(blocks ? blocks : 1) * sizeof *bj);
+        if (!bj) {
+            free(aj[0]);
+            free(aj[1]);
+            goto out;
+        }
+        for (int q = 0; q < nj; q++) {
+            const long long end = q + 1 < nj ? aj[0][q + 1].block0 : blocks;
+            for (long long k = aj[0][q].block0; k < end; k++) bj[k] = (unsigned short)q;
+        }
+        ad->n_blocks = (int)blocks;
+        int e = h2d_new((void **)&ad->d_jobs[0], aj[0], nj * sizeof(rf_ade_job_t), b->st) ||
+                h2d_new((void **)&ad->d_jobs[1], aj[1], nj * sizeof(rf_ade_job_t), b->st) ||
+                h2d_new((void **)&ad->d_block_job, bj, (size_t)blocks * sizeof *bj, b->st) || rf_sync(b->st);
+        free(aj[0]);
+        free(aj[1]);
+        if (e) goto out;
+    }
     if (h2d_new((void **)&b->d_probe[0], pj[0], n * sizeof(rf_probe_job_t), b->st) ||
         h2d_new((void **)&b->d_probe[1], pj[1], n * sizeof(rf_probe_job_t), b->st) ||
         h2d_new((void **)&b->d_pdft, dj, n * sizeof(rf_pdft_job_t), b->st) || rf_sync(b->st))
@@ -1751,11 +1848,10 @@ static int batch_enqueue_step(fdtd_batch_t *b) {
         pk.qpar = ph;
         if (rf_launch_leapfrog_packed(b->prec, &pk, b->st)) return -1;
     }
-    for (size_t q = 0; q < b->n; q++) {
-        fdtd_sim_t *s = b->sims[q];
-        const bscene_t *c = &b->sc[q];
-        if (c->ade && rf_launch_ade_pass(s->prec, &s->geom, c->alo, c->ahi, s->f, s->f2, &s->mat.d, NULL, s->K,
-                                         s->d_step, b->st))
+    for (int g = 0; g < b->n_ades; g++) {
+        const bade_t *ad = &b->ades[g];
+        if (rf_launch_ade_pass_packed(b->prec, ad->d_jobs[ph], ad->d_block_job, ad->n_blocks, ad->tag_bytes,
+                                      ad->ns_class, ad->lds, b->st))
             return -1;
     }
     for (int g = 0; g < b->n_shells && !overlap; g++) {
@@ -1972,10 +2068,11 @@ int fdtd_batch_run(fdtd_batch_t *b, size_t n_steps) {
 
 size_t fdtd_batch_size(const fdtd_batch_t *b) { return b ? b->n : 0; }
 
-int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell) {
+int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell, int *ade) {
     if (!b || !b->built) return -1;
     if (leapfrog) *leapfrog = b->n_groups;
     if (shell) *shell = 2 * b->n_shells;
+    if (ade) *ade = b->n_ades;
     return 0;
 }
 

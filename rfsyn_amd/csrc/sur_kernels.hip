@@ -1,5 +1,5 @@
 // Kernels of the learned surrogate (fdtd_surrogate.h): a BF16 matrix
-// product on the gfx950 matrix cores (v_mfma_f32_16x16x16_bf16, FP32
+// product on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
 // accumulation) with split-K, 3D convolution as im2col / col2im, GELU,
 // and the Adam step.  No reference counterpart: the reference has no
 // learned part.
@@ -17,7 +17,6 @@
 
 namespace {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even; NaN stays NaN.
@@ -29,72 +28,140 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 
 // C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m sam + k sak] and
-// B(k, n) = B[k sbk + n sbn] (any transposition by strides).  A block
-// of 4 wavefronts computes a 64x64 tile; each wavefront a 32x32 quarter
-// as 2x2 16x16x16 products.  Fragment layout of the instruction (lane l):
-// A row l%16, k 4(l/16)..+3; B column l%16, k 4(l/16)..+3; D column l%16,
-// rows 4(l/16)..+3.  gridDim.z splits K: split z covers k in
-// [z kchunk, (z+1) kchunk) and writes its partial tile to part + z M N
-// (or, with one split, C + bias directly).  M tiles run along x, N tiles
-// along y.
-constexpr int TM = 64, TN = 64, TK = 16;
+// B(k, n) = B[k sbk + n sbn] (any transposition by strides), on the gfx950
+// v_mfma_f32_16x16x32_bf16 instruction (BF16 operands, FP32 accumulation).
+// Fragment layout (lane l): A row l%16, k 8(l/16)..+7; B column l%16,
+// k 8(l/16)..+7; D column l%16, rows 4(l/16)..+3.  A block of 4
+// wavefronts stacked along M computes a (64 WM) x (16 WN) tile, each
+// wavefront WM x WN fragments; k advances 32 at a time.  The block
+// stages A as rows of k and B transposed (columns of k) in LDS, so every
+// fragment is one 16-byte LDS read; an operand whose k (or, for A, m; for
+// B, n) stride is 1 is read 8 values at a time.  gridDim.z splits K:
+// split z covers k in [z kchunk, (z+1) kchunk) and writes its partial
+// tile to part + z M N (or, with one split, C + bias directly).  M tiles
+// run along x, N tiles along y.
+constexpr int TK = 32, KP = TK + 8;   // k per step, padded LDS row (80 bytes)
 
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned short v8u __attribute__((ext_vector_type(8)));
+
+template <int WM, int WN>
 __global__ void __launch_bounds__(256)
 k_gemm(int M, int N, int K, const uint16_t *__restrict__ A, long long sam, long long sak,
        const uint16_t *__restrict__ B, long long sbk, long long sbn, float *__restrict__ C, long long ldc,
        const float *__restrict__ bias, int kchunk, float *__restrict__ part) {
-    __shared__ uint16_t sa[TM][TK + 2];
-    __shared__ uint16_t sb[TK][TN + 2];
+    constexpr int BM = 64 * WM, BN = 16 * WN;
+    __shared__ __align__(16) uint16_t sa[BM][KP];
+    __shared__ __align__(16) uint16_t sb[BN][KP];   // B transposed: sb[n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;   // M tiles on x: up to 2^31 blocks
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int kb = blockIdx.z * kchunk;
     const int ke = min(K, kb + kchunk);
-    v4f acc[2][2];
+    v4f acc[WM][WN];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    // with unit stride along k the tile rows are read along k, else along m (n)
-    const bool a_k = sak == 1, b_n = sbn == 1;
+        for (int j = 0; j < WN; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    // 8-wide global reads where the contiguous axis allows them
+    const bool a_k8 = sak == 1 && (sam % 8) == 0 && (reinterpret_cast<uintptr_t>(A) % 16) == 0;
+    const bool a_m8 = sam == 1 && (sak % 8) == 0 && (reinterpret_cast<uintptr_t>(A) % 16) == 0;
+    const bool b_k8 = sbk == 1 && (sbn % 8) == 0 && (reinterpret_cast<uintptr_t>(B) % 16) == 0;
+    const bool b_n8 = sbn == 1 && (sbk % 8) == 0 && (reinterpret_cast<uintptr_t>(B) % 16) == 0;
     for (int k0 = kb; k0 < ke; k0 += TK) {
-        for (int e = tid; e < TM * TK; e += 256) {
-            const int r = a_k ? e / TK : e % TM, c = a_k ? e % TK : e / TM;
-            const int m = m0 + r, k = k0 + c;
-            sa[r][c] = (m < M && k < ke) ? A[m * sam + k * sak] : uint16_t(0);
+        // ---- A tile [BM][TK]
+        if (a_k8) {
+            for (int e = tid; e < BM * (TK / 8); e += 256) {
+                const int r = e / (TK / 8), c = (e % (TK / 8)) * 8;
+                const int m = m0 + r, k = k0 + c;
+                v8u v = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+                if (m < M && k + 7 < ke) {
+                    v = *reinterpret_cast<const v8u *>(A + m * sam + k);
+                } else if (m < M) {
+#pragma unroll
+                    for (int t = 0; t < 8; t++) v[t] = k + t < ke ? A[m * sam + (k + t)] : uint16_t(0);
+                }
+                *reinterpret_cast<v8u *>(&sa[r][c]) = v;
+            }
+        } else if (a_m8) {
+            for (int e = tid; e < (BM / 8) * TK; e += 256) {
+                const int r = (e % (BM / 8)) * 8, c = e / (BM / 8);
+                const int m = m0 + r, k = k0 + c;
+                v8u v = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+                if (k < ke && m + 7 < M) {
+                    v = *reinterpret_cast<const v8u *>(A + m + k * sak);
+                } else if (k < ke) {
+#pragma unroll
+                    for (int t = 0; t < 8; t++) v[t] = m + t < M ? A[(m + t) + k * sak] : uint16_t(0);
+                }
+#pragma unroll
+                for (int t = 0; t < 8; t++) sa[r + t][c] = v[t];
+            }
+        } else {
+            for (int e = tid; e < BM * TK; e += 256) {
+                const int r = e / TK, c = e % TK;
+                const int m = m0 + r, k = k0 + c;
+                sa[r][c] = (m < M && k < ke) ? A[m * sam + k * sak] : uint16_t(0);
+            }
         }
-        for (int e = tid; e < TK * TN; e += 256) {
-            const int r = b_n ? e / TN : e % TK, c = b_n ? e % TN : e / TK;
-            const int k = k0 + r, n = n0 + c;
-            sb[r][c] = (k < ke && n < N) ? B[k * sbk + n * sbn] : uint16_t(0);
+        // ---- B tile, transposed [BN][TK]
+        if (b_k8) {
+            for (int e = tid; e < BN * (TK / 8); e += 256) {
+                const int r = e / (TK / 8), c = (e % (TK / 8)) * 8;
+                const int n = n0 + r, k = k0 + c;
+                v8u v = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+                if (n < N && k + 7 < ke) {
+                    v = *reinterpret_cast<const v8u *>(B + k + n * sbn);
+                } else if (n < N) {
+#pragma unroll
+                    for (int t = 0; t < 8; t++) v[t] = k + t < ke ? B[(k + t) + n * sbn] : uint16_t(0);
+                }
+                *reinterpret_cast<v8u *>(&sb[r][c]) = v;
+            }
+        } else if (b_n8 && BN >= 8) {
+            for (int e = tid; e < (BN / 8) * TK; e += 256) {
+                const int r = (e % (BN / 8)) * 8, c = e / (BN / 8);
+                const int n = n0 + r, k = k0 + c;
+                v8u v = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+                if (k < ke && n + 7 < N) {
+                    v = *reinterpret_cast<const v8u *>(B + k * sbk + n);
+                } else if (k < ke) {
+#pragma unroll
+                    for (int t = 0; t < 8; t++) v[t] = n + t < N ? B[k * sbk + (n + t)] : uint16_t(0);
+                }
+#pragma unroll
+                for (int t = 0; t < 8; t++) sb[r + t][c] = v[t];
+            }
+        } else {
+            for (int e = tid; e < BN * TK; e += 256) {
+                const int r = e / TK, c = e % TK;
+                const int n = n0 + r, k = k0 + c;
+                sb[r][c] = (k < ke && n < N) ? B[k * sbk + n * sbn] : uint16_t(0);
+            }
         }
         __syncthreads();
-        const int kk = (lane >> 4) * 4;
+        const int kk = (lane >> 4) * 8;
+        v8bf b[WN];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int j = 0; j < WN; j++)
+            b[j] = *reinterpret_cast<const v8bf *>(&sb[j * 16 + (lane & 15)][kk]);
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int ar = wm * 32 + i * 16 + (lane & 15), bc = wn * 32 + j * 16 + (lane & 15);
-                v4s a, b;
+        for (int i = 0; i < WM; i++) {
+            const v8bf a = *reinterpret_cast<const v8bf *>(&sa[wave * 16 * WM + i * 16 + (lane & 15)][kk]);
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    a[t] = static_cast<short>(sa[ar][kk + t]);
-                    b[t] = static_cast<short>(sb[kk + t][bc]);
-                }
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
-            }
+            for (int j = 0; j < WN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
     float *out = gridDim.z > 1 ? part + (long long)blockIdx.z * M * N : C;
     const long long ld = gridDim.z > 1 ? N : ldc;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < WN; j++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + t;
-                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                const int m = m0 + wave * 16 * WM + i * 16 + (lane >> 4) * 4 + t;
+                const int n = n0 + j * 16 + (lane & 15);
                 if (m < M && n < N) {
                     float v = acc[i][j][t];
                     if (gridDim.z == 1 && bias) v += bias[n];
@@ -301,9 +368,19 @@ int rf_sur_gemm(int M, int N, int K, const uint16_t *A, long long sam, long long
     int kchunk = (K + splits - 1) / splits;
     kchunk = (kchunk + TK - 1) / TK * TK;
     splits = (K + kchunk - 1) / kchunk;
-    if ((N + TN - 1) / TN > 65535 || splits > 65535) return -1;
-    const dim3 grd((M + TM - 1) / TM, (N + TN - 1) / TN, splits);
-    hipLaunchKernelGGL(k_gemm, grd, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, kchunk, part);
+    // tile shape by the width of the product: 256 x 16, 128 x 32 or 128 x 64
+    const int bm = N <= 16 ? 256 : 128, bn = N <= 16 ? 16 : (N <= 32 ? 32 : 64);
+    if ((N + bn - 1) / bn > 65535 || splits > 65535) return -1;
+    const dim3 grd((M + bm - 1) / bm, (N + bn - 1) / bn, splits);
+    if (N <= 16)
+        hipLaunchKernelGGL((k_gemm<4, 1>), grd, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias,
+                           kchunk, part);
+    else if (N <= 32)
+        hipLaunchKernelGGL((k_gemm<2, 2>), grd, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias,
+                           kchunk, part);
+    else
+        hipLaunchKernelGGL((k_gemm<2, 4>), grd, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias,
+                           kchunk, part);
     if (splits > 1)
         hipLaunchKernelGGL(k_split_sum, dim3(grid_for((long long)M * N)), dim3(256), 0, st, M, N, splits, part, C, ldc,
                            bias);

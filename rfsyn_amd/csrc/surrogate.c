@@ -259,10 +259,12 @@ int fdtd_sur_set_params(fdtd_sur_t *m, const float *in) {
 }
 
 /* C[M][N] = A B with split-K chosen so that about 512 blocks run and the
- * partials fit the workspace. */
+ * partials fit the workspace (tiles as rf_sur_gemm shapes them: 256 x 16,
+ * 128 x 32 or 128 x 64 by the width N). */
 static int gemm(fdtd_sur_t *m, int M, int N, int K, const uint16_t *A, long long sam, long long sak,
                 const uint16_t *B, long long sbk, long long sbn, float *C, const float *bias) {
-    const long long tiles = (long long)((M + 63) / 64) * ((N + 63) / 64);
+    const int bm = N <= 16 ? 256 : 128, bn = N <= 16 ? 16 : (N <= 32 ? 32 : 64);
+    const long long tiles = (long long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
     long long splits = 1;
     if (tiles < 512) splits = (512 + tiles - 1) / tiles;
     if (splits > K / 256) splits = K / 256;

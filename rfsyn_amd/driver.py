@@ -474,10 +474,10 @@ class Batch:
         assert self.lib.fdtd_batch_run(self.b, steps) == 0, self.lib.fdtd_last_error()
 
     def launches(self):
-        """(packed leapfrog launches, shell launches) per step."""
-        lf, sh = C.c_int(), C.c_int()
-        assert self.lib.fdtd_batch_launches(self.b, C.byref(lf), C.byref(sh)) == 0
-        return lf.value, sh.value
+        """(packed leapfrog, shell, split ADE pass) launches per step."""
+        lf, sh, ad = C.c_int(), C.c_int(), C.c_int()
+        assert self.lib.fdtd_batch_launches(self.b, C.byref(lf), C.byref(sh), C.byref(ad)) == 0
+        return lf.value, sh.value, ad.value
 
     def close(self):
         if self.b:

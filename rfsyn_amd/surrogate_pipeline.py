@@ -145,11 +145,12 @@ def source_spectrum(lib, dt, steps):
     return (w[None, :] * np.exp(-1j * om[:, None] * (t[None, :] + dt))).sum(axis=1) * dt
 
 
-def label_scenes(lib, indices, steps, batch=128, n=64, npml=8, seed=21, out_dir=None):
+def label_scenes(lib, indices, steps, batch=128, n=64, npml=8, seed=21, out_dir=None, log=None):
     """(maps [N, n, n, 3], coords [N, 8, 6], labels [N, 8, 128], seconds of
-    FDTD) for the scenes, stepped together in batches (fdtd_sim_run_many).
-    With out_dir, each scene's transfer functions and its JSON description
-    are written there too (write_scene)."""
+    FDTD) for the scenes, stepped together in batches (fdtd_sim_run_many,
+    packed launches).  With out_dir, each scene's transfer functions and
+    its JSON description are written there too (write_scene); log, if
+    given, receives a line per batch."""
     maps, coords, labels = [], [], []
     spec = None
     elapsed = 0.0
@@ -170,6 +171,8 @@ def label_scenes(lib, indices, steps, batch=128, n=64, npml=8, seed=21, out_dir=
             maps.append(scene_map(sc))
             coords.append(scene_coords(sc))
             run.close()
+        if log:
+            log(f"labelled {len(labels)}/{len(indices)} scenes, {elapsed:.1f} s of FDTD")
     return np.stack(maps), np.stack(coords), np.stack(labels), elapsed
 
 

@@ -49,3 +49,20 @@ def test_python_bindings_cover_the_library():
         pytest.skip("library not built")
     lib = rfsyn_amd.capi.bind(ctypes.CDLL(rfsyn_amd.LIB_PATH))
     assert lib.fdtd_sim_kernel_times3 is not None
+
+
+def test_batch_argument_checks():
+    """The packed-batch entry points refuse bad arguments before touching
+    the device: no handles, NULL handles, NULL batches."""
+    if not os.path.exists(rfsyn_amd.LIB_PATH):
+        pytest.skip("library not built")
+    lib = rfsyn_amd.capi.bind(ctypes.CDLL(rfsyn_amd.LIB_PATH))
+    assert not lib.fdtd_batch_create(None, 0)
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert not lib.fdtd_batch_create(arr, 2)
+    assert b"NULL handle" in lib.fdtd_last_error()
+    assert lib.fdtd_batch_run(None, 10) == -1
+    assert lib.fdtd_batch_size(None) == 0
+    assert lib.fdtd_batch_launches(None, None, None, None) == -1
+    lib.fdtd_batch_destroy(None)
+    assert lib.fdtd_sim_dft_block(None) == -1

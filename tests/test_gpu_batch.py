@@ -131,8 +131,8 @@ def test_packed_batch_equals_separate_runs(gpu, precision):
     scenes = _mixed(gpu, precision, 200)
     b = Batch(gpu, [r for r, _, _ in scenes])
     b.run(64)
-    lf, sh = b.launches()
-    assert lf == 1 and sh == 2
+    lf, sh, ad = b.launches()
+    assert lf == 1 and sh == 2 and ad == 1
     scenes[0][0].run(10)
     b.run(86)
     b.close()
