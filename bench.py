@@ -269,7 +269,7 @@ class C2:
         return el, self.n[0] * self.n[1] * self.n[2], "port"
 
 
-WORKLOADS = {"c1": C1, "c2": C2, "c3": C3, "c4": None, "c5": None}
+WORKLOADS = {"c1": C1, "c2": C2, "c3": C3, "c4": None, "c5": None, "c5train": None}
 
 
 def sur_gemm_flops(n, grid=64, chans=(3, 16, 32, 64, 128), width=512, queries=8, outputs=128, coords=78):
@@ -288,6 +288,41 @@ def sur_gemm_flops(n, grid=64, chans=(3, 16, 32, 64, 128), width=512, queries=8,
     bwd += (2 * W * O * q + 2 * q * O + 2 * q * O * W + 2 * W * W * q + 2 * q * W + 2 * q * W * W
             + 2 * coords * W * q + 2 * q * W + 2 * F * W * n + 2 * n * F * W)
     return fwd, fwd + bwd
+
+
+def bench_c5train(args, lib):
+    """The C5 network's Adam step alone at its training batch (256 scenes of
+    the dispersive stack distribution as 64^3 maps, 8 queries, random
+    labels; no FDTD): W untimed then K timed steps, each ending with the
+    loss read back.  Roofline: the step's product FLOPs against the BF16
+    MFMA peak."""
+    from rfsyn_amd import surrogate_pipeline as sp
+    bt = args.c5_batch
+    rng = np.random.default_rng(3)
+    scs = [sp.scene(i) for i in range(bt)]
+    maps = np.stack([sp.scene_map(sc) for sc in scs])
+    coords = np.stack([sp.scene_coords(sc) for sc in scs])
+    labels = rng.standard_normal((bt, sp.N_PROBES, 2 * sp.N_FREQS)).astype(np.float32)
+    net = sp.make_net(lib, batch=bt)
+    for _ in range(args.warmup):
+        net.train_step(maps, coords, labels, args.c5_lr)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.train_step(maps, coords, labels, args.c5_lr)
+    wall = time.perf_counter() - t0
+    net.close()
+    flops = sur_gemm_flops(bt)[1]
+    tf = flops * args.steps / wall / 1e12
+    print(json.dumps({
+        "metric": "surrogate training step throughput (BF16 products, FP32 accumulation)",
+        "value": round(tf, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "c5 network training step: 64^3 maps, 4 stride-2 3x3x3 convs 16-32-64-128, "
+                               "MLP width 512, 8 queries x 128 outputs", "batch": bt},
+        "roofline": {"bound": "mfma", "kernel": "training step (all products)", "achieved": round(tf, 3),
+                     "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tf / 2500.0, 5), "traffic": None,
+                     "flops_per_step": flops}}))
 
 
 def bench_c5(args, lib):
@@ -526,6 +561,9 @@ def main():
         return
     if args.workload == "c5":
         bench_c5(args, lib)
+        return
+    if args.workload == "c5train":
+        bench_c5train(args, lib)
         return
     wl = WORKLOADS[args.workload]()
     timed_steps = args.warmup + args.steps
