@@ -543,6 +543,9 @@ def main():
     ap.add_argument("--c5-fdtd-steps", type=int, default=1024, help="c5: FDTD steps per label")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiling of the fused step instead of timing the candidates first")
+    ap.add_argument("--fp32-rows", type=int, default=0,
+                    help="c1-c3: run the FP32 fused step at this many rows per work item (1: row mode, every z "
+                         "column per item; 4 or 8: 62-column tiles) instead of autotuning its tiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -571,10 +574,15 @@ def main():
     res = {}
     for prec in (wl.headline, capi.FDTD_FP32 if wl.headline == capi.FDTD_FP64 else capi.FDTD_FP64):
         tag = "fp32" if prec == capi.FDTD_FP32 else "fp64"
+        fixed = args.fp32_rows and prec == capi.FDTD_FP32
+        if fixed:   # read by the library when the handle is created
+            os.environ["RFSYN_LEAP_ROWS"] = str(args.fp32_rows)
         run, pids, mons = wl.build(lib, prec, total)
-        chosen = (C.c_int * 4)(0, 0, 0, 0)
+        if fixed:
+            del os.environ["RFSYN_LEAP_ROWS"]
+        chosen = (C.c_int * 4)(args.fp32_rows if fixed else 0, 0, 0, 0)
         two = C.c_int(0)
-        if not args.no_autotune:
+        if not args.no_autotune and not fixed:
             assert lib.fdtd_sim_autotune(run.sim, 3, chosen) == 0, lib.fdtd_last_error()
             assert lib.fdtd_sim_select_path(run.sim, 3, C.byref(two)) == 0, lib.fdtd_last_error()
         run.run(args.warmup)
@@ -602,8 +610,9 @@ def main():
         run.close()
         res[prec] = dict(wall=wall, h_ms=h_ms / nk, e_ms=e_ms / nk, sh_ms=sh_ms / nk, dft=dft, fused=fused, box=box,
                          planes=planes, block=block,
-                         tiling=None if args.no_autotune or not fused else
-                         {"rows": chosen[0], "planes_per_item": chosen[1], "z_split": chosen[2], "coop": chosen[3]})
+                         tiling=None if (args.no_autotune and not fixed) or not fused else
+                         {"rows": chosen[0], "planes_per_item": chosen[1], "z_split": chosen[2], "coop": chosen[3],
+                          "row_mode": prec == capi.FDTD_FP32 and chosen[0] == 1})
 
     n = wl.n
     cells = n[0] * n[1] * n[2]

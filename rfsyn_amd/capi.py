@@ -233,6 +233,7 @@ EXTENSION_API = {
     "fdtd_sim_energy_audit": (C.c_int, [_vp, _dp, C.POINTER(_sz)]),
     "fdtd_sim_run_many": (C.c_int, [C.POINTER(_vp), _sz, _sz]),
     "fdtd_sim_dft_block": (C.c_int, [_vp]),
+    "fdtd_sim_leap_rows": (C.c_int, [_vp]),
     "fdtd_batch_create": (_vp, [C.POINTER(_vp), _sz]),
     "fdtd_batch_run": (C.c_int, [_vp, _sz]),
     "fdtd_batch_destroy": (None, [_vp]),

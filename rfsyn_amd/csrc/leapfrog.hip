@@ -871,6 +871,414 @@ k_leapfrog_packed(const rf_leap_t *__restrict__ tab, const int *__restrict__ ite
     }
 }
 
+// ------------------------------------------------------------ row mode
+// FP32, z CPML inside the box and the box spanning every z column [0, nz]:
+// one wavefront owns R rows (y) of all the columns and marches along x
+// through a chunk of planes.  Lane l of chunk c holds column 64 c + l, so
+// a row is loaded and stored as whole 128-byte lines that no other
+// wavefront touches, and every wavefront does the same work per plane (the
+// z CPML slabs lie in its first and last chunk): the wavefronts of an XCD
+// stay at nearby planes and their halo rows meet in its L2.  A chunk's
+// edge lane takes its z neighbour from the next or previous chunk by
+// readlane.  When nz = 64 ROW_NCH, column nz lies past the chunks; there
+// only hz changes (the boundary's tangential E and the H components with a
+// z difference keep their values, equal in both state buffers), so its E
+// values are read as wavefront-uniform values and its hz update is done by
+// every lane and stored by one.  The expressions and their order are those
+// of leap_item.
+constexpr int ROW_NCH = 4;   // chunks of 64 columns: 64 (ROW_NCH - 1) < nz <= 64 ROW_NCH
+
+__device__ __forceinline__ float lane_of(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ float uniform_of(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+template <typename TagT, int R>
+__device__ __forceinline__ void leap_row(const rf_leap_t &a, const LeapLds<float> &lv, const float *s_zk,
+                                         long long step, int tj, int tc, int lane) {
+    using T = float;
+    constexpr int NCH = ROW_NCH;
+    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+    const Geo g = geo_of(a.g);
+    const int nls = lv.nls, ncache = lv.ncache;
+    const LSrc *s_src = lv.src;
+    const T *s_ca = lv.ca, *s_cb = lv.cb;
+    const uint32_t *s_fl = lv.fl;
+
+    const int j0 = a.ib0[1] + tj * R;
+    const int jend = a.ib1[1];
+    const int i_begin = a.ib0[0] + tc * a.chunk;
+    const int i_end = min(i_begin + a.chunk, a.ib1[0]);
+    const int sy = static_cast<int>(g.sy);
+    const bool bnd = g.nz == 64 * NCH;   // column nz past the chunks
+
+    // this lane's column in each chunk (clamped to nz past it), ownership
+    // (nz > 64 (NCH - 1): only the last chunk reaches past nz)
+    int kc[NCH];
+    bool col_own[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        const int k = 64 * c + lane;
+        kc[c] = c < NCH - 1 ? k : min(k, g.nz);
+        col_own[c] = c < NCH - 1 || k <= g.nz;
+    }
+    int jr[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) {
+        const int jt = j0 - 1 + r;
+        jr[r] = g.py ? (jt + g.ny) % g.ny : min(max(jt, 0), g.ny);
+    }
+    // z CPML of the first (e = 0) and last (e = 1) chunk's column: outside
+    // the plain range, slab coordinate; the profile values are read from
+    // the block's LDS copy (zprof) where they are used
+    bool zon[2], zs[2];
+    int zcq[2];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const int q = e == 0 ? kc[0] : kc[NCH - 1];
+        zon[e] = q < a.zplo || q >= a.zphi;
+        zcq[e] = q < a.zlo ? q : (q >= a.zhi ? a.zlo + q - a.zhi : -1);
+        zs[e] = __ballot(zon[e]) != 0;
+    }
+    auto zprof = [&](int e, int p) { return s_zk[(e * 6 + p) * WL + lane]; };
+    const int zr = 2 * a.zpar, zw = 2 - 2 * a.zpar;
+    const rsrc_t RZ = make_rsrc(a.zpsi, static_cast<unsigned>(a.zbytes));
+    const unsigned zsb = static_cast<unsigned>(a.zlen) * sizeof(T);
+    auto zrow = [&](int ii, int jj) {
+        return (static_cast<unsigned>(ii) * static_cast<unsigned>(g.ny + 1) + static_cast<unsigned>(jj)) * 6 * zsb;
+    };
+    const bool src_tile = a.n_srcs > 0 && a.src_box[2] < j0 + R && a.src_box[3] >= j0;
+    bool src_lane[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) src_lane[c] = kc[c] >= a.src_box[4] && kc[c] <= a.src_box[5];
+
+    constexpr unsigned ES = sizeof(T);
+    const unsigned fsb = static_cast<unsigned>(a.fstride) * ES;
+    const rsrc_t RA = make_rsrc(a.blk_a, 6 * fsb), RB = make_rsrc(a.blk_b, 6 * fsb);
+    const rsrc_t RT = make_rsrc(a.tag, TAGS ? static_cast<unsigned>(a.fstride) * sizeof(TagT) : 0u);
+    unsigned vk[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) vk[c] = static_cast<unsigned>(kc[c]) * ES;
+    const unsigned vb = static_cast<unsigned>(g.nz) * ES;   // column nz
+    const unsigned sxb = static_cast<unsigned>(g.sx) * ES, syb = static_cast<unsigned>(g.sy) * ES;
+    unsigned rowb[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) rowb[r] = static_cast<unsigned>(jr[r]) * syb;
+    const T cb = static_cast<T>(a.cb), dx = static_cast<T>(a.dx);
+    const T vac = static_cast<T>(a.vac_ca);
+
+    const bool warm = i_begin > 0 || g.px;
+    int i = warm ? (i_begin == 0 ? g.nx - 1 : i_begin - 1) : i_begin;
+    const int iters = (i_end - i_begin) + (warm ? 1 : 0);
+
+    T cx[NCH][R + 2], cy[NCH][R + 2], cz[NCH][R + 2];   // E^n, plane i
+    T nx[NCH][R + 2], ny[NCH][R + 2], nz[NCH][R + 2];   // E^n, plane i+1
+    T py[NCH][R + 1], pz[NCH][R + 1];                   // hy, hz of plane i-1
+    T bx[R + 2], by[R + 1];                             // ex, ey of column nz, plane i
+    {
+        const unsigned pb = static_cast<unsigned>(i) * sxb;
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+            for (int r = 0; r < R + 2; r++) {
+                cx[c][r] = bld<T>(RA, vk[c], pb + rowb[r]);
+                cy[c][r] = r <= R ? bld<T>(RA, vk[c], fsb + pb + rowb[r]) : T(0);
+                cz[c][r] = bld<T>(RA, vk[c], 2 * fsb + pb + rowb[r]);
+            }
+#pragma unroll
+        for (int r = 0; r < R + 2; r++) {
+            bx[r] = bnd ? uniform_of(bld<T>(RA, vb, pb + rowb[r])) : T(0);
+            if (r <= R) by[r] = bnd ? uniform_of(bld<T>(RA, vb, fsb + pb + rowb[r])) : T(0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+        for (int r = 0; r <= R; r++) py[c][r] = pz[c][r] = T(0);
+
+    for (int it = 0; it < iters; it++) {
+        const bool first = warm && it == 0;
+        int ip = i + 1;
+        if (g.px) {
+            if (ip == g.nx) ip = 0;
+        } else if (ip > g.nx) {
+            ip = g.nx;
+        }
+        const unsigned pb = static_cast<unsigned>(i) * sxb, npb = static_cast<unsigned>(ip) * sxb;
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+            for (int r = 0; r < R + 2; r++) {
+                nx[c][r] = bld<T>(RA, vk[c], npb + rowb[r]);
+                ny[c][r] = r <= R ? bld<T>(RA, vk[c], fsb + npb + rowb[r]) : T(0);
+                nz[c][r] = bld<T>(RA, vk[c], 2 * fsb + npb + rowb[r]);
+            }
+        T hx[NCH][R + 1], hy[NCH][R + 1], hz[NCH][R + 1];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+            for (int r = 0; r <= R; r++) {
+                hx[c][r] = bld<T>(RA, vk[c], 3 * fsb + pb + rowb[r]);
+                hy[c][r] = r > 0 ? bld<T>(RA, vk[c], 4 * fsb + pb + rowb[r]) : T(0);
+                hz[c][r] = bld<T>(RA, vk[c], 5 * fsb + pb + rowb[r]);
+            }
+        T nbx[R + 2], nby[R + 1], bhz[R + 1];
+#pragma unroll
+        for (int r = 0; r < R + 2; r++) {
+            nbx[r] = bnd ? uniform_of(bld<T>(RA, vb, npb + rowb[r])) : T(0);
+            if (r <= R) {
+                nby[r] = bnd ? uniform_of(bld<T>(RA, vb, fsb + npb + rowb[r])) : T(0);
+                bhz[r] = bnd && r >= 1 ? uniform_of(bld<T>(RA, vb, 5 * fsb + pb + rowb[r])) : T(0);
+            }
+        }
+        // z-psi of the slab lanes of the first and last chunk
+        T zh0[2][R + 1], zh1[2][R + 1], ze0[2][R + 1], ze1[2][R + 1];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+#pragma unroll
+            for (int r = 0; r <= R; r++) zh0[e][r] = zh1[e][r] = ze0[e][r] = ze1[e][r] = T(0);
+            if (zs[e]) {
+                const bool slab = zcq[e] >= 0;
+                const unsigned zl = static_cast<unsigned>(slab ? zcq[e] : 0) * ES;
+#pragma unroll
+                for (int r = 0; r <= R; r++) {
+                    const unsigned zb = zrow(i, jr[r]);
+                    const T h0 = bld<T>(RZ, zl, zb + zr * zsb), h1 = bld<T>(RZ, zl, zb + (zr + 1) * zsb);
+                    zh0[e][r] = slab ? h0 : T(0);
+                    zh1[e][r] = slab ? h1 : T(0);
+                    if (r >= 1) {
+                        const T e0 = bld<T>(RZ, zl, zb + 4 * zsb), e1 = bld<T>(RZ, zl, zb + 5 * zsb);
+                        ze0[e][r] = slab ? e0 : T(0);
+                        ze1[e][r] = slab ? e1 : T(0);
+                    }
+                }
+            }
+        }
+        unsigned tg[NCH][R + 1];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+            for (int r = 0; r <= R; r++) {
+                tg[c][r] = 0;
+                if constexpr (TAGS)
+                    if (r >= 1)
+                        tg[c][r] = btag<TagT>(RT, static_cast<unsigned>(kc[c]) * sizeof(TagT),
+                                              (static_cast<unsigned>(i) * static_cast<unsigned>(g.sx) +
+                                               static_cast<unsigned>(jr[r]) * static_cast<unsigned>(sy)) *
+                                                  sizeof(TagT));
+            }
+
+        // ---- H^{n+1/2}(i), rows j0-1 .. j0+R-1 (grid.c:70-89, cpml.c:185-215)
+        const bool hx_i = i < g.nx + !g.px, hyz_i = i < g.nx;
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int e = c == 0 ? 0 : 1;
+            const bool zce = c == 0 || c == NCH - 1;
+#pragma unroll
+            for (int r = 0; r <= R; r++) {
+                const int jc = jr[r];
+                T eyk = from_upper(cy[c][r]);   // ey(i, j, k+1)
+                T exk = from_upper(cx[c][r]);   // ex(i, j, k+1)
+                const T uy = c + 1 < NCH ? lane_of(cy[c + 1 < NCH ? c + 1 : c][r], 0) : by[r];
+                const T ux = c + 1 < NCH ? lane_of(cx[c + 1 < NCH ? c + 1 : c][r], 0) : bx[r];
+                eyk = lane == WL - 1 ? uy : eyk;
+                exk = lane == WL - 1 ? ux : exk;
+                const int k = kc[c];
+                const bool own = !first && r >= 1 && j0 - 1 + r < jend && col_own[c];
+                const bool do_x = hx_i && jc < g.ny && k < g.nz;
+                const bool do_y = r >= 1 && hyz_i && jc < g.ny + !g.py && k < g.nz;
+                const bool do_z = hyz_i && jc < g.ny && k < g.nz + !g.pz;
+                const T dAx = cz[c][r + 1] - cz[c][r];
+                T dBx = eyk - cy[c][r];
+                T dAy = exk - cx[c][r];
+                const T dBy = nz[c][r] - cz[c][r];
+                const T dAz = ny[c][r] - cy[c][r], dBz = cx[c][r + 1] - cx[c][r];
+                if (zce && zs[e] && zon[e] && (do_x || do_y)) {
+                    const bool slab = zcq[e] >= 0;
+                    const T kh = zprof(e, 0), bh = zprof(e, 1), ah = zprof(e, 2);
+                    T px_, py_;
+                    if (do_x) dBx = zterm<T>(dBx, kh, bh, ah, slab, zh0[e][r], px_, dx);
+                    if (do_y) dAy = zterm<T>(dAy, kh, bh, ah, slab, zh1[e][r], py_, dx);
+                    if (slab && own) {
+                        const unsigned zl = static_cast<unsigned>(zcq[e]) * ES, zb = zrow(i, jc);
+                        if (do_x) bst(px_, RZ, zl, zb + zw * zsb);
+                        if (do_y) bst(py_, RZ, zl, zb + (zw + 1) * zsb);
+                    }
+                }
+                const T nhx = hx[c][r] - cb * (dAx - dBx);
+                const T nhy = hy[c][r] - cb * (dAy - dBy);
+                const T nhz = hz[c][r] - cb * (dAz - dBz);
+                hx[c][r] = do_x ? nhx : hx[c][r];
+                hy[c][r] = do_y ? nhy : hy[c][r];
+                hz[c][r] = do_z ? nhz : hz[c][r];
+                if (own) {
+                    bst(hx[c][r], RB, vk[c], 3 * fsb + pb + rowb[r]);
+                    bst(hy[c][r], RB, vk[c], 4 * fsb + pb + rowb[r]);
+                    bst(hz[c][r], RB, vk[c], 5 * fsb + pb + rowb[r]);
+                }
+            }
+        }
+        // hz of column nz (z is not periodic: k = nz < nz + 1)
+        if (bnd && !first) {
+#pragma unroll
+            for (int r = 1; r <= R; r++) {
+                const int jc = jr[r];
+                const bool do_z = hyz_i && jc < g.ny;
+                const T dAz = nby[r] - by[r], dBz = bx[r + 1] - bx[r];
+                const T nhz = bhz[r] - cb * (dAz - dBz);
+                if (j0 - 1 + r < jend && lane == 0) bst(do_z ? nhz : bhz[r], RB, vb, 5 * fsb + pb + rowb[r]);
+            }
+        }
+
+        // ---- sources and E^{n+1}(i), rows j0 .. j0+R-1 (materials.c:975-1058)
+        if (!first) {
+            const bool i_ok = g.px || i >= 1;
+            const bool src = src_tile && a.src_box[0] <= i && a.src_box[1] >= i;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const int e_ = c == 0 ? 0 : 1;
+                const bool zce = c == 0 || c == NCH - 1;
+                const int kk = kc[c];
+#pragma unroll
+                for (int r = 1; r <= R; r++) {
+                    T hxk = from_lower(hx[c][r]);   // hx(i, j, k-1)
+                    T hyk = from_lower(hy[c][r]);   // hy(i, j, k-1)
+                    if (c > 0) {
+                        const T lx = lane_of(hx[c > 0 ? c - 1 : 0][r], WL - 1);
+                        const T ly = lane_of(hy[c > 0 ? c - 1 : 0][r], WL - 1);
+                        hxk = lane == 0 ? lx : hxk;
+                        hyk = lane == 0 ? ly : hyk;
+                    }
+                    const int jc = jr[r];
+                    if (j0 - 1 + r < jend && col_own[c]) {
+                        const bool in = i < g.nx && jc < g.ny && kk < g.nz;
+                        const bool j_ok = g.py || jc >= 1, k_ok = g.pz || kk >= 1;
+                        const bool on[3] = {in && j_ok && k_ok, in && i_ok && k_ok, in && i_ok && j_ok};
+                        T ev[3] = {cx[c][r], cy[c][r], cz[c][r]};
+                        if (src && src_lane[c]) {
+#pragma unroll
+                            for (int q = 0; q < 3; q++)
+                                if ((a.src_pre >> q) & 1)
+                                    ev[q] = nls == a.n_srcs
+                                                ? apply_lsrc<T>(s_src, nls, ev[q], q, i, jc, kk, g.nx, g.ny, g.nz, 0)
+                                                : apply_sources<T>(a.srcs, a.n_srcs, ev[q], q, i, jc, kk, g.nx, g.ny,
+                                                                   g.nz, step, 0);
+                        }
+                        T dA[3] = {hz[c][r] - hz[c][r - 1], hx[c][r] - hxk, hy[c][r] - py[c][r]};
+                        T dB[3] = {hy[c][r] - hyk, hz[c][r] - pz[c][r], hx[c][r] - hx[c][r - 1]};
+                        if (zce && zs[e_] && zon[e_] && (on[0] || on[1])) {
+                            const bool slab = zcq[e_] >= 0;
+                            const T km = zprof(e_, 3), bm = zprof(e_, 4), am = zprof(e_, 5);
+                            T px_, py_;
+                            if (on[0]) dB[0] = zterm<T>(dB[0], km, bm, am, slab, ze0[e_][r], px_, dx);
+                            if (on[1]) dA[1] = zterm<T>(dA[1], km, bm, am, slab, ze1[e_][r], py_, dx);
+                            if (slab) {
+                                const unsigned zl = static_cast<unsigned>(zcq[e_]) * ES, zb = zrow(i, jc);
+                                if (on[0]) bst(px_, RZ, zl, zb + 4 * zsb);
+                                if (on[1]) bst(py_, RZ, zl, zb + 5 * zsb);
+                            }
+                        }
+                        if constexpr (!TAGS) {
+#pragma unroll
+                            for (int q = 0; q < 3; q++) {
+                                const T n = T(1) * ev[q] + vac * (dA[q] - dB[q]);
+                                ev[q] = on[q] ? n : ev[q];
+                            }
+                        } else {
+                            // dispersive edges (split ADE: plain part here)
+                            // and plain edges take the update; tensor edges
+                            // keep their value for the tensor kernel, as in
+                            // e_flagged3 (row mode runs only where no ADE
+                            // update is left to the fused kernel)
+                            const Coefs<T> kf = entry_coefs<T>(a.entries, s_ca, s_cb, s_fl, ncache, tg[c][r]);
+                            const uint32_t fl = kf.fl & ~(RF_EDGE_ADE * 0x010101u);
+#pragma unroll
+                            for (int q = 0; q < 3; q++) {
+                                const T n = kf.ca[q] * ev[q] + kf.cb[q] * (dA[q] - dB[q]);
+                                const bool upd = on[q] && !(((fl >> (8 * q)) & 0xffu) & RF_EDGE_TENSOR);
+                                ev[q] = upd ? n : ev[q];
+                            }
+                        }
+                        if (src && src_lane[c]) {
+#pragma unroll
+                            for (int q = 0; q < 3; q++)
+                                if ((a.src_post >> q) & 1)
+                                    ev[q] = nls == a.n_srcs
+                                                ? apply_lsrc<T>(s_src, nls, ev[q], q, i, jc, kk, g.nx, g.ny, g.nz, 1)
+                                                : apply_sources<T>(a.srcs, a.n_srcs, ev[q], q, i, jc, kk, g.nx, g.ny,
+                                                                   g.nz, step, 1);
+                        }
+                        bst(ev[0], RB, vk[c], pb + rowb[r]);
+                        bst(ev[1], RB, vk[c], fsb + pb + rowb[r]);
+                        bst(ev[2], RB, vk[c], 2 * fsb + pb + rowb[r]);
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+#pragma unroll
+            for (int r = 1; r <= R; r++) {
+                py[c][r] = hy[c][r];
+                pz[c][r] = hz[c][r];
+            }
+#pragma unroll
+            for (int r = 0; r < R + 2; r++) {
+                cx[c][r] = nx[c][r];
+                cy[c][r] = ny[c][r];
+                cz[c][r] = nz[c][r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R + 2; r++) {
+            bx[r] = nbx[r];
+            if (r <= R) by[r] = nby[r];
+        }
+        i = first ? i_begin : i + 1;
+    }
+}
+
+// Row-mode step: items are (row tile, x chunk), row tile fastest, taken
+// from the XCD-major queue of k_leapfrog (independent wavefronts).
+// (at most 168 VGPRs: three wavefronts per SIMD; four would spill)
+template <typename TagT, int R>
+__global__ void __launch_bounds__(WL *WPB) __attribute__((amdgpu_waves_per_eu(3)))
+k_leap_rows(rf_leap_t a) {
+    constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    const LeapLds<float> lv = leap_lds_view<float>(s_raw, a.n_srcs, a.n_adec, a.ncache);
+    const long long step = *a.d_step;
+    leap_stage<float, TAGS>(a, lv, step, threadIdx.x, blockDim.x);
+    // z CPML profiles (kinv_h b_h a_h kinv_m b_m a_m) of the columns of
+    // the first and last chunk: 1, 0, 0 inside the plain range
+    __shared__ float s_zk[2 * 6 * WL];
+    const int nz = a.g.n[2];
+    for (int t = threadIdx.x; t < 2 * 6 * WL; t += blockDim.x) {
+        const int l = t % WL, p = (t / WL) % 6, e = t / (6 * WL);
+        const int q = e == 0 ? l : min(64 * (ROW_NCH - 1) + l, nz);
+        const bool on = q < a.zplo || q >= a.zphi;
+        const int qq = p < 3 ? min(q, nz - 1) : q;   // half-grid profiles have nz entries
+        s_zk[t] = on ? static_cast<const float *>(a.zk[p])[qq] : (p % 3 == 0 ? 1.f : 0.f);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (WL - 1);
+    const int xcd = blockIdx.x % N_XCD;
+    if (blockIdx.x == 0 && threadIdx.x < N_XCD) a.ctr[(1 - a.qpar) * N_XCD + threadIdx.x] = 0u;
+    const int item_lo = (int)(((long long)a.n_items * xcd) / N_XCD);
+    const int item_hi = (int)(((long long)a.n_items * (xcd + 1)) / N_XCD);
+    unsigned *my_ctr = a.ctr + a.qpar * N_XCD + xcd;
+    for (;;) {
+        unsigned ticket = 0;
+        if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
+        const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
+        if (item >= item_hi) break;
+        leap_row<TagT, R>(a, lv, s_zk, step, item % a.ntj, item / a.ntj, lane);
+    }
+}
+
 // ------------------------------------------------------------- shell
 // The nodes around the fused box (CPML slabs and their one-node margin)
 // are updated one node per thread, H then E, with the expressions of the
@@ -1511,6 +1919,38 @@ int leap_capacity(int prec, const rf_leap_t *a) {
     return a->rows == 4 ? capacity_rows<float, 4>(a) : capacity_rows<float, 8>(a);
 }
 
+// Row mode: the kernel of a's tag width and rows.
+template <int R>
+const void *zrow_fn_rows(const rf_leap_t *a) {
+    if (!a->tag) return reinterpret_cast<const void *>(k_leap_rows<NoTag, R>);
+    switch (a->tag_bytes) {
+    case 1: return reinterpret_cast<const void *>(k_leap_rows<uint8_t, R>);
+    case 2: return reinterpret_cast<const void *>(k_leap_rows<uint16_t, R>);
+    case 4: return reinterpret_cast<const void *>(k_leap_rows<uint32_t, R>);
+    }
+    return nullptr;
+}
+
+const void *zrow_fn(const rf_leap_t *a) { return a->rows == 1 ? zrow_fn_rows<1>(a) : nullptr; }
+
+int zrow_capacity(const rf_leap_t *a) {
+    const void *fn = zrow_fn(a);
+    int nb = 0, cus = 0, dev = 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, WL * WPB, leap_lds<float>(a)) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || nb <= 0)
+        return 0;
+    return nb * cus * WPB;
+}
+
+int launch_zrow(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
+    const void *fn = zrow_fn(a);
+    if (!fn) return -1;
+    void *args[] = {const_cast<rf_leap_t *>(a)};
+    if (hipLaunchKernel(fn, dim3(blocks), dim3(WL * WPB), args, leap_lds<float>(a), st) != hipSuccess) return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <typename T, int R>
 int launch_rows(const rf_leap_t *a, unsigned blocks, hipStream_t st) {
     if (!a->tag) return launch_tag<T, NoTag, R>(a, blocks, st);
@@ -1571,6 +2011,19 @@ int shell_e_packed_tag(const rf_shell_job_t *jobs, const unsigned short *bj, int
 
 }  // namespace
 
+extern "C" const char *rf_leap_zrow_refusal(const rf_leap_t *a) {
+    const int nz = a->g.n[2];
+    if (!a->zc || ((a->g.per >> 2) & 1) || a->ib0[2] != 0 || a->ib1[2] != nz + 1)
+        return "the box does not span every z column with the z CPML inside it";
+    if (nz <= 64 * (ROW_NCH - 1) || nz > 64 * ROW_NCH) return "nz is not in (192, 256]";
+    if (a->zplo > 64 || a->zlo > 64 || a->zphi < 64 * (ROW_NCH - 1) || a->zhi < 64 * (ROW_NCH - 1))
+        return "a z CPML slab reaches past the first or last 64 columns";
+    if (a->tag && a->n_slots > 0 && !a->ade_split) return "dispersive edges are updated in the fused kernel";
+    if (nz == 64 * ROW_NCH && a->n_srcs > 0 && a->src_box[5] >= nz) return "a source may touch column nz";
+    if (a->coop || a->zsplit) return "cooperative blocks or a z split were requested";
+    return nullptr;
+}
+
 extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves) {
     for (int q = 0; q < 3; q++) {
         const int top = ((a->g.per >> q) & 1) ? a->g.n[q] : a->g.n[q] + 1;
@@ -1581,8 +2034,10 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
     const long long kn = a->ib1[2] - a->ib0[2];
     if (a->kbrk < a->ib0[2] || a->kbrk > a->ib1[2]) return -1;
     if (rows <= 0) rows = 4;
-    if (prec == 0 ? !(rows == 2 || rows == 4) : !(rows == 4 || rows == 8)) return -1;
+    if (prec == 0 ? !(rows == 2 || rows == 4) : !(rows == 1 || rows == 4 || rows == 8)) return -1;
     a->rows = rows;
+    a->zrow = prec == 1 && rows == 1;
+    if (a->zrow && rf_leap_zrow_refusal(a)) return -1;
     // up to 40 KiB of LDS per block for the ADE sets (at most 16) and the
     // material table (plus the staged sources)
     const size_t ade_sz = prec == 0 ? sizeof(rf_ade64_t) : sizeof(rf_ade32_t);
@@ -1594,9 +2049,39 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
         a->chunk = 1;
         return 0;
     }
+    a->ntj = (int)((jn + rows - 1) / rows);
+    if (a->zrow) {   // one tile of every z column per row tile, one part
+        a->ntk = a->ntk0 = 1;
+        a->n_parts = 1;
+        a->ptk[0][0] = 0;
+        a->ptk[0][1] = 1;
+        a->ptk[0][2] = a->ptk[0][3] = 0;
+        a->pzc[0] = 1;
+        a->pgz[0] = a->pgy[0] = 0;
+        a->gk = 0;
+        int resident = target_waves > 0 ? target_waves : zrow_capacity(a);
+        if (resident <= 0) resident = 4096;
+        if (chunk <= 0) {
+            // two items per resident wavefront: every wavefront of the
+            // device has an item in both rounds (one round leaves more
+            // planes at the tail; the 512 x 512 x 256 dispersive stack
+            // measured 1109 us with one, 1074 us with two, 1082 us with four)
+            long long chunks = (2LL * resident + a->ntj - 1) / a->ntj;
+            if (chunks < 1) chunks = 1;
+            chunk = (int)((planes + chunks - 1) / chunks);
+        }
+        if (chunk > planes) chunk = (int)planes;
+        a->chunk = chunk;
+        const long long items = (long long)a->ntj * ((planes + chunk - 1) / chunk);
+        if (items > (1LL << 30)) return -1;
+        a->pitems[0] = (int)items;
+        long long blocks = (resident + WPB - 1) / WPB;
+        if (blocks > (items + WPB - 1) / WPB) blocks = (items + WPB - 1) / WPB;
+        a->pblocks[0] = (int)((blocks + N_XCD - 1) / N_XCD * N_XCD);
+        return 0;
+    }
     a->ntk0 = (a->kbrk - a->ib0[2] + KW - 1) / KW;
     a->ntk = a->ntk0 + (a->ib1[2] - a->kbrk + KW - 1) / KW;
-    a->ntj = (int)((jn + rows - 1) / rows);
     // z tiles whose halo lane or owned lanes lie in the z CPML (outside the
     // plain range) need the z CPML code; with the slabs at both ends of the
     // z range they are a prefix [0, zt0) and a suffix [zt1, ntk)
@@ -1732,7 +2217,10 @@ extern "C" int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s) 
         b.ctr = a->ctr + 16 * p;
         const unsigned blocks = static_cast<unsigned>(a->pblocks[p]);
         int rc = -1;
-        if (prec == 0) {
+        if (a->zrow) {
+            if (prec != 1 || a->n_parts != 1 || b.gz != 0) return -1;
+            rc = launch_zrow(&b, blocks, st);
+        } else if (prec == 0) {
             switch (a->rows) {
             case 2: rc = launch_rows<double, 2>(&b, blocks, st); break;
             case 4: rc = launch_rows<double, 4>(&b, blocks, st); break;

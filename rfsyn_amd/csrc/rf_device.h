@@ -249,6 +249,7 @@ typedef struct {
     int qpar;
     /* tiling, filled by rf_leap_plan */
     int rows, chunk;            /* rows per item, x planes per item */
+    int zrow;                   /* row mode: each item spans every z column */
     int ntk, ntk0, ntj;         /* tiles along z (ntk0 below kbrk) and y */
     int ncache, n_adec;         /* material entries and ADE sets staged in LDS */
     /* the step runs in n_parts launches over disjoint sets of z tiles:
@@ -278,9 +279,17 @@ typedef struct {
 /* Tiling of the fused step over a->ib0..ib1; rows: 2 or 4 (FP64), 4 or 8
  * (FP32), 0 = default; chunk 0 = about twelve work items per resident
  * wavefront, at least 8 planes each (target_waves > 0 overrides the
- * residency the device reports).  0 on success, -1 for an unsupported
+ * residency the device reports).  FP32 rows 1 selects row mode: an
+ * item is one row of every z column (lane l of chunk c holds
+ * column 64 c + l), for a box spanning z columns 0..nz with the z CPML
+ * inside it, 192 < nz <= 256, both z CPML slabs within the first and last
+ * 64 columns, dispersive edges left to the ADE pass, independent
+ * wavefronts, and (nz = 256) no source on column nz; chunk 0 = about two
+ * items per resident wavefront.  0 on success, -1 for an unsupported
  * request. */
 int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves);
+/* Why row mode cannot run on a's box and options (NULL: it can). */
+const char *rf_leap_zrow_refusal(const rf_leap_t *a);
 /* tag NULL = vacuum (vac_ca); an empty box launches nothing. */
 int rf_launch_leapfrog(int prec, const rf_leap_t *a, rf_stream_t *s);
 

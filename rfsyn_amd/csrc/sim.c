@@ -115,6 +115,7 @@ struct fdtd_sim {
     int parity;
     void *f2[6];
     int leap_rows, leap_chunk, leap_waves;   /* tiling requests (0: automatic) */
+    int plan_rows;           /* rows per item of the last planned fused step */
     int leap_zsplit;         /* z CPML tiles in a launch of their own */
     int leap_coop;           /* cooperative blocks (rf_leap_t.coop) */
     rf_leap_src_t *d_srcs;   /* device source table of the fused step */
@@ -816,9 +817,12 @@ static int fill_leap(fdtd_sim_t *s, rf_leap_t *out, const rf_mat_t **mout, int a
     a.coop = s->leap_coop;
     a.ade_split = ade_pass_box(s, alo, ahi);
     if (rf_leap_plan(s->prec, &a, s->leap_rows, s->leap_chunk, s->leap_waves)) {
-        rf_set_error("fused step: unsupported tiling (rows %d, chunk %d)", s->leap_rows, s->leap_chunk);
+        const char *why = s->prec == 1 && s->leap_rows == 1 ? rf_leap_zrow_refusal(&a) : NULL;
+        rf_set_error("fused step: unsupported tiling (rows %d, chunk %d)%s%s", s->leap_rows, s->leap_chunk,
+                     why ? ": " : "", why ? why : "");
         return -1;
     }
+    s->plan_rows = a.rows;
     *out = a;
     *mout = m;
     return 0;
@@ -1087,7 +1091,10 @@ static int run_enqueue(fdtd_sim_t *s, size_t *left, size_t max) {
         if (s->profiling && s->ev_used + 6 > s->n_ev && prof_collect(s) != 0) return -1;
         if (!s->profiling && s->pending == 0 && *left >= (size_t)s->K && budget >= (size_t)s->K) {
             if (!s->graphs[s->parity] && capture_block(s) != 0) {
-                rf_set_error("graph capture: %s", rf_error_string());
+                /* a HIP error, else the message the failing step left */
+                char inner[200];
+                snprintf(inner, sizeof inner, "%s", rf_error_string()[0] ? rf_error_string() : fdtd_last_error());
+                rf_set_error("graph capture: %s", inner);
                 return -1;
             }
             if (rf_graph_launch(s->graphs[s->parity], s->st)) return -1;
@@ -1187,19 +1194,24 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]) {
     if (!s->fused) return 0;   /* nothing to tune on the two-kernel path */
     rf_event_t *e0 = rf_event_create(), *e1 = rf_event_create();
     int rc = (e0 && e1) ? 0 : -1;
-    const int rows32[2] = {4, 8}, rows64[2] = {2, 4};
+    /* FP32 rows 1 is row mode (independent wavefronts, no z split), tried
+     * with its own chunk choice and skipped where the box does not allow it */
+    const int rows32[3] = {4, 8, 1}, rows64[2] = {2, 4};
     const int chunks[5] = {0, 6, 10, 16, 32};
     const int *rows = s->prec == 0 ? rows64 : rows32;
+    const int n_rows = s->prec == 0 ? 2 : 3;
     /* independent; cooperative blocks of 2 or 4 row tiles, in lockstep or
      * with a slack of 1 plane */
     const int coops[5] = {0, 2, 4, 16 + 2, 16 + 4};
     const int r0 = s->leap_rows, c0 = s->leap_chunk, z0 = s->leap_zsplit, g0 = s->leap_coop;
     int best[4] = {r0, c0, z0, g0};
     float best_ms = -1.f;
-    for (int ri = 0; ri < 2 && !rc; ri++)
+    for (int ri = 0; ri < n_rows && !rc; ri++)
         for (int ci = 0; ci < 5 && !rc; ci++)
             for (int zi = 0; zi < (s->zc ? 2 : 1) && !rc; zi++)
             for (int gi = 0; gi < 5 && !rc; gi++) {
+                const int zrow = s->prec == 1 && rows[ri] == 1;
+                if (zrow && (ci > 0 || zi > 0 || gi > 0)) continue;
                 s->leap_rows = rows[ri];
                 s->leap_chunk = chunks[ci];
                 s->leap_zsplit = zi;
@@ -1207,6 +1219,7 @@ int fdtd_sim_autotune(fdtd_sim_t *s, int launches, int chosen[4]) {
                 rf_leap_t a;
                 const rf_mat_t *m;
                 int alo[3], ahi[3];
+                if (zrow && fill_leap(s, &a, &m, alo, ahi)) continue;   /* not available for this box */
                 for (int w = 0; w <= launches && !rc; w++) {   /* one warm-up launch */
                     if (w == 1 && rf_event_record(e0, s->st)) rc = -1;
                     if (!rc && fill_leap(s, &a, &m, alo, ahi)) rc = -1;
@@ -1482,13 +1495,15 @@ static void swap_state(fdtd_sim_t *s) {
 /* The fused kernel's descriptor of s for a packed launch: one part (no z
  * split), independent wavefronts, tables within BATCH_LDS_WAVE. */
 static int packed_leap(fdtd_sim_t *s, rf_leap_t *a, int alo[3], int ahi[3], int *ade) {
-    const int zs = s->leap_zsplit, co = s->leap_coop;
+    const int zs = s->leap_zsplit, co = s->leap_coop, rows = s->leap_rows;
     const rf_mat_t *m;
     s->leap_zsplit = 0;
     s->leap_coop = 0;
+    if (s->prec == 1 && rows == 1) s->leap_rows = 0;   /* row mode has no packed variant */
     const int rc = fill_leap(s, a, &m, alo, ahi);
     s->leap_zsplit = zs;
     s->leap_coop = co;
+    s->leap_rows = rows;
     if (rc) return -1;
     if (a->n_parts != 1) return -1;
     for (int q = 0; q < 4; q++) a->tkr[q] = a->ptk[0][q];
@@ -2082,6 +2097,8 @@ int fdtd_sim_dft_block(const fdtd_sim_t *s) {
     if (!s) return -1;
     return s->started ? s->K : dft_block_of(s, want_fused(s));
 }
+
+int fdtd_sim_leap_rows(const fdtd_sim_t *s) { return s ? s->plan_rows : -1; }
 
 int fdtd_sim_streams(const fdtd_sim_t *s) {
     if (!s) return -1;

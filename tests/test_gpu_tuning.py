@@ -49,7 +49,7 @@ def test_tuned_run_is_bitwise_untuned(gpu, precision, scene):
     a, _ = scene(gpu, precision, False)
     b, chosen = scene(gpu, precision, True)
     if chosen is not None:
-        assert chosen[0] in ((4, 8) if precision == capi.FDTD_FP32 else (2, 4))
+        assert chosen[0] in ((1, 4, 8) if precision == capi.FDTD_FP32 else (2, 4))
     if "td" in a:
         assert np.max(np.abs(a["td"])) > 0
         assert same_values(a["td"], b["td"])
