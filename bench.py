@@ -575,17 +575,17 @@ def main():
     for prec in (wl.headline, capi.FDTD_FP32 if wl.headline == capi.FDTD_FP64 else capi.FDTD_FP64):
         tag = "fp32" if prec == capi.FDTD_FP32 else "fp64"
         fixed = args.fp32_rows and prec == capi.FDTD_FP32
-        if fixed:   # read by the library when the handle is created
+        if fixed:   # read by the library at the handle's first run
             os.environ["RFSYN_LEAP_ROWS"] = str(args.fp32_rows)
         run, pids, mons = wl.build(lib, prec, total)
-        if fixed:
-            del os.environ["RFSYN_LEAP_ROWS"]
         chosen = (C.c_int * 4)(args.fp32_rows if fixed else 0, 0, 0, 0)
         two = C.c_int(0)
         if not args.no_autotune and not fixed:
             assert lib.fdtd_sim_autotune(run.sim, 3, chosen) == 0, lib.fdtd_last_error()
             assert lib.fdtd_sim_select_path(run.sim, 3, C.byref(two)) == 0, lib.fdtd_last_error()
         run.run(args.warmup)
+        if fixed:
+            del os.environ["RFSYN_LEAP_ROWS"]
         run.set_profiling(True)
         t0 = time.perf_counter()
         run.run(args.steps)

@@ -151,7 +151,8 @@ size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 int fdtd_sim_dft_block(const fdtd_sim_t *s);
 /* Rows per work item of the fused step's last planned tiling: 1 is row
  * mode (FP32: an item spans every z column), 0 before the fused step was
- * planned; -1 for NULL.  RFSYN_LEAP_ROWS requests a value at creation. */
+ * planned; -1 for NULL.  RFSYN_LEAP_ROWS, read at the handle's first
+ * run, requests a value (fdtd_sim_autotune replaces it). */
 int fdtd_sim_leap_rows(const fdtd_sim_t *s);
 /* Diagnostics: streams the handle's next step uses (1, or 2 when the
  * CPML shell's H update runs on a second stream beside the fused kernel;
