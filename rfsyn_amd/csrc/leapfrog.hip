@@ -120,10 +120,14 @@ __device__ __forceinline__ const rf_leap_cold_t *cold_view(const rf_leap_cold_t 
 // `after`, in table order (point sources, then plane sources, as the
 // separate source kernels apply them).
 template <typename T>
-__device__ __forceinline__ T apply_sources(const rf_leap_src_t *tab, int n, T v, int c, int i, int j, int k,
+__device__ __forceinline__ T apply_sources(const rf_leap_src_t *tab_p, int n, T v, int c, int i, int j, int k,
                                         int gnx, int gny, int gnz, long long step, int after) {
+    // the table and the waveforms are device memory: global accesses, not
+    // FLAT ones (the table pointer may come from a device job table)
+    typedef const __attribute__((address_space(1))) rf_leap_src_t gsrc_t;
+    gsrc_t *tab = (gsrc_t *)tab_p;
     for (int s = 0; s < n; s++) {
-        const rf_leap_src_t &p = tab[s];
+        gsrc_t &p = tab[s];
         if (p.after != after || p.comp != c) continue;
         bool hit;
         if (p.plane) {
@@ -135,7 +139,7 @@ __device__ __forceinline__ T apply_sources(const rf_leap_src_t *tab, int n, T v,
             hit = p.i == i && p.j == j && p.k == k;
         }
         if (hit) {
-            const double w = p.wave[step];
+            const double w = ((const __attribute__((address_space(1))) double *)p.wave)[step];
             v = p.hard ? static_cast<T>(w) : static_cast<T>(static_cast<double>(v) + w);
         }
     }
@@ -179,10 +183,26 @@ struct Coefs {
     uint32_t fl;
 };
 
+// When every active lane's entry is cached (a wavefront-uniform test) the
+// reads are LDS reads only: one merged path for both sources compiles to
+// FLAT loads, which count against the vector memory counter too, so the
+// wait for the coefficients would also wait for every field load the
+// wavefront has in flight.
 template <typename T>
 __device__ __forceinline__ Coefs<T> entry_coefs(const void *entries, const T *s_ca, const T *s_cb,
                                                 const uint32_t *s_fl, int ncache, unsigned tag) {
     Coefs<T> o;
+    if (__ballot(static_cast<int>(tag) >= ncache) == 0) {
+        typedef __attribute__((address_space(3))) const T lds_t;
+        typedef __attribute__((address_space(3))) const uint32_t lds_u;
+        o.fl = ((lds_u *)s_fl)[tag];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            o.ca[c] = ((lds_t *)s_ca)[3 * tag + c];
+            o.cb[c] = ((lds_t *)s_cb)[3 * tag + c];
+        }
+        return o;
+    }
     if (static_cast<int>(tag) < ncache) {
         o.fl = s_fl[tag];
 #pragma unroll
@@ -1279,6 +1299,13 @@ k_leap_rows(rf_leap_t a) {
     }
 }
 
+// Global-memory views: loads and stores through a pointer the compiler
+// cannot prove global (one read from a device table) compile to FLAT
+// instructions, which also count against the LDS counter; every array the
+// shell kernels touch is device memory, so they access it through these.
+template <typename T> using gmem_c = const __attribute__((address_space(1))) T;
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+
 // ------------------------------------------------------------- shell
 // The nodes around the fused box (CPML slabs and their one-node margin)
 // are updated one node per thread, H then E, with the expressions of the
@@ -1331,13 +1358,13 @@ __device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, in
 template <typename T>
 __device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, bool h, int i, int j, int k,
                                       const rf_zpsi_t &z, int rd, int wr, T dx) {
-    const T *kinv = static_cast<const T *>(h ? c.kinv_h[2] : c.kinv_m[2]);
-    const T *bb = static_cast<const T *>(h ? c.b_h[2] : c.b_m[2]);
-    const T *aa = static_cast<const T *>(h ? c.a_h[2] : c.a_m[2]);
+    gmem_c<T> *kinv = (gmem_c<T> *)(h ? c.kinv_h[2] : c.kinv_m[2]);
+    gmem_c<T> *bb = (gmem_c<T> *)(h ? c.b_h[2] : c.b_m[2]);
+    gmem_c<T> *aa = (gmem_c<T> *)(h ? c.a_h[2] : c.a_m[2]);
     const int cq = slab_coord(c, 2, k);
     T psi = T(0);
     if (cq >= 0) {
-        T *zp = static_cast<T *>(z.zpsi);
+        gmem<T> *zp = (gmem<T> *)z.zpsi;
         const long long zo = ((long long)i * (g.ny + 1) + j) * 6 * z.zlen + cq;
         psi = bb[k] * zp[zo + (long long)rd * z.zlen] + aa[k] * diff;
         zp[zo + (long long)wr * z.zlen] = psi;
@@ -1348,11 +1375,14 @@ __device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, 
 // H^{n+1/2} of shell node t.
 template <typename T, bool CPML>
 __device__ __forceinline__ void shell_h_node(const rf_geom_t &gg, const rf_shell_t &sh, long long t,
-                                             const T *__restrict__ ex, const T *__restrict__ ey,
-                                             const T *__restrict__ ez, const T *__restrict__ hxa,
-                                             const T *__restrict__ hya, const T *__restrict__ hza,
-                                             T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb,
-                                             T dx, const rf_cpml_t &cp, const rf_zpsi_t &zs) {
+                                             const T *__restrict__ ex_p, const T *__restrict__ ey_p,
+                                             const T *__restrict__ ez_p, const T *__restrict__ hxa_p,
+                                             const T *__restrict__ hya_p, const T *__restrict__ hza_p,
+                                             T *__restrict__ hxb_p, T *__restrict__ hyb_p, T *__restrict__ hzb_p,
+                                             T cb, T dx, const rf_cpml_t &cp, const rf_zpsi_t &zs) {
+    gmem_c<T> *const ex = (gmem_c<T> *)ex_p, *const ey = (gmem_c<T> *)ey_p, *const ez = (gmem_c<T> *)ez_p;
+    gmem_c<T> *const hxa = (gmem_c<T> *)hxa_p, *const hya = (gmem_c<T> *)hya_p, *const hza = (gmem_c<T> *)hza_p;
+    gmem<T> *const hxb = (gmem<T> *)hxb_p, *const hyb = (gmem<T> *)hyb_p, *const hzb = (gmem<T> *)hzb_p;
     const Geo g = geo_of(gg);
     int i, j, k;
     if (!shell_node(sh, t, i, j, k)) return;
@@ -1455,28 +1485,50 @@ __device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_
     if constexpr (std::is_same<TagT, NoTag>::value) {
         return T(1) * e_old + static_cast<T>(m.vac_ca) * (dA - dB);
     } else {
-        const Entry &en = static_cast<int>(tag) < n_lds ? s_ent[tag] : static_cast<const Entry *>(m.entries)[tag];
-        const uint32_t fl = en.flags[c];
+        // LDS reads only when every active lane's entry is staged (one
+        // merged path for LDS and HBM entries compiles to FLAT loads, which
+        // also hold the vector memory counter)
+        uint32_t fl;
+        T ca, cb;
+        int aid;
+        if (__ballot(static_cast<int>(tag) >= n_lds) == 0) {
+            typedef __attribute__((address_space(3))) const Entry lds_entry;
+            const lds_entry &en = ((lds_entry *)s_ent)[tag];
+            fl = en.flags[c];
+            ca = en.Ca[c];
+            cb = en.Cb[c];
+            aid = en.ade[c];
+        } else {
+            const Entry &en =
+                static_cast<int>(tag) < n_lds ? s_ent[tag] : static_cast<const Entry *>(m.entries)[tag];
+            fl = en.flags[c];
+            ca = en.Ca[c];
+            cb = en.Cb[c];
+            aid = en.ade[c];
+        }
         if (fl & RF_EDGE_TENSOR) return e_old;
         if (fl & RF_EDGE_ADE) {
-            const Ade &ad = static_cast<const Ade *>(m.ade)[en.ade[c]];
-            return ade_edge<T>(ad, en.Ca[c], en.Cb[c], e_old, dA, dB, static_cast<T *>(m.J),
-                               static_cast<T *>(m.P), 3 * ad.st, ade_off(ad, c, i, j, k));
+            const Ade &ad = static_cast<const Ade *>(m.ade)[aid];
+            return ade_edge<T>(ad, ca, cb, e_old, dA, dB, static_cast<T *>(m.J), static_cast<T *>(m.P), 3 * ad.st,
+                               ade_off(ad, c, i, j, k));
         }
-        return en.Ca[c] * e_old + en.Cb[c] * (dA - dB);
+        return ca * e_old + cb * (dA - dB);
     }
 }
 
 // Sources and E^{n+1} of shell node t.
 template <typename T, typename TagT, bool CPML>
 __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell_t &sh, long long t,
-                                             const T *__restrict__ exa, const T *__restrict__ eya,
-                                             const T *__restrict__ eza, const T *__restrict__ hx,
-                                             const T *__restrict__ hy, const T *__restrict__ hz,
-                                             T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx,
-                                             const rf_cpml_t &cp, const rf_zpsi_t &zs, const rf_mat_t &m,
+                                             const T *__restrict__ exa_p, const T *__restrict__ eya_p,
+                                             const T *__restrict__ eza_p, const T *__restrict__ hx_p,
+                                             const T *__restrict__ hy_p, const T *__restrict__ hz_p,
+                                             T *__restrict__ exb_p, T *__restrict__ eyb_p, T *__restrict__ ezb_p,
+                                             T dx, const rf_cpml_t &cp, const rf_zpsi_t &zs, const rf_mat_t &m,
                                              const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
                                              const typename EntryOf<T>::type *s_ent, int n_lds) {
+    gmem_c<T> *const exa = (gmem_c<T> *)exa_p, *const eya = (gmem_c<T> *)eya_p, *const eza = (gmem_c<T> *)eza_p;
+    gmem_c<T> *const hx = (gmem_c<T> *)hx_p, *const hy = (gmem_c<T> *)hy_p, *const hz = (gmem_c<T> *)hz_p;
+    gmem<T> *const exb = (gmem<T> *)exb_p, *const eyb = (gmem<T> *)eyb_p, *const ezb = (gmem<T> *)ezb_p;
     const Geo g = geo_of(gg);
     int i, j, k;
     if (!shell_node(sh, t, i, j, k)) return;
@@ -1490,7 +1542,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
     const long long di = (i == 0) ? (long long)(g.nx - 1) * g.sx : -g.sx;
     const long long dj = (j == 0) ? (long long)(g.ny - 1) * g.sy : -g.sy;
     const long long dk = (k == 0) ? (long long)(g.nz - 1) : -1;
-    const long long step = *d_step;
+    const long long step = *(gmem_c<long long> *)d_step;
     T e0 = exa[idx], e1 = eya[idx], e2 = eza[idx];
     if (n_srcs) {
         e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 0);

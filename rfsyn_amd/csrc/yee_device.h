@@ -46,15 +46,20 @@ __device__ __forceinline__ long long slab_index(const rf_cpml_t &c, const Geo &g
 template <typename T>
 __device__ __forceinline__ T stretch(T diff, const rf_cpml_t &c, const Geo &g, bool h, int axis,
                                      int q, T *psi_arr, int i, int j, int k, T dx) {
-    const T *kinv = static_cast<const T *>(h ? c.kinv_h[axis] : c.kinv_m[axis]);
-    const T *bb = static_cast<const T *>(h ? c.b_h[axis] : c.b_m[axis]);
-    const T *aa = static_cast<const T *>(h ? c.a_h[axis] : c.a_m[axis]);
+    // device memory, accessed as global memory (a pointer read from a
+    // device table would otherwise be accessed with FLAT instructions)
+    typedef const __attribute__((address_space(1))) T gc_t;
+    typedef __attribute__((address_space(1))) T g_t;
+    gc_t *kinv = (gc_t *)(h ? c.kinv_h[axis] : c.kinv_m[axis]);
+    gc_t *bb = (gc_t *)(h ? c.b_h[axis] : c.b_m[axis]);
+    gc_t *aa = (gc_t *)(h ? c.a_h[axis] : c.a_m[axis]);
     int cq = slab_coord(c, axis, q);
     T psi = T(0);
     if (cq >= 0) {
         long long s = slab_index(c, g, axis, cq, i, j, k);
-        psi = bb[q] * psi_arr[s] + aa[q] * diff;
-        psi_arr[s] = psi;
+        g_t *pa = (g_t *)psi_arr;
+        psi = bb[q] * pa[s] + aa[q] * diff;
+        pa[s] = psi;
     }
     return diff * kinv[q] + dx * psi;
 }
@@ -67,7 +72,7 @@ struct NoTag { };
 
 template <typename TagT>
 __device__ __forceinline__ unsigned load_tag(const void *p, long long idx) {
-    return static_cast<unsigned>(static_cast<const TagT *>(p)[idx]);
+    return static_cast<unsigned>(((const __attribute__((address_space(1))) TagT *)p)[idx]);
 }
 
 // Element t of the plane captures cs (capture q takes the elements
