@@ -1750,8 +1750,10 @@ __device__ __forceinline__ void ade_pass_block(const rf_geom_t &gg, int lo0, int
     const Geo g = geo_of(gg);
     const unsigned stride = n_blocks * blockDim.x;
     const unsigned t0 = blk * blockDim.x + threadIdx.x;
-    const T *__restrict__ J = static_cast<const T *>(m.J);
-    const T *__restrict__ P = static_cast<const T *>(m.P);
+    // device memory, accessed as global memory (the packed pass reads
+    // these pointers from its job table)
+    gmem_c<T> *const J = (gmem_c<T> *)m.J;
+    gmem_c<T> *const P = (gmem_c<T> *)m.P;
     long long idx[ADE_NPT];
     unsigned tag[ADE_NPT];
     bool node[ADE_NPT];
@@ -1797,8 +1799,8 @@ __device__ __forceinline__ void ade_pass_block(const rf_geom_t &gg, int lo0, int
     }
     if (!any) return;
     // the loads of all dispersive edges of the nodes, then the updates
-    const T *ea_[3] = {exa, eya, eza};
-    T *eb[3] = {exb, eyb, ezb};
+    gmem_c<T> *ea_[3] = {(gmem_c<T> *)exa, (gmem_c<T> *)eya, (gmem_c<T> *)eza};
+    gmem<T> *eb[3] = {(gmem<T> *)exb, (gmem<T> *)eyb, (gmem<T> *)ezb};
     T eo[ADE_NPT][3], et[ADE_NPT][3], jv[ADE_NPT][3][NS], pv[ADE_NPT][3][NS];
     long long off[ADE_NPT][3];
 #pragma unroll
@@ -1822,8 +1824,8 @@ __device__ __forceinline__ void ade_pass_block(const rf_geom_t &gg, int lo0, int
                 if (ad.has_p[s]) pv[u][c][s] = P[o];
             }
         }
-    T *__restrict__ Jw = static_cast<T *>(m.J);
-    T *__restrict__ Pw = static_cast<T *>(m.P);
+    gmem<T> *const Jw = (gmem<T> *)m.J;
+    gmem<T> *const Pw = (gmem<T> *)m.P;
 #pragma unroll
     for (int u = 0; u < ADE_NPT; u++)
 #pragma unroll
