@@ -422,16 +422,18 @@ def bench_c4(args, lib):
     wall = time.perf_counter() - t0
     launches = batch.launches()
     batch.close()
-    os.environ["RFSYN_PACKED"] = "0"
-    run_many(lib, handles, args.warmup)
-    t1 = time.perf_counter()
-    run_many(lib, handles, args.steps)
-    wall_streams = time.perf_counter() - t1
-    os.environ.pop("RFSYN_PACKED")
-    one = time.perf_counter()
-    for r in handles[:8]:   # the same 8 scenes one after another, for the batching gain
-        r.run(args.steps)
-    one = time.perf_counter() - one
+    wall_streams = one = None
+    if not args.c4_packed_only:
+        os.environ["RFSYN_PACKED"] = "0"
+        run_many(lib, handles, args.warmup)
+        t1 = time.perf_counter()
+        run_many(lib, handles, args.steps)
+        wall_streams = time.perf_counter() - t1
+        os.environ.pop("RFSYN_PACKED")
+        one = time.perf_counter()
+        for r in handles[:8]:   # the same 8 scenes one after another, for the batching gain
+            r.run(args.steps)
+        one = time.perf_counter() - one
     cells8 = sum(sp["n"][0] * sp["n"][1] * sp["n"][2] for sp in specs[:8])
     slab = sum(slab_cells(sp["n"], (10, 10, 10)) for sp in specs)
     byt = step_bytes(cells, slab, 4, 1)
@@ -451,8 +453,8 @@ def bench_c4(args, lib):
                      "bytes_per_launch": byt},
         "packed_launches_per_step": {"leapfrog": launches[0], "shell": launches[1], "ade_pass": launches[2],
                                      "probes": 1},
-        "streams_per_scene_mcell_s": round(cells * args.steps / wall_streams / 1e6, 3),
-        "sequential_8_scenes_mcell_s": round(cells8 * args.steps / one / 1e6, 3),
+        "streams_per_scene_mcell_s": round(cells * args.steps / wall_streams / 1e6, 3) if wall_streams else None,
+        "sequential_8_scenes_mcell_s": round(cells8 * args.steps / one / 1e6, 3) if one else None,
         "autotuned": not args.no_autotune,
     }
     print(json.dumps(out))
@@ -543,6 +545,9 @@ def main():
     ap.add_argument("--c5-fdtd-steps", type=int, default=1024, help="c5: FDTD steps per label")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiling of the fused step instead of timing the candidates first")
+    ap.add_argument("--c4-packed-only", action="store_true",
+                    help="c4: time the packed batch only, without the one-stream-per-handle and sequential "
+                         "comparison runs")
     ap.add_argument("--fp32-rows", type=int, default=0,
                     help="c1-c3: run the FP32 fused step at this many rows per work item (1: row mode, every z "
                          "column per item; 4 or 8: 62-column tiles) instead of autotuning its tiling")
