@@ -117,6 +117,9 @@ fdtd_batch_t *fdtd_batch_create(fdtd_sim_t *const *sims, size_t n_sims);
 int fdtd_batch_run(fdtd_batch_t *b, size_t n_steps);
 void fdtd_batch_destroy(fdtd_batch_t *b);
 size_t fdtd_batch_size(const fdtd_batch_t *b);
+/* Per-scene failure flags: finite[q] = fdtd_sim_fields_finite of scene q
+ * (1 finite, 0 diverged) after the batch's queued steps.  0 on success. */
+int fdtd_batch_fields_finite(fdtd_batch_t *b, int *finite);
 /* Diagnostics after a run: packed leapfrog launches per step (one per
  * tiling variant), shell launches and split-ADE-pass launches per step
  * (any pointer may be NULL).  0, or -1 before the first run. */
@@ -149,6 +152,13 @@ int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 /* Steps per DFT block the handle runs with (see dft_block); -1 for NULL. */
 int fdtd_sim_dft_block(const fdtd_sim_t *s);
+/* Whether every value of the handle's current state (E and H on every
+ * node, the padding included) is finite: 1 yes, 0 no (a NaN or an
+ * infinity: the run has diverged), -1 on error.  One device scan of the
+ * state, about its size in HBM reads; waits for the handle's stream.  The
+ * reference's validators make the same check on the host (isfinite gates,
+ * e.g. validation_common.h:53). */
+int fdtd_sim_fields_finite(fdtd_sim_t *s);
 /* Rows per work item of the fused step's last planned tiling: 1 is row
  * mode (FP32: an item spans every z column), 0 before the fused step was
  * planned; -1 for NULL.  RFSYN_LEAP_ROWS, read at the handle's first

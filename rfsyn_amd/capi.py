@@ -234,6 +234,8 @@ EXTENSION_API = {
     "fdtd_sim_run_many": (C.c_int, [C.POINTER(_vp), _sz, _sz]),
     "fdtd_sim_dft_block": (C.c_int, [_vp]),
     "fdtd_sim_leap_rows": (C.c_int, [_vp]),
+    "fdtd_sim_fields_finite": (C.c_int, [_vp]),
+    "fdtd_batch_fields_finite": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fdtd_batch_create": (_vp, [C.POINTER(_vp), _sz]),
     "fdtd_batch_run": (C.c_int, [_vp, _sz]),
     "fdtd_batch_destroy": (None, [_vp]),

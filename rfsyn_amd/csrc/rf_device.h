@@ -578,6 +578,10 @@ int rf_launch_audit(int prec, const rf_geom_t *g, void *const fn[6], void *const
 int rf_launch_flux(int prec, const rf_geom_t *g, int axis, int index, void *const fn[6], void *const fp[6],
                    double *partials, int n_partials, double *acc, rf_stream_t *s);
 
+/* *flag |= 1 when any of the n values at v (prec 0 double, 1 float) is a
+ * NaN or an infinity; the caller clears *flag. */
+int rf_launch_nonfinite(int prec, const void *v, long long n, unsigned *flag, rf_stream_t *s);
+
 /* Precision conversion between the double host layout and device fields. */
 int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,
                       rf_stream_t *s);

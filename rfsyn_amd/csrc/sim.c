@@ -116,6 +116,7 @@ struct fdtd_sim {
     void *f2[6];
     int leap_rows, leap_chunk, leap_waves;   /* tiling requests (0: automatic) */
     int plan_rows;           /* rows per item of the last planned fused step */
+    unsigned *d_nonfinite;   /* flag of fdtd_sim_fields_finite */
     int leap_zsplit;         /* z CPML tiles in a launch of their own */
     int leap_coop;           /* cooperative blocks (rf_leap_t.coop) */
     rf_leap_src_t *d_srcs;   /* device source table of the fused step */
@@ -240,6 +241,7 @@ void fdtd_sim_destroy(fdtd_sim_t *s) {
     rf_free(s->d_cold);
     rf_free(s->zpsi);
     rf_free(s->d_ctr);
+    rf_free(s->d_nonfinite);
     free(s->planes);
     free(s->probes);
     free(s->freqs_all);
@@ -2082,6 +2084,31 @@ int fdtd_batch_run(fdtd_batch_t *b, size_t n_steps) {
 }
 
 size_t fdtd_batch_size(const fdtd_batch_t *b) { return b ? b->n : 0; }
+
+/* The isfinite gates of the reference's validators (e.g.
+ * validation_common.h:53), on the device: one scan of the state. */
+int fdtd_sim_fields_finite(fdtd_sim_t *s) {
+    if (!s) return -1;
+    if (!s->d_nonfinite && dalloc(s, (void **)&s->d_nonfinite, sizeof(unsigned))) return -1;
+    unsigned flag = 0;
+    if (rf_memset(s->d_nonfinite, 0, sizeof(unsigned), s->st) ||
+        rf_launch_nonfinite(s->prec, s->blk, 6 * (long long)s->cells, s->d_nonfinite, s->st) ||
+        rf_d2h(&flag, s->d_nonfinite, sizeof(unsigned), s->st) || rf_sync(s->st)) {
+        rf_set_error("fdtd_sim_fields_finite: %s", rf_error_string());
+        return -1;
+    }
+    return flag ? 0 : 1;
+}
+
+int fdtd_batch_fields_finite(fdtd_batch_t *b, int *finite) {
+    if (!b || !finite) return -1;
+    if (rf_sync(b->st)) return -1;   /* the batch's steps are on its own stream */
+    for (size_t q = 0; q < b->n; q++) {
+        finite[q] = fdtd_sim_fields_finite(b->sims[q]);
+        if (finite[q] < 0) return -1;
+    }
+    return 0;
+}
 
 int fdtd_batch_launches(const fdtd_batch_t *b, int *leapfrog, int *shell, int *ade) {
     if (!b || !b->built) return -1;

@@ -774,6 +774,16 @@ int launch_flux_t(const rf_geom_t *g, int a, int p, void *const fn[6], void *con
     return launched();
 }
 
+// Divergence check: *flag |= 1 when any of the n values is a NaN or an
+// infinity (one atomic per wavefront that saw one).
+template <typename T>
+__global__ void __launch_bounds__(256) k_nonfinite(const T *__restrict__ v, long long n, unsigned *flag) {
+    bool bad = false;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        bad = bad || !__builtin_isfinite(v[t]);
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 extern "C" {
 
 int rf_launch_update_h(int prec, const rf_geom_t *g, void *const f[6], double cb, double dx,
@@ -964,6 +974,19 @@ int rf_launch_flux(int prec, const rf_geom_t *g, int axis, int index, void *cons
     if (axis < 0 || axis > 2 || index < 0 || index >= g->n[axis] || n_partials < 1) return -1;
     return prec == 0 ? launch_flux_t<double>(g, axis, index, fn, fp, partials, n_partials, acc, hs(s))
                      : launch_flux_t<float>(g, axis, index, fn, fp, partials, n_partials, acc, hs(s));
+}
+
+int rf_launch_nonfinite(int prec, const void *v, long long n, unsigned *flag, rf_stream_t *s) {
+    if (n <= 0) return 0;
+    if (!v || !flag) return -1;
+    const unsigned blocks = (unsigned)((n + 255) / 256 > 8192 ? 8192 : (n + 255) / 256);
+    if (prec == 0)
+        hipLaunchKernelGGL((k_nonfinite<double>), dim3(blocks), dim3(256), 0, hs(s), static_cast<const double *>(v),
+                           n, flag);
+    else
+        hipLaunchKernelGGL((k_nonfinite<float>), dim3(blocks), dim3(256), 0, hs(s), static_cast<const float *>(v), n,
+                           flag);
+    return launched();
 }
 
 int rf_launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long long n,

@@ -466,6 +466,7 @@ class Batch:
 
     def __init__(self, lib, runs):
         self.lib = lib
+        self.n = len(runs)
         arr = (C.c_void_p * len(runs))(*[r.sim for r in runs])
         self.b = lib.fdtd_batch_create(arr, len(runs))
         assert self.b, lib.fdtd_last_error()
@@ -478,6 +479,12 @@ class Batch:
         lf, sh, ad = C.c_int(), C.c_int(), C.c_int()
         assert self.lib.fdtd_batch_launches(self.b, C.byref(lf), C.byref(sh), C.byref(ad)) == 0
         return lf.value, sh.value, ad.value
+
+    def finite(self):
+        """Per-scene failure flags: 1 where every field value is finite."""
+        out = (C.c_int * self.n)()
+        assert self.lib.fdtd_batch_fields_finite(self.b, out) == 0, self.lib.fdtd_last_error()
+        return list(out)
 
     def close(self):
         if self.b:
