@@ -5,7 +5,7 @@ import ctypes as C
 import numpy as np
 
 from rfsyn_amd import capi
-from scenes import field_arrays, pattern_fill
+from rfsyn_amd.driver import field_arrays, pattern_fill
 
 MODES = ("vacuum", "materials", "cpml", "cpml_materials")
 

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from rfsyn_amd import capi
-from scenes import same_values
+from rfsyn_amd.driver import same_values
 
 
 @pytest.fixture(scope="module")

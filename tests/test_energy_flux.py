@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from rfsyn_amd import capi
-from scenes import NativeRun, random_fill
+from rfsyn_amd.driver import NativeRun, random_fill
 
 EPS0, MU0 = 8.8541878128e-12, 1.25663706212e-6   # include/constants.h
 

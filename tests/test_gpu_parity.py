@@ -12,7 +12,7 @@ import pytest
 
 from rfsyn_amd import capi
 from lowlevel import wrapping_case, cpml_scene, tensor_scene
-from scenes import same_values, run_c1, NativeRun, c1_cpml_params, modgauss_params, c1_freqs, random_fill
+from rfsyn_amd.driver import same_values, run_c1, NativeRun, c1_cpml_params, modgauss_params, c1_freqs, random_fill
 
 pytestmark = pytest.mark.gpu
 

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from rfsyn_amd import capi
-from scenes import same_values, NativeRun, c1_cpml_params, modgauss_params, c1_freqs
+from rfsyn_amd.driver import same_values, NativeRun, c1_cpml_params, modgauss_params, c1_freqs
 
 pytestmark = pytest.mark.gpu
 

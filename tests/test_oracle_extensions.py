@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from rfsyn_amd import capi
-from scenes import field_arrays
+from rfsyn_amd.driver import field_arrays
 
 EPS0 = 8.8541878128e-12
 

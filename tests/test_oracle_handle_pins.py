@@ -17,7 +17,7 @@ import pytest
 
 from rfsyn_amd import capi
 from rfsyn_amd.driver import NativeRun, field_arrays, random_fill
-from scenes import same_values
+from rfsyn_amd.driver import same_values
 
 
 def _gauss(lib, g, t0=30.0, tau=7.0):

@@ -10,7 +10,7 @@ import pytest
 
 from rfsyn_amd import capi
 from lowlevel import wrapping_case, cpml_scene, tensor_scene
-from scenes import same_values
+from rfsyn_amd.driver import same_values
 
 
 def _default_preset():

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from rfsyn_amd import capi
-from scenes import NativeRun, c1_cpml_params, modgauss_params
+from rfsyn_amd.driver import NativeRun, c1_cpml_params, modgauss_params
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "example_000000.iq")
 
