@@ -152,6 +152,13 @@ int fdtd_sim_select_path(fdtd_sim_t *s, int launches, int *two_kernel);
 size_t fdtd_sim_steps_done(const fdtd_sim_t *s);
 /* Steps per DFT block the handle runs with (see dft_block); -1 for NULL. */
 int fdtd_sim_dft_block(const fdtd_sim_t *s);
+/* GPU failures recorded by the reference entry points that return nothing
+ * (fdtd_update_h, fdtd_update_e, their _cpml and _materials variants).  By
+ * default such a failure prints a message and aborts the process; with
+ * RFSYN_GPU_ERRORS=record (read at each failure) the call instead records
+ * the failure in fdtd_last_error(), counts it here and returns with the
+ * grid's host fields not updated by that call. */
+int fdtd_gpu_errors(void);
 /* Whether every value of the handle's current state (E and H on every
  * node, the padding included) is finite: 1 yes, 0 no (a NaN or an
  * infinity: the run has diverged), -1 on error.  One device scan of the

@@ -235,6 +235,7 @@ EXTENSION_API = {
     "fdtd_sim_dft_block": (C.c_int, [_vp]),
     "fdtd_sim_leap_rows": (C.c_int, [_vp]),
     "fdtd_sim_fields_finite": (C.c_int, [_vp]),
+    "fdtd_gpu_errors": (C.c_int, []),
     "fdtd_batch_fields_finite": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fdtd_batch_create": (_vp, [C.POINTER(_vp), _sz]),
     "fdtd_batch_run": (C.c_int, [_vp, _sz]),

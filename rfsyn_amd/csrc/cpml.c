@@ -264,23 +264,23 @@ rf_cpml_dev_t *rf_cpml_dev(cpml_t *p) {
 void fdtd_update_h_cpml(fdtd_grid_t *g, cpml_t *pml) {
     if (!g || !pml || !rf_cpml_matches(pml, g)) return;
     rf_cpml_dev_t *c = rf_cpml_dev(pml);
-    if (!c || rf_grid_upload(g, 077u) != 0) rf_fatal("fdtd_update_h_cpml setup");
+    if (!c || rf_grid_upload(g, 077u) != 0) RF_FAIL("fdtd_update_h_cpml setup");
     rf_geom_t geo = rf_geom_of(g);
     double cb = g->dt / (MU0 * g->dx);
     if (rf_launch_update_h(0, &geo, g->dev->f, cb, g->dx, &c->d, g->dev->st) != 0)
-        rf_fatal("fdtd_update_h_cpml launch");
-    if (rf_grid_download(g, 070u) != 0) rf_fatal("fdtd_update_h_cpml download");
+        RF_FAIL("fdtd_update_h_cpml launch");
+    if (rf_grid_download(g, 070u) != 0) RF_FAIL("fdtd_update_h_cpml download");
 }
 
 /* engine/fdtd/cpml.c:221-264 */
 void fdtd_update_e_cpml(fdtd_grid_t *g, cpml_t *pml) {
     if (!g || !pml || !rf_cpml_matches(pml, g)) return;
     rf_cpml_dev_t *c = rf_cpml_dev(pml);
-    if (!c || rf_grid_upload(g, 077u) != 0) rf_fatal("fdtd_update_e_cpml setup");
+    if (!c || rf_grid_upload(g, 077u) != 0) RF_FAIL("fdtd_update_e_cpml setup");
     rf_geom_t geo = rf_geom_of(g);
     rf_mat_t vac = {0};
     vac.vac_ca = g->dt / (EPS0 * g->dx);
     if (rf_launch_update_e(0, &geo, g->dev->f, g->dx, &c->d, &vac, g->dev->st) != 0)
-        rf_fatal("fdtd_update_e_cpml launch");
-    if (rf_grid_download(g, 007u) != 0) rf_fatal("fdtd_update_e_cpml download");
+        RF_FAIL("fdtd_update_e_cpml launch");
+    if (rf_grid_download(g, 007u) != 0) RF_FAIL("fdtd_update_e_cpml download");
 }

@@ -114,23 +114,23 @@ int rf_grid_download(fdtd_grid_t *g, unsigned mask) {
 /* engine/fdtd/grid.c:62-90 */
 void fdtd_update_h(fdtd_grid_t *g) {
     if (!g) return;
-    if (rf_grid_upload(g, 077u) != 0) rf_fatal("fdtd_update_h upload");
+    if (rf_grid_upload(g, 077u) != 0) RF_FAIL("fdtd_update_h upload");
     rf_geom_t geo = rf_geom_of(g);
     double cb = g->dt / (MU0 * g->dx);
     if (rf_launch_update_h(0, &geo, g->dev->f, cb, g->dx, NULL, g->dev->st) != 0)
-        rf_fatal("fdtd_update_h launch");
-    if (rf_grid_download(g, 070u) != 0) rf_fatal("fdtd_update_h download");
+        RF_FAIL("fdtd_update_h launch");
+    if (rf_grid_download(g, 070u) != 0) RF_FAIL("fdtd_update_h download");
 }
 
 /* engine/fdtd/grid.c:92-118.  E += ca (curl) is the materials update with
  * Ca = 1, Cb = ca, which rounds identically (1*E is exact). */
 void fdtd_update_e(fdtd_grid_t *g) {
     if (!g) return;
-    if (rf_grid_upload(g, 077u) != 0) rf_fatal("fdtd_update_e upload");
+    if (rf_grid_upload(g, 077u) != 0) RF_FAIL("fdtd_update_e upload");
     rf_geom_t geo = rf_geom_of(g);
     rf_mat_t vac = {0};
     vac.vac_ca = g->dt / (EPS0 * g->dx);
     if (rf_launch_update_e(0, &geo, g->dev->f, g->dx, NULL, &vac, g->dev->st) != 0)
-        rf_fatal("fdtd_update_e launch");
-    if (rf_grid_download(g, 007u) != 0) rf_fatal("fdtd_update_e download");
+        RF_FAIL("fdtd_update_e launch");
+    if (rf_grid_download(g, 007u) != 0) RF_FAIL("fdtd_update_e download");
 }

@@ -17,7 +17,12 @@
 void rf_set_error(const char *fmt, ...);
 /* For the void update calls of the reference API: a device failure cannot
  * be returned, so it is reported on stderr and the process aborts. */
+/* A GPU failure inside a reference entry point that returns nothing
+ * (fdtd_update_*): aborts with a message, or with RFSYN_GPU_ERRORS=record
+ * records it (fdtd_last_error, fdtd_gpu_errors) and returns; RF_FAIL then
+ * returns from the caller without updating. */
 void rf_fatal(const char *what);
+#define RF_FAIL(what) do { rf_fatal(what); return; } while (0)
 
 /* ------------------------------------------------------- hash map */
 

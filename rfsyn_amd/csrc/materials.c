@@ -848,17 +848,17 @@ static void update_e_dev(fdtd_grid_t *g, cpml_t *pml, materials_t *m, const char
     rf_cpml_dev_t *c = NULL;
     if (pml) {
         c = rf_cpml_dev(pml);
-        if (!c) rf_fatal(what);
+        if (!c) RF_FAIL(what);
     }
     rf_mat_dev_t *d = rf_materials_dev(m, g);
-    if (!d || rf_grid_upload(g, 077u) != 0) rf_fatal(what);
+    if (!d || rf_grid_upload(g, 077u) != 0) RF_FAIL(what);
     rf_geom_t geo = rf_geom_of(g);
     if (rf_launch_update_e(0, &geo, g->dev->f, g->dx, c ? &c->d : NULL, &d->d, g->dev->st) != 0)
-        rf_fatal(what);
+        RF_FAIL(what);
     if (d->n_tensor &&
         rf_launch_tensor(0, &geo, g->dev->f, d->tensor_edges, d->n_tensor, g->dev->st) != 0)
-        rf_fatal(what);
-    if (rf_grid_download(g, 007u) != 0) rf_fatal(what);
+        RF_FAIL(what);
+    if (rf_grid_download(g, 007u) != 0) RF_FAIL(what);
 }
 
 /* materials.c:1062-1070 */

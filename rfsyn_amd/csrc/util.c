@@ -5,6 +5,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 static __thread char g_last_error[512];
 
@@ -17,7 +18,19 @@ void rf_set_error(const char *fmt, ...) {
 
 const char *fdtd_last_error(void) { return g_last_error; }
 
+static int g_gpu_errors;
+
+int fdtd_gpu_errors(void) { return __atomic_load_n(&g_gpu_errors, __ATOMIC_RELAXED); }
+
 void rf_fatal(const char *what) {
+    const char *mode = getenv("RFSYN_GPU_ERRORS");
+    if (mode && !strcmp(mode, "record")) {
+        char gpu[256];
+        snprintf(gpu, sizeof gpu, "%s", rf_error_string());
+        rf_set_error("%s failed on the GPU: %s", what, gpu);
+        __atomic_add_fetch(&g_gpu_errors, 1, __ATOMIC_RELAXED);
+        return;
+    }
     fprintf(stderr, "librfsyn_fdtd: %s failed on the GPU: %s (%s)\n", what, rf_error_string(),
             g_last_error);
     fflush(stderr);

@@ -242,3 +242,28 @@ def test_material_table_full(product, ref):
         lib.fdtd_grid_destroy(g)
     assert res[0] == res[1]
     assert -1 in res[1][0]
+
+
+def test_update_gpu_failure_recorded_not_aborting(product, monkeypatch):
+    """fdtd_update_* return nothing, so a GPU failure inside them aborts by
+    default; with RFSYN_GPU_ERRORS=record it is recorded instead and the
+    call returns without touching the host fields.  Without a visible GPU
+    (this suite's usual host) the upload fails and must be recorded; with
+    one the update runs and records nothing."""
+    monkeypatch.setenv("RFSYN_GPU_ERRORS", "record")
+    g = product.fdtd_grid_create(6, 5, 4, 1e-3, 0.99)
+    assert g
+    gg = g.contents
+    idx = (2 * (gg.ny + 1) + 2) * (gg.nz + 1) + 2
+    gg.ez[idx] = 1.0
+    before = product.fdtd_gpu_errors()
+    product.fdtd_update_h(g)
+    product.fdtd_update_e(g)
+    after = product.fdtd_gpu_errors()
+    if after > before:   # no device: both calls recorded their failure
+        assert after == before + 2
+        assert b"failed on the GPU" in product.fdtd_last_error()
+        assert gg.ez[idx] == 1.0 and gg.hx[idx] == 0.0
+    else:   # a device: the step ran (the curl of the lone Ez moved H)
+        assert any(gg.hx[q] != 0.0 or gg.hy[q] != 0.0 for q in range(6 * 7 * 5))
+    product.fdtd_grid_destroy(g)
