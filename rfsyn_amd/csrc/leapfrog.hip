@@ -900,13 +900,13 @@ k_leapfrog_packed(const rf_leap_t *__restrict__ tab, const int *__restrict__ ite
 // z CPML slabs lie in its first and last chunk): the wavefronts of an XCD
 // stay at nearby planes and their halo rows meet in its L2.  A chunk's
 // edge lane takes its z neighbour from the next or previous chunk by
-// readlane.  When nz = 64 ROW_NCH, column nz lies past the chunks; there
+// readlane.  When nz = 64 NCH, column nz lies past the chunks; there
 // only hz changes (the boundary's tangential E and the H components with a
 // z difference keep their values, equal in both state buffers), so its E
 // values are read as wavefront-uniform values and its hz update is done by
 // every lane and stored by one.  The expressions and their order are those
 // of leap_item.
-constexpr int ROW_NCH = 4;   // chunks of 64 columns: 64 (ROW_NCH - 1) < nz <= 64 ROW_NCH
+constexpr int ROW_NCH_MAX = 4;   // NCH chunks of 64 columns: 64 (NCH - 1) < nz <= 64 NCH
 
 __device__ __forceinline__ float lane_of(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -915,11 +915,10 @@ __device__ __forceinline__ float uniform_of(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-template <typename TagT, int R>
+template <typename TagT, int R, int NCH>
 __device__ __forceinline__ void leap_row(const rf_leap_t &a, const LeapLds<float> &lv, const float *s_zk,
                                          long long step, int tj, int tc, int lane) {
     using T = float;
-    constexpr int NCH = ROW_NCH;
     constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
     const Geo g = geo_of(a.g);
     const int nls = lv.nls, ncache = lv.ncache;
@@ -1264,7 +1263,7 @@ __device__ __forceinline__ void leap_row(const rf_leap_t &a, const LeapLds<float
 // Row-mode step: items are (row tile, x chunk), row tile fastest, taken
 // from the XCD-major queue of k_leapfrog (independent wavefronts).
 // (at most 168 VGPRs: three wavefronts per SIMD; four would spill)
-template <typename TagT, int R>
+template <typename TagT, int R, int NCH>
 __global__ void __launch_bounds__(WL *WPB) __attribute__((amdgpu_waves_per_eu(3)))
 k_leap_rows(rf_leap_t a) {
     constexpr bool TAGS = !std::is_same<TagT, NoTag>::value;
@@ -1278,7 +1277,7 @@ k_leap_rows(rf_leap_t a) {
     const int nz = a.g.n[2];
     for (int t = threadIdx.x; t < 2 * 6 * WL; t += blockDim.x) {
         const int l = t % WL, p = (t / WL) % 6, e = t / (6 * WL);
-        const int q = e == 0 ? l : min(64 * (ROW_NCH - 1) + l, nz);
+        const int q = e == 0 ? l : min(64 * (NCH - 1) + l, nz);
         const bool on = q < a.zplo || q >= a.zphi;
         const int qq = p < 3 ? min(q, nz - 1) : q;   // half-grid profiles have nz entries
         s_zk[t] = on ? static_cast<const float *>(a.zk[p])[qq] : (p % 3 == 0 ? 1.f : 0.f);
@@ -1295,7 +1294,7 @@ k_leap_rows(rf_leap_t a) {
         if (lane == 0) ticket = atomicAdd(my_ctr, 1u);
         const int item = item_lo + static_cast<int>(__builtin_amdgcn_readfirstlane(ticket));
         if (item >= item_hi) break;
-        leap_row<TagT, R>(a, lv, s_zk, step, item % a.ntj, item / a.ntj, lane);
+        leap_row<TagT, R, NCH>(a, lv, s_zk, step, item % a.ntj, item / a.ntj, lane);
     }
 }
 
@@ -1973,19 +1972,31 @@ int leap_capacity(int prec, const rf_leap_t *a) {
     return a->rows == 4 ? capacity_rows<float, 4>(a) : capacity_rows<float, 8>(a);
 }
 
-// Row mode: the kernel of a's tag width and rows.
-template <int R>
-const void *zrow_fn_rows(const rf_leap_t *a) {
-    if (!a->tag) return reinterpret_cast<const void *>(k_leap_rows<NoTag, R>);
-    switch (a->tag_bytes) {
-    case 1: return reinterpret_cast<const void *>(k_leap_rows<uint8_t, R>);
-    case 2: return reinterpret_cast<const void *>(k_leap_rows<uint16_t, R>);
-    case 4: return reinterpret_cast<const void *>(k_leap_rows<uint32_t, R>);
+// Row mode: the kernel of a's tag width, rows and chunk count (fewer than
+// four chunks: vacuum and one-byte tags only).
+template <int R, int NCH>
+const void *zrow_fn_nch(const rf_leap_t *a) {
+    if (!a->tag) return reinterpret_cast<const void *>(k_leap_rows<NoTag, R, NCH>);
+    if (a->tag_bytes == 1) return reinterpret_cast<const void *>(k_leap_rows<uint8_t, R, NCH>);
+    if constexpr (NCH == ROW_NCH_MAX) {
+        if (a->tag_bytes == 2) return reinterpret_cast<const void *>(k_leap_rows<uint16_t, R, NCH>);
+        if (a->tag_bytes == 4) return reinterpret_cast<const void *>(k_leap_rows<uint32_t, R, NCH>);
     }
     return nullptr;
 }
 
-const void *zrow_fn(const rf_leap_t *a) { return a->rows == 1 ? zrow_fn_rows<1>(a) : nullptr; }
+int zrow_chunks(const rf_leap_t *a) { return (a->g.n[2] + 63) / 64; }
+
+const void *zrow_fn(const rf_leap_t *a) {
+    if (a->rows != 1) return nullptr;
+    switch (zrow_chunks(a)) {
+    case 1: return zrow_fn_nch<1, 1>(a);
+    case 2: return zrow_fn_nch<1, 2>(a);
+    case 3: return zrow_fn_nch<1, 3>(a);
+    case 4: return zrow_fn_nch<1, 4>(a);
+    }
+    return nullptr;
+}
 
 int zrow_capacity(const rf_leap_t *a) {
     const void *fn = zrow_fn(a);
@@ -2069,11 +2080,13 @@ extern "C" const char *rf_leap_zrow_refusal(const rf_leap_t *a) {
     const int nz = a->g.n[2];
     if (!a->zc || ((a->g.per >> 2) & 1) || a->ib0[2] != 0 || a->ib1[2] != nz + 1)
         return "the box does not span every z column with the z CPML inside it";
-    if (nz <= 64 * (ROW_NCH - 1) || nz > 64 * ROW_NCH) return "nz is not in (192, 256]";
-    if (a->zplo > 64 || a->zlo > 64 || a->zphi < 64 * (ROW_NCH - 1) || a->zhi < 64 * (ROW_NCH - 1))
+    const int nch = zrow_chunks(a);
+    if (nz < 1 || nch > ROW_NCH_MAX) return "nz is above 256";
+    if (a->zplo > 64 || a->zlo > 64 || a->zphi < 64 * (nch - 1) || a->zhi < 64 * (nch - 1))
         return "a z CPML slab reaches past the first or last 64 columns";
+    if (!zrow_fn(a)) return "no row-mode kernel for this tag width and nz";
     if (a->tag && a->n_slots > 0 && !a->ade_split) return "dispersive edges are updated in the fused kernel";
-    if (nz == 64 * ROW_NCH && a->n_srcs > 0 && a->src_box[5] >= nz) return "a source may touch column nz";
+    if (nz == 64 * nch && a->n_srcs > 0 && a->src_box[5] >= nz) return "a source may touch column nz";
     if (a->coop || a->zsplit) return "cooperative blocks or a z split were requested";
     return nullptr;
 }
@@ -2123,6 +2136,7 @@ extern "C" int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int tar
             long long chunks = (2LL * resident + a->ntj - 1) / a->ntj;
             if (chunks < 1) chunks = 1;
             chunk = (int)((planes + chunks - 1) / chunks);
+            if (chunk < 8) chunk = 8;   // small boxes: each item recomputes a warm-up plane
         }
         if (chunk > planes) chunk = (int)planes;
         a->chunk = chunk;

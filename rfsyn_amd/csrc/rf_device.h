@@ -281,12 +281,13 @@ typedef struct {
  * wavefront, at least 8 planes each (target_waves > 0 overrides the
  * residency the device reports).  FP32 rows 1 selects row mode: an
  * item is one row of every z column (lane l of chunk c holds
- * column 64 c + l), for a box spanning z columns 0..nz with the z CPML
- * inside it, 192 < nz <= 256, both z CPML slabs within the first and last
- * 64 columns, dispersive edges left to the ADE pass, independent
- * wavefronts, and (nz = 256) no source on column nz; chunk 0 = about two
- * items per resident wavefront.  0 on success, -1 for an unsupported
- * request. */
+ * column 64 c + l, ceil(nz / 64) chunks), for a box spanning z columns
+ * 0..nz with the z CPML inside it, nz <= 256 (below 193: vacuum or
+ * one-byte tags), both z CPML slabs within the first and last 64 columns,
+ * dispersive edges left to the ADE pass, independent wavefronts, and (nz
+ * a multiple of 64) no source on column nz; chunk 0 = about two items per
+ * resident wavefront, at least 8 planes each.  0 on success, -1 for an
+ * unsupported request. */
 int rf_leap_plan(int prec, rf_leap_t *a, int rows, int chunk, int target_waves);
 /* Why row mode cannot run on a's box and options (NULL: it can). */
 const char *rf_leap_zrow_refusal(const rf_leap_t *a);

@@ -1,5 +1,5 @@
 """Row mode of the FP32 fused step (RFSYN_LEAP_ROWS=1): a wavefront owns
-one row of every z column, in four 64-lane chunks, and takes the z
+one row of every z column, in one to four 64-lane chunks, and takes the z
 neighbours of a chunk's edge lanes from the next or previous chunk.  The
 arithmetic is that of the tiled step, so every field, probe series and
 plane-monitor DFT must equal the tiled run's bit for bit (FP32; the tiled
@@ -57,33 +57,35 @@ def _closed_setup(n):
 
 def _run(lib, case, rows, steps=40):
     p = modgauss_params(40e9, 30e9)
-    if case in ("stack", "stack_nz230"):
-        n = (16, 12, 256 if case == "stack" else 230)
+    kind, _, znum = case.partition("_nz")
+    nz = int(znum) if znum else None
+    if kind == "stack":
+        n = (16, 12, nz or 256)
         run = NativeRun(lib, n, 0.5e-3, 0.99, periodic=3, cpml=capi.CpmlParams(16, 3.0, 1.0, 1.0, 0.05, 1e-8),
                         precision=capi.FDTD_FP32, material_setup=_stack_setup(n[2]),
                         init_fields=lambda g: random_fill(g, 5))
         run.add_plane_source(capi.FIELD_EX, 2, 24, "source_waveform_modulated_gaussian", p)
         mons = [(run.add_plane_monitor(2, k, FREQS), (n[0], n[1], len(FREQS))) for k in (20, n[2] - 22)]
         pid = run.add_probe(capi.FIELD_EX, (3, 5, n[2] // 2 + 8), steps, None)
-    elif case == "closed":
-        n = (20, 18, 256)
+    elif kind == "closed":
+        n = (20, 18, nz or 256)
         run = NativeRun(lib, n, 1e-3, 0.99, cpml=capi.CpmlParams(6, 3.0, 1.0, 2.5, 0.05, 1e-8),
                         precision=capi.FDTD_FP32, material_setup=_closed_setup(n),
                         init_fields=lambda g: random_fill(g, 7))
-        run.add_point_source(capi.FIELD_EZ, (10, 9, 128), "source_waveform_modulated_gaussian", p)
+        run.add_point_source(capi.FIELD_EZ, (10, 9, n[2] // 2), "source_waveform_modulated_gaussian", p)
         run.add_point_source(capi.FIELD_EY, (n[0] - 2, 2, n[2] - 2), "source_waveform_modulated_gaussian", p,
                              hard=True)
         run.add_plane_source(capi.FIELD_EZ, 0, n[0] // 2, "source_waveform_modulated_gaussian", p, amp=0.2,
                              after_e=1)
         mons = [(run.add_plane_monitor(1, 7, FREQS), (n[0], n[2] + 1, len(FREQS)))]
-        pid = run.add_probe(capi.FIELD_EZ, (11, 9, 129), steps, None)
+        pid = run.add_probe(capi.FIELD_EZ, (11, 9, n[2] // 2 + 1), steps, None)
     else:   # vacuum, x periodic, CPML on y and z
-        n = (14, 22, 240)
+        n = (14, 22, nz or 240)
         run = NativeRun(lib, n, 1e-3, 0.99, periodic=1, cpml=capi.CpmlParams(10, 3.0, 1.0, 1.0, 0.05, 1e-8),
                         precision=capi.FDTD_FP32, init_fields=lambda g: random_fill(g, 9))
-        run.add_point_source(capi.FIELD_EX, (7, 11, 60), "source_waveform_modulated_gaussian", p)
+        run.add_point_source(capi.FIELD_EX, (7, 11, n[2] // 4), "source_waveform_modulated_gaussian", p)
         mons = []
-        pid = run.add_probe(capi.FIELD_EX, (7, 12, 62), steps, None)
+        pid = run.add_probe(capi.FIELD_EX, (7, 12, n[2] // 4 + 2), steps, None)
     run.run(steps)
     out = {"rows": run.lib.fdtd_sim_leap_rows(run.sim), "td": run.td(pid, steps), "fields": run.fields(),
            "mons": [run.plane_dft(m, w, shape) for m, shape in mons for w in range(2)]}
@@ -92,7 +94,11 @@ def _run(lib, case, rows, steps=40):
 
 
 @pytest.mark.parametrize("chunk", ["0", "5"])
-@pytest.mark.parametrize("case", ["stack", "stack_nz230", "closed", "vacuum"])
+@pytest.mark.parametrize("case", ["stack", "stack_nz230", "closed", "vacuum",
+                                  # one to three 64-column chunks, with and
+                                  # without column nz past the last chunk
+                                  "vacuum_nz48", "vacuum_nz64", "closed_nz100", "stack_nz120",
+                                  "closed_nz192"])
 def test_row_mode_bitwise_tiled(gpu, case, chunk, monkeypatch):
     monkeypatch.setenv("RFSYN_LEAP_CHUNK", chunk)
     monkeypatch.setenv("RFSYN_LEAP_ROWS", "4")
