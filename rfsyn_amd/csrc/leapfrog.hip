@@ -49,7 +49,7 @@ constexpr int KW = WL - 2;   // columns owned per wavefront
 static_assert(KW == RF_LEAP_COLS, "the host plans z tiles of RF_LEAP_COLS columns");
 constexpr int WPB = 4;       // wavefronts per block (independent mode)
 constexpr int WPB_MAX = 4;   // wavefronts per block (cooperative mode)
-constexpr int N_XCD = 8;     // blocks are dealt round-robin to the XCDs
+constexpr int N_XCD = RF_XCDS;   // blocks are dealt round-robin to the XCDs
 
 // DPP wavefront shifts: lane l receives lane l+1 (from_upper) or lane l-1
 // (from_lower); the edge lane receives 0 and is a halo lane whose result is
@@ -1439,21 +1439,23 @@ k_shell_h(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ ex, const T *__rest
           const T *__restrict__ ez, const T *__restrict__ hxa, const T *__restrict__ hya,
           const T *__restrict__ hza, T *__restrict__ hxb, T *__restrict__ hyb, T *__restrict__ hzb, T cb, T dx,
           rf_cpml_t cp, rf_zpsi_t zs) {
-    shell_h_node<T, CPML>(gg, sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, ex, ey, ez, hxa, hya, hza, hxb,
-                          hyb, hzb, cb, dx, cp, zs);
+    // blocks renumbered per XCD (xcd_block), as in every shell sweep
+    shell_h_node<T, CPML>(gg, sh, (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, ex, ey, ez,
+                          hxa, hya, hza, hxb, hyb, hzb, cb, dx, cp, zs);
 }
 
-// The shell H updates of many scenes: block b updates nodes of the job
-// block_job[b], from the job's first block on.
+// The shell H updates of many scenes: block b (renumbered per XCD)
+// updates nodes of the job block_job[b], from the job's first block on.
 template <typename T, bool CPML>
 __global__ void __launch_bounds__(256)
 k_shell_h_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *__restrict__ block_job) {
-    const rf_shell_job_t &jb = jobs[block_job[blockIdx.x]];
+    const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    const rf_shell_job_t &jb = jobs[block_job[b]];
     const T *const *fa = reinterpret_cast<const T *const *>(jb.fa);
     T *const *hb = reinterpret_cast<T *const *>(jb.hb);
-    shell_h_node<T, CPML>(jb.g, jb.sh, (long long)(blockIdx.x - jb.block0) * blockDim.x + threadIdx.x, fa[0], fa[1],
-                          fa[2], fa[3], fa[4], fa[5], hb[0], hb[1], hb[2], static_cast<T>(jb.cb),
-                          static_cast<T>(jb.dx), jb.cp, jb.zs);
+    shell_h_node<T, CPML>(jb.g, jb.sh, (long long)(b - jb.block0) * blockDim.x + threadIdx.x, fa[0], fa[1], fa[2],
+                          fa[3], fa[4], fa[5], hb[0], hb[1], hb[2], static_cast<T>(jb.cb), static_cast<T>(jb.dx),
+                          jb.cp, jb.zs);
 }
 
 // The first SHELL_LDS_ENTRIES entries of the material table, staged in LDS
@@ -1601,23 +1603,26 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
           rf_mat_t m, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
     __shared__ typename EntryOf<T>::type s_ent[SHELL_LDS_ENTRIES];
     const int n_lds = shell_stage_entries<T, TagT>(m, s_ent);
-    shell_e_node<T, TagT, CPML>(gg, sh, (long long)blockIdx.x * blockDim.x + threadIdx.x, exa, eya, eza, hx, hy, hz,
-                                exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step, s_ent, n_lds);
+    shell_e_node<T, TagT, CPML>(gg, sh, (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, exa,
+                                eya, eza, hx, hy, hz, exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step, s_ent,
+                                n_lds);
 }
 
-// The shell E updates of many scenes (block b: job block_job[b]).
+// The shell E updates of many scenes (block b, renumbered per XCD: job
+// block_job[b]).
 template <typename T, typename TagT, bool CPML>
 __global__ void __launch_bounds__(256)
 k_shell_e_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *__restrict__ block_job) {
-    const rf_shell_job_t &jb = jobs[block_job[blockIdx.x]];
+    const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    const rf_shell_job_t &jb = jobs[block_job[b]];
     const T *const *fa = reinterpret_cast<const T *const *>(jb.fa);
     const T *const *hb = reinterpret_cast<const T *const *>(jb.hb);
     T *const *eb = reinterpret_cast<T *const *>(jb.eb);
     __shared__ typename EntryOf<T>::type s_ent[SHELL_LDS_ENTRIES];
     const int n_lds = shell_stage_entries<T, TagT>(jb.m, s_ent);
-    shell_e_node<T, TagT, CPML>(jb.g, jb.sh, (long long)(blockIdx.x - jb.block0) * blockDim.x + threadIdx.x, fa[0],
-                                fa[1], fa[2], hb[0], hb[1], hb[2], eb[0], eb[1], eb[2], static_cast<T>(jb.dx), jb.cp,
-                                jb.zs, jb.m, jb.srcs, jb.n_srcs, jb.d_step, s_ent, n_lds);
+    shell_e_node<T, TagT, CPML>(jb.g, jb.sh, (long long)(b - jb.block0) * blockDim.x + threadIdx.x, fa[0], fa[1],
+                                fa[2], hb[0], hb[1], hb[2], eb[0], eb[1], eb[2], static_cast<T>(jb.dx), jb.cp, jb.zs,
+                                jb.m, jb.srcs, jb.n_srcs, jb.d_step, s_ent, n_lds);
 }
 
 template <typename T>
@@ -1850,29 +1855,31 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
            const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
            T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, unsigned ade_blocks, rf_caps_t cs,
            int cap_ring, const long long *d_step) {
-    // the blocks past the ADE range capture the plane monitors of the step
-    // (their planes lie outside the ADE box, so their values are final)
-    if (blockIdx.x >= ade_blocks) {
-        plane_capture_elem<T>(gg, cs, (long long)(blockIdx.x - ade_blocks) * blockDim.x + threadIdx.x, cap_ring,
-                              *d_step);
+    // blocks renumbered per XCD (xcd_block); the blocks past the ADE range
+    // capture the plane monitors of the step (their planes lie outside the
+    // ADE box, so their values are final)
+    const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    if (b >= ade_blocks) {
+        plane_capture_elem<T>(gg, cs, (long long)(b - ade_blocks) * blockDim.x + threadIdx.x, cap_ring, *d_step);
         return;
     }
-    ade_pass_block<T, TagT, NS>(gg, lo0, lo1, lo2, nj, nk, count, exa, eya, eza, exb, eyb, ezb, m, ncache, blockIdx.x,
+    ade_pass_block<T, TagT, NS>(gg, lo0, lo1, lo2, nj, nk, count, exa, eya, eza, exb, eyb, ezb, m, ncache, b,
                                 ade_blocks);
 }
 
-// The ADE passes of many scenes: block b runs block b - block0 of the
-// job block_job[b].
+// The ADE passes of many scenes: block b (renumbered per XCD) runs block
+// b - block0 of the job block_job[b].
 template <typename T, typename TagT, int NS>
 __global__ void __launch_bounds__(256)
 k_ade_pass_packed(const rf_ade_job_t *__restrict__ jobs, const unsigned short *__restrict__ block_job) {
-    const rf_ade_job_t &jb = jobs[block_job[blockIdx.x]];
+    const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    const rf_ade_job_t &jb = jobs[block_job[b]];
     const unsigned nb = (jb.count + 255u) / 256u;
     ade_pass_block<T, TagT, NS>(jb.g, jb.lo[0], jb.lo[1], jb.lo[2], jb.nj, jb.nk, jb.count,
                                 static_cast<const T *>(jb.ea[0]), static_cast<const T *>(jb.ea[1]),
                                 static_cast<const T *>(jb.ea[2]), static_cast<T *>(jb.eb[0]),
                                 static_cast<T *>(jb.eb[1]), static_cast<T *>(jb.eb[2]), jb.m, jb.ncache,
-                                blockIdx.x - jb.block0, nb);
+                                b - jb.block0, nb);
 }
 
 template <typename T, typename TagT>

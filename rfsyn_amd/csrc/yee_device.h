@@ -17,6 +17,17 @@ struct Geo {
     long long sx, sy;
 };
 
+// Block b of a grid of nb blocks runs on XCD b % RF_XCDS (blocks are dealt
+// round-robin to the XCDs); renumbered so that each XCD takes one
+// contiguous range of the grid's blocks in order (a bijection on [0, nb)).
+// Node-per-thread sweeps whose threads read the rows j-1 and planes i-1
+// that earlier blocks loaded then find them in the L2 of their own XCD.
+constexpr unsigned RF_XCDS = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned x = b % RF_XCDS, r = b / RF_XCDS, per = nb / RF_XCDS, rem = nb % RF_XCDS;
+    return x * per + (x < rem ? x : rem) + r;
+}
+
 __device__ __forceinline__ Geo geo_of(const rf_geom_t &g) {
     Geo o;
     o.nx = g.n[0]; o.ny = g.n[1]; o.nz = g.n[2];

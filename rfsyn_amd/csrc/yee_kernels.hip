@@ -24,6 +24,18 @@ namespace {
 
 using namespace rfk;
 
+// Node (k, j, i) of the thread: blocks of 64 x 4 nodes over (k, j), one
+// plane i per grid z, with the blocks renumbered per XCD (xcd_block) so that
+// the rows j-1 and plane i-1 a thread reads were loaded on its own XCD.
+__device__ __forceinline__ void grid_node(int &k, int &j, int &i) {
+    const unsigned nbx = gridDim.x, nby = gridDim.y;
+    const unsigned b = xcd_block(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
+    const unsigned q = b / nbx;
+    k = static_cast<int>((b - q * nbx) * blockDim.x + threadIdx.x);
+    j = static_cast<int>((q % nby) * blockDim.y + threadIdx.y);
+    i = static_cast<int>(q / nby);
+}
+
 // ----------------------------------------------------------------- H
 
 template <typename T, bool CPML>
@@ -32,9 +44,8 @@ k_update_h(rf_geom_t gg, const T *__restrict__ ex, const T *__restrict__ ey,
            const T *__restrict__ ez, T *__restrict__ hx, T *__restrict__ hy, T *__restrict__ hz,
            T cb, T dx, rf_cpml_t cp) {
     const Geo g = geo_of(gg);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
+    int k, j, i;
+    grid_node(k, j, i);
     if (k > g.nz || j > g.ny) return;
     const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
 
@@ -108,9 +119,8 @@ k_update_e(rf_geom_t gg, T *__restrict__ ex, T *__restrict__ ey, T *__restrict__
            const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
            T dx, rf_cpml_t cp, rf_mat_t m) {
     const Geo g = geo_of(gg);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
+    int k, j, i;
+    grid_node(k, j, i);
     if (k >= g.nz || j >= g.ny) return;
     const long long idx = (long long)i * g.sx + (long long)j * g.sy + k;
 
