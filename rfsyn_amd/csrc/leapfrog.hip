@@ -146,6 +146,28 @@ __device__ __forceinline__ T apply_sources(const rf_leap_src_t *tab_p, int n, T 
     return v;
 }
 
+// Whether any source of the table sits on node (i, j, k), whatever its
+// component and phase (apply_sources leaves every value of a node it
+// returns false for unchanged).
+__device__ __forceinline__ bool sources_touch(const rf_leap_src_t *tab_p, int n, int i, int j, int k, int gnx,
+                                              int gny, int gnz) {
+    typedef const __attribute__((address_space(1))) rf_leap_src_t gsrc_t;
+    gsrc_t *tab = (gsrc_t *)tab_p;
+    bool hit = false;
+    for (int s = 0; s < n; s++) {
+        // both tests without branches, so a source's fields are read in one
+        // batch of loads
+        gsrc_t &p = tab[s];
+        const int plane = p.plane, ax = p.axis, index = p.index, pi = p.i, pj = p.j, pk = p.k;
+        const int x = ax == 0 ? i : (ax == 1 ? j : k);
+        const bool in_x = ax == 0 || i < gnx, in_y = ax == 1 || j < gny, in_z = ax == 2 || k < gnz;
+        const bool on_plane = x == index && in_x && in_y && in_z;
+        const bool on_point = pi == i && pj == j && pk == k;
+        hit = hit || (plane ? on_plane : on_point);
+    }
+    return hit;
+}
+
 // A source of the table with its value at the current step, staged in
 // LDS at the start of the kernel.
 struct LSrc {
@@ -1323,24 +1345,50 @@ __device__ __forceinline__ unsigned div_magic(unsigned n, unsigned m, unsigned s
 // the index inside the box is split with multiply-high divisions.
 __device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, int &i, int &j, int &k) {
     if (t >= sh.start[sh.n]) return false;
-    long long s0 = 0;
-    int l0 = 0, l1 = 0, l2 = 0;
-    unsigned nk = 1, nj = 1, mk = 0, mj = 0, k1 = 0, k2 = 0, j1 = 0, j2 = 0;
+    // the box of the node: the boxes that start at or before it, less one
+    int bx = 0;
 #pragma unroll
-    for (int b = 0; b < 6; b++) {
-        if (b < sh.n && t >= sh.start[b]) {
-            s0 = sh.start[b];
-            l0 = sh.lo[b][0];
-            l1 = sh.lo[b][1];
-            l2 = sh.lo[b][2];
-            nj = static_cast<unsigned>(sh.hi[b][1] - sh.lo[b][1]);
-            nk = static_cast<unsigned>(sh.hi[b][2] - sh.lo[b][2]);
-            mk = sh.mk[b];
-            mj = sh.mj[b];
-            k1 = sh.sk[b][0];
-            k2 = sh.sk[b][1];
-            j1 = sh.sj[b][0];
-            j2 = sh.sj[b][1];
+    for (int b = 1; b < 6; b++) bx += (b < sh.n && t >= sh.start[b]) ? 1 : 0;
+    long long s0;
+    int l0, l1, l2;
+    unsigned nk, nj, mk, mj, k1, k2, j1, j2;
+    const int b0 = __builtin_amdgcn_readfirstlane(bx);
+    if (__ballot(bx != b0) == 0) {
+        // one box for every active lane (all but the wavefronts that
+        // straddle two boxes): its fields are read once, as scalars
+        s0 = sh.start[b0];
+        l0 = sh.lo[b0][0];
+        l1 = sh.lo[b0][1];
+        l2 = sh.lo[b0][2];
+        nj = static_cast<unsigned>(sh.hi[b0][1] - sh.lo[b0][1]);
+        nk = static_cast<unsigned>(sh.hi[b0][2] - sh.lo[b0][2]);
+        mk = sh.mk[b0];
+        mj = sh.mj[b0];
+        k1 = sh.sk[b0][0];
+        k2 = sh.sk[b0][1];
+        j1 = sh.sj[b0][0];
+        j2 = sh.sj[b0][1];
+    } else {
+        s0 = 0;
+        l0 = l1 = l2 = 0;
+        nk = nj = 1;
+        mk = mj = k1 = k2 = j1 = j2 = 0;
+#pragma unroll
+        for (int b = 0; b < 6; b++) {
+            if (b == bx) {
+                s0 = sh.start[b];
+                l0 = sh.lo[b][0];
+                l1 = sh.lo[b][1];
+                l2 = sh.lo[b][2];
+                nj = static_cast<unsigned>(sh.hi[b][1] - sh.lo[b][1]);
+                nk = static_cast<unsigned>(sh.hi[b][2] - sh.lo[b][2]);
+                mk = sh.mk[b];
+                mj = sh.mj[b];
+                k1 = sh.sk[b][0];
+                k2 = sh.sk[b][1];
+                j1 = sh.sj[b][0];
+                j2 = sh.sj[b][1];
+            }
         }
     }
     const unsigned r = static_cast<unsigned>(t - s0);
@@ -1545,7 +1593,9 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
     const long long dk = (k == 0) ? (long long)(g.nz - 1) : -1;
     const long long step = *(gmem_c<long long> *)d_step;
     T e0 = exa[idx], e1 = eya[idx], e2 = eza[idx];
-    if (n_srcs) {
+    // the sources are applied only by wavefronts with a node under one
+    const bool src_here = n_srcs && __ballot(sources_touch(srcs, n_srcs, i, j, k, g.nx, g.ny, g.nz)) != 0;
+    if (src_here) {
         e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 0);
         e1 = apply_sources<T>(srcs, n_srcs, e1, 1, i, j, k, g.nx, g.ny, g.nz, step, 0);
         e2 = apply_sources<T>(srcs, n_srcs, e2, 2, i, j, k, g.nx, g.ny, g.nz, step, 0);
@@ -1585,7 +1635,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
         }
         e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k, s_ent, n_lds);
     }
-    if (n_srcs) {
+    if (src_here) {
         e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 1);
         e1 = apply_sources<T>(srcs, n_srcs, e1, 1, i, j, k, g.nx, g.ny, g.nz, step, 1);
         e2 = apply_sources<T>(srcs, n_srcs, e2, 2, i, j, k, g.nx, g.ny, g.nz, step, 1);
