@@ -1419,6 +1419,67 @@ __device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, 
     return diff * kinv[k] + dx * psi;
 }
 
+// A CPML term of stretch / zstretch in three steps, so that a node's loads
+// can all be issued before any of its stores: term_load reads the profile
+// values and psi, term_apply gives the stretched difference (the same
+// arithmetic) and leaves the new psi in the term, term_store writes it.
+template <typename T>
+struct Term {
+    T kinv, b, a, psi;
+    long long rd, wr;   // psi read and write index, -1 outside the slab
+};
+
+template <typename T>
+__device__ __forceinline__ Term<T> term_load(const rf_cpml_t &c, const Geo &g, bool h, int axis, int q,
+                                             const void *psi_arr, int i, int j, int k) {
+    Term<T> o;
+    o.kinv = ((gmem_c<T> *)(h ? c.kinv_h[axis] : c.kinv_m[axis]))[q];
+    o.b = o.a = o.psi = T(0);
+    o.rd = o.wr = -1;
+    const int cq = slab_coord(c, axis, q);
+    if (cq >= 0) {
+        o.rd = o.wr = slab_index(c, g, axis, cq, i, j, k);
+        o.b = ((gmem_c<T> *)(h ? c.b_h[axis] : c.b_m[axis]))[q];
+        o.a = ((gmem_c<T> *)(h ? c.a_h[axis] : c.a_m[axis]))[q];
+        o.psi = ((gmem_c<T> *)psi_arr)[o.rd];
+    }
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ Term<T> zterm_load(const rf_cpml_t &c, const Geo &g, bool h, int i, int j, int k,
+                                              const rf_zpsi_t &z, int rd, int wr) {
+    Term<T> o;
+    o.kinv = ((gmem_c<T> *)(h ? c.kinv_h[2] : c.kinv_m[2]))[k];
+    o.b = o.a = o.psi = T(0);
+    o.rd = o.wr = -1;
+    const int cq = slab_coord(c, 2, k);
+    if (cq >= 0) {
+        const long long zo = ((long long)i * (g.ny + 1) + j) * 6 * z.zlen + cq;
+        o.rd = zo + (long long)rd * z.zlen;
+        o.wr = zo + (long long)wr * z.zlen;
+        o.b = ((gmem_c<T> *)(h ? c.b_h[2] : c.b_m[2]))[k];
+        o.a = ((gmem_c<T> *)(h ? c.a_h[2] : c.a_m[2]))[k];
+        o.psi = ((gmem_c<T> *)z.zpsi)[o.rd];
+    }
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T term_apply(Term<T> &t, T diff, T dx) {
+    T psi = T(0);
+    if (t.rd >= 0) {
+        psi = t.b * t.psi + t.a * diff;
+        t.psi = psi;
+    }
+    return diff * t.kinv + dx * psi;
+}
+
+template <typename T>
+__device__ __forceinline__ void term_store(const Term<T> &t, void *psi_arr) {
+    if (t.wr >= 0) ((gmem<T> *)psi_arr)[t.wr] = t.psi;
+}
+
 // H^{n+1/2} of shell node t.
 template <typename T, bool CPML>
 __device__ __forceinline__ void shell_h_node(const rf_geom_t &gg, const rf_shell_t &sh, long long t,
@@ -1602,38 +1663,81 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
     }
     unsigned tag = 0;
     if constexpr (!std::is_same<TagT, NoTag>::value) tag = load_tag<TagT>(m.tag, idx);
+    // every load of the node is issued first, then the updates, then the
+    // psi stores: a psi store placed between them would keep the next
+    // component's loads behind it (the psi arrays may alias as far as the
+    // compiler knows)
+    const T hxc = hx[idx], hyc = hy[idx], hzc = hz[idx];
+    T hz_j = T(0), hy_k = T(0), hx_k = T(0), hz_i = T(0), hy_i = T(0), hx_j = T(0);
+    if (do_x) {
+        hz_j = hz[idx + dj];
+        hy_k = hy[idx + dk];
+    }
+    if (do_y) {
+        hx_k = hx[idx + dk];
+        hz_i = hz[idx + di];
+    }
+    if (do_z) {
+        hy_i = hy[idx + di];
+        hx_j = hx[idx + dj];
+    }
+    Term<T> ta[3] = {}, tb[3] = {};
+    if (CPML) {
+        if (do_x) {
+            ta[0] = term_load<T>(cp, g, false, 1, j, cp.psi_e[0][0], i, j, k);
+            tb[0] = zs.zpsi ? zterm_load<T>(cp, g, false, i, j, k, zs, 4, 4)
+                            : term_load<T>(cp, g, false, 2, k, cp.psi_e[0][1], i, j, k);
+        }
+        if (do_y) {
+            ta[1] = zs.zpsi ? zterm_load<T>(cp, g, false, i, j, k, zs, 5, 5)
+                            : term_load<T>(cp, g, false, 2, k, cp.psi_e[1][0], i, j, k);
+            tb[1] = term_load<T>(cp, g, false, 0, i, cp.psi_e[1][1], i, j, k);
+        }
+        if (do_z) {
+            ta[2] = term_load<T>(cp, g, false, 0, i, cp.psi_e[2][0], i, j, k);
+            tb[2] = term_load<T>(cp, g, false, 1, j, cp.psi_e[2][1], i, j, k);
+        }
+    }
     if (do_x) {   // materials.c:975-1000
-        T dA = hz[idx] - hz[idx + dj];
-        T dB = hy[idx] - hy[idx + dk];
+        T dA = hzc - hz_j;
+        T dB = hyc - hy_k;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[0][0]), i, j, k, dx);
-            if (zs.zpsi)
-                dB = zstretch<T>(dB, cp, g, false, i, j, k, zs, 4, 4, dx);
-            else
-                dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[0], dA, dx);
+            dB = term_apply<T>(tb[0], dB, dx);
         }
         e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k, s_ent, n_lds);
     }
     if (do_y) {   // materials.c:1015-1040
-        T dA = hx[idx] - hx[idx + dk];
-        T dB = hz[idx] - hz[idx + di];
+        T dA = hxc - hx_k;
+        T dB = hzc - hz_i;
         if (CPML) {
-            if (zs.zpsi)
-                dA = zstretch<T>(dA, cp, g, false, i, j, k, zs, 5, 5, dx);
-            else
-                dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[1][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[1], dA, dx);
+            dB = term_apply<T>(tb[1], dB, dx);
         }
         e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k, s_ent, n_lds);
     }
     if (do_z) {   // materials.c:1050-1058
-        T dA = hy[idx] - hy[idx + di];
-        T dB = hx[idx] - hx[idx + dj];
+        T dA = hyc - hy_i;
+        T dB = hxc - hx_j;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[2][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[2][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[2], dA, dx);
+            dB = term_apply<T>(tb[2], dB, dx);
         }
         e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k, s_ent, n_lds);
+    }
+    if (CPML) {
+        if (do_x) {
+            term_store<T>(ta[0], cp.psi_e[0][0]);
+            term_store<T>(tb[0], zs.zpsi ? zs.zpsi : cp.psi_e[0][1]);
+        }
+        if (do_y) {
+            term_store<T>(ta[1], zs.zpsi ? zs.zpsi : cp.psi_e[1][0]);
+            term_store<T>(tb[1], cp.psi_e[1][1]);
+        }
+        if (do_z) {
+            term_store<T>(ta[2], cp.psi_e[2][0]);
+            term_store<T>(tb[2], cp.psi_e[2][1]);
+        }
     }
     if (src_here) {
         e0 = apply_sources<T>(srcs, n_srcs, e0, 0, i, j, k, g.nx, g.ny, g.nz, step, 1);
