@@ -1400,29 +1400,12 @@ __device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, in
     return true;
 }
 
-// z CPML term of a shell node with its psi in the z-psi store (same
-// arithmetic as stretch, yee_device.h): read slot rd, write slot wr.
-template <typename T>
-__device__ __forceinline__ T zstretch(T diff, const rf_cpml_t &c, const Geo &g, bool h, int i, int j, int k,
-                                      const rf_zpsi_t &z, int rd, int wr, T dx) {
-    gmem_c<T> *kinv = (gmem_c<T> *)(h ? c.kinv_h[2] : c.kinv_m[2]);
-    gmem_c<T> *bb = (gmem_c<T> *)(h ? c.b_h[2] : c.b_m[2]);
-    gmem_c<T> *aa = (gmem_c<T> *)(h ? c.a_h[2] : c.a_m[2]);
-    const int cq = slab_coord(c, 2, k);
-    T psi = T(0);
-    if (cq >= 0) {
-        gmem<T> *zp = (gmem<T> *)z.zpsi;
-        const long long zo = ((long long)i * (g.ny + 1) + j) * 6 * z.zlen + cq;
-        psi = bb[k] * zp[zo + (long long)rd * z.zlen] + aa[k] * diff;
-        zp[zo + (long long)wr * z.zlen] = psi;
-    }
-    return diff * kinv[k] + dx * psi;
-}
-
-// A CPML term of stretch / zstretch in three steps, so that a node's loads
-// can all be issued before any of its stores: term_load reads the profile
-// values and psi, term_apply gives the stretched difference (the same
-// arithmetic) and leaves the new psi in the term, term_store writes it.
+// A CPML term of a shell node (the arithmetic of stretch, yee_device.h) in
+// three steps, so that a node's loads can all be issued before any of its
+// stores: term_load reads the profile values and psi (zterm_load: psi from
+// the z-psi store, read slot rd, write slot wr), term_apply gives the
+// stretched difference and leaves the new psi in the term, term_store
+// writes it.
 template <typename T>
 struct Term {
     T kinv, b, a, psi;
@@ -1502,40 +1485,80 @@ __device__ __forceinline__ void shell_h_node(const rf_geom_t &gg, const rf_shell
     const long long di = (g.px && i + 1 == g.nx) ? -(long long)i * g.sx : g.sx;
     const long long dj = (g.py && j + 1 == g.ny) ? -(long long)j * g.sy : g.sy;
     const long long dk = (g.pz && k + 1 == g.nz) ? -(long long)k : 1;
+    // every load of the node first, then the updates, then the stores (as
+    // in shell_e_node)
     T hx = hxa[idx], hy = hya[idx], hz = hza[idx];
     const T exc = ex[idx], eyc = ey[idx], ezc = ez[idx];
+    T ez_j = T(0), ey_k = T(0), ex_k = T(0), ez_i = T(0), ey_i = T(0), ex_j = T(0);
+    if (do_x) {
+        ez_j = ez[idx + dj];
+        ey_k = ey[idx + dk];
+    }
+    if (do_y) {
+        ex_k = ex[idx + dk];
+        ez_i = ez[idx + di];
+    }
+    if (do_z) {
+        ey_i = ey[idx + di];
+        ex_j = ex[idx + dj];
+    }
+    Term<T> ta[3] = {}, tb[3] = {};
+    if (CPML) {
+        if (do_x) {
+            ta[0] = term_load<T>(cp, g, true, 1, j, cp.psi_h[0][0], i, j, k);
+            tb[0] = zs.zpsi ? zterm_load<T>(cp, g, true, i, j, k, zs, 2 * zs.zpar, 2 - 2 * zs.zpar)
+                            : term_load<T>(cp, g, true, 2, k, cp.psi_h[0][1], i, j, k);
+        }
+        if (do_y) {
+            ta[1] = zs.zpsi ? zterm_load<T>(cp, g, true, i, j, k, zs, 2 * zs.zpar + 1, 3 - 2 * zs.zpar)
+                            : term_load<T>(cp, g, true, 2, k, cp.psi_h[1][0], i, j, k);
+            tb[1] = term_load<T>(cp, g, true, 0, i, cp.psi_h[1][1], i, j, k);
+        }
+        if (do_z) {
+            ta[2] = term_load<T>(cp, g, true, 0, i, cp.psi_h[2][0], i, j, k);
+            tb[2] = term_load<T>(cp, g, true, 1, j, cp.psi_h[2][1], i, j, k);
+        }
+    }
     if (do_x) {   // grid.c:70, cpml.c:185-189
-        T dA = ez[idx + dj] - ezc;
-        T dB = ey[idx + dk] - eyc;
+        T dA = ez_j - ezc;
+        T dB = ey_k - eyc;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[0][0]), i, j, k, dx);
-            if (zs.zpsi)
-                dB = zstretch<T>(dB, cp, g, true, i, j, k, zs, 2 * zs.zpar, 2 - 2 * zs.zpar, dx);
-            else
-                dB = stretch<T>(dB, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[0][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[0], dA, dx);
+            dB = term_apply<T>(tb[0], dB, dx);
         }
         hx -= cb * (dA - dB);
     }
     if (do_y) {   // grid.c:78, cpml.c:198-202
-        T dA = ex[idx + dk] - exc;
-        T dB = ez[idx + di] - ezc;
+        T dA = ex_k - exc;
+        T dB = ez_i - ezc;
         if (CPML) {
-            if (zs.zpsi)
-                dA = zstretch<T>(dA, cp, g, true, i, j, k, zs, 2 * zs.zpar + 1, 3 - 2 * zs.zpar, dx);
-            else
-                dA = stretch<T>(dA, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[1][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[1][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[1], dA, dx);
+            dB = term_apply<T>(tb[1], dB, dx);
         }
         hy -= cb * (dA - dB);
     }
     if (do_z) {   // grid.c:86, cpml.c:211-215
-        T dA = ey[idx + di] - eyc;
-        T dB = ex[idx + dj] - exc;
+        T dA = ey_i - eyc;
+        T dB = ex_j - exc;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[2][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[2][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[2], dA, dx);
+            dB = term_apply<T>(tb[2], dB, dx);
         }
         hz -= cb * (dA - dB);
+    }
+    if (CPML) {
+        if (do_x) {
+            term_store<T>(ta[0], cp.psi_h[0][0]);
+            term_store<T>(tb[0], zs.zpsi ? zs.zpsi : cp.psi_h[0][1]);
+        }
+        if (do_y) {
+            term_store<T>(ta[1], zs.zpsi ? zs.zpsi : cp.psi_h[1][0]);
+            term_store<T>(tb[1], cp.psi_h[1][1]);
+        }
+        if (do_z) {
+            term_store<T>(ta[2], cp.psi_h[2][0]);
+            term_store<T>(tb[2], cp.psi_h[2][1]);
+        }
     }
     hxb[idx] = hx;
     hyb[idx] = hy;
