@@ -1320,13 +1320,6 @@ k_leap_rows(rf_leap_t a) {
     }
 }
 
-// Global-memory views: loads and stores through a pointer the compiler
-// cannot prove global (one read from a device table) compile to FLAT
-// instructions, which also count against the LDS counter; every array the
-// shell kernels touch is device memory, so they access it through these.
-template <typename T> using gmem_c = const __attribute__((address_space(1))) T;
-template <typename T> using gmem = __attribute__((address_space(1))) T;
-
 // ------------------------------------------------------------- shell
 // The nodes around the fused box (CPML slabs and their one-node margin)
 // are updated one node per thread, H then E, with the expressions of the
@@ -1400,35 +1393,8 @@ __device__ __forceinline__ bool shell_node(const rf_shell_t &sh, long long t, in
     return true;
 }
 
-// A CPML term of a shell node (the arithmetic of stretch, yee_device.h) in
-// three steps, so that a node's loads can all be issued before any of its
-// stores: term_load reads the profile values and psi (zterm_load: psi from
-// the z-psi store, read slot rd, write slot wr), term_apply gives the
-// stretched difference and leaves the new psi in the term, term_store
-// writes it.
-template <typename T>
-struct Term {
-    T kinv, b, a, psi;
-    long long rd, wr;   // psi read and write index, -1 outside the slab
-};
-
-template <typename T>
-__device__ __forceinline__ Term<T> term_load(const rf_cpml_t &c, const Geo &g, bool h, int axis, int q,
-                                             const void *psi_arr, int i, int j, int k) {
-    Term<T> o;
-    o.kinv = ((gmem_c<T> *)(h ? c.kinv_h[axis] : c.kinv_m[axis]))[q];
-    o.b = o.a = o.psi = T(0);
-    o.rd = o.wr = -1;
-    const int cq = slab_coord(c, axis, q);
-    if (cq >= 0) {
-        o.rd = o.wr = slab_index(c, g, axis, cq, i, j, k);
-        o.b = ((gmem_c<T> *)(h ? c.b_h[axis] : c.b_m[axis]))[q];
-        o.a = ((gmem_c<T> *)(h ? c.a_h[axis] : c.a_m[axis]))[q];
-        o.psi = ((gmem_c<T> *)psi_arr)[o.rd];
-    }
-    return o;
-}
-
+// The z CPML term of a shell node (term_load, yee_device.h) with its psi in
+// the z-psi store: read slot rd, write slot wr.
 template <typename T>
 __device__ __forceinline__ Term<T> zterm_load(const rf_cpml_t &c, const Geo &g, bool h, int i, int j, int k,
                                               const rf_zpsi_t &z, int rd, int wr) {
@@ -1446,21 +1412,6 @@ __device__ __forceinline__ Term<T> zterm_load(const rf_cpml_t &c, const Geo &g, 
         o.psi = ((gmem_c<T> *)z.zpsi)[o.rd];
     }
     return o;
-}
-
-template <typename T>
-__device__ __forceinline__ T term_apply(Term<T> &t, T diff, T dx) {
-    T psi = T(0);
-    if (t.rd >= 0) {
-        psi = t.b * t.psi + t.a * diff;
-        t.psi = psi;
-    }
-    return diff * t.kinv + dx * psi;
-}
-
-template <typename T>
-__device__ __forceinline__ void term_store(const Term<T> &t, void *psi_arr) {
-    if (t.wr >= 0) ((gmem<T> *)psi_arr)[t.wr] = t.psi;
 }
 
 // H^{n+1/2} of shell node t.

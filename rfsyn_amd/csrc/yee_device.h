@@ -75,6 +75,57 @@ __device__ __forceinline__ T stretch(T diff, const rf_cpml_t &c, const Geo &g, b
     return diff * kinv[q] + dx * psi;
 }
 
+// Global-memory views: loads and stores through a pointer the compiler
+// cannot prove global (one read from a device table) compile to FLAT
+// instructions, which also count against the LDS counter; device arrays
+// reached through such pointers are accessed through these.
+template <typename T> using gmem_c = const __attribute__((address_space(1))) T;
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+
+// A CPML term (the arithmetic of stretch) in three steps, so that a node's
+// loads can all be issued before any of its stores: term_load reads the
+// profile values and psi, term_apply gives the stretched difference and
+// leaves the new psi in the term, term_store writes it.  Without the split
+// a psi store would keep the next component's loads behind it (the psi
+// arrays may alias the fields as far as the compiler knows).
+template <typename T>
+struct Term {
+    T kinv, b, a, psi;
+    long long rd, wr;   // psi read and write index, -1 outside the slab
+};
+
+template <typename T>
+__device__ __forceinline__ Term<T> term_load(const rf_cpml_t &c, const Geo &g, bool h, int axis, int q,
+                                             const void *psi_arr, int i, int j, int k) {
+    Term<T> o;
+    o.kinv = ((gmem_c<T> *)(h ? c.kinv_h[axis] : c.kinv_m[axis]))[q];
+    o.b = o.a = o.psi = T(0);
+    o.rd = o.wr = -1;
+    const int cq = slab_coord(c, axis, q);
+    if (cq >= 0) {
+        o.rd = o.wr = slab_index(c, g, axis, cq, i, j, k);
+        o.b = ((gmem_c<T> *)(h ? c.b_h[axis] : c.b_m[axis]))[q];
+        o.a = ((gmem_c<T> *)(h ? c.a_h[axis] : c.a_m[axis]))[q];
+        o.psi = ((gmem_c<T> *)psi_arr)[o.rd];
+    }
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T term_apply(Term<T> &t, T diff, T dx) {
+    T psi = T(0);
+    if (t.rd >= 0) {
+        psi = t.b * t.psi + t.a * diff;
+        t.psi = psi;
+    }
+    return diff * t.kinv + dx * psi;
+}
+
+template <typename T>
+__device__ __forceinline__ void term_store(const Term<T> &t, void *psi_arr) {
+    if (t.wr >= 0) ((gmem<T> *)psi_arr)[t.wr] = t.psi;
+}
+
 template <typename T> struct EntryOf;
 template <> struct EntryOf<double> { using type = rf_entry64_t; using ade = rf_ade64_t; };
 template <> struct EntryOf<float> { using type = rf_entry32_t; using ade = rf_ade32_t; };

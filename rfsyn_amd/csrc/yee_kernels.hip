@@ -59,58 +59,97 @@ k_update_h(rf_geom_t gg, const T *__restrict__ ex, const T *__restrict__ ey,
     const long long dj = (g.py && j + 1 == g.ny) ? -(long long)j * g.sy : g.sy;
     const long long dk = (g.pz && k + 1 == g.nz) ? -(long long)k : 1;
 
+    // every load of the node first, then the updates, then the stores
+    // (term_load, yee_device.h)
     const T exc = ex[idx], eyc = ey[idx], ezc = ez[idx];
-    if (do_x) {   // grid.c:70, cpml.c:185-189
-        T dA = ez[idx + dj] - ezc;
-        T dB = ey[idx + dk] - eyc;
-        if (CPML) {
-            dA = stretch<T>(dA, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[0][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[0][1]), i, j, k, dx);
+    T ez_j = T(0), ey_k = T(0), ex_k = T(0), ez_i = T(0), ey_i = T(0), ex_j = T(0);
+    T hxo = T(0), hyo = T(0), hzo = T(0);
+    if (do_x) {
+        ez_j = ez[idx + dj];
+        ey_k = ey[idx + dk];
+        hxo = hx[idx];
+    }
+    if (do_y) {
+        ex_k = ex[idx + dk];
+        ez_i = ez[idx + di];
+        hyo = hy[idx];
+    }
+    if (do_z) {
+        ey_i = ey[idx + di];
+        ex_j = ex[idx + dj];
+        hzo = hz[idx];
+    }
+    Term<T> ta[3] = {}, tb[3] = {};
+    if (CPML) {
+        if (do_x) {
+            ta[0] = term_load<T>(cp, g, true, 1, j, cp.psi_h[0][0], i, j, k);
+            tb[0] = term_load<T>(cp, g, true, 2, k, cp.psi_h[0][1], i, j, k);
         }
-        hx[idx] -= cb * (dA - dB);
+        if (do_y) {
+            ta[1] = term_load<T>(cp, g, true, 2, k, cp.psi_h[1][0], i, j, k);
+            tb[1] = term_load<T>(cp, g, true, 0, i, cp.psi_h[1][1], i, j, k);
+        }
+        if (do_z) {
+            ta[2] = term_load<T>(cp, g, true, 0, i, cp.psi_h[2][0], i, j, k);
+            tb[2] = term_load<T>(cp, g, true, 1, j, cp.psi_h[2][1], i, j, k);
+        }
+    }
+    if (do_x) {   // grid.c:70, cpml.c:185-189
+        T dA = ez_j - ezc;
+        T dB = ey_k - eyc;
+        if (CPML) {
+            dA = term_apply<T>(ta[0], dA, dx);
+            dB = term_apply<T>(tb[0], dB, dx);
+            term_store<T>(ta[0], cp.psi_h[0][0]);
+            term_store<T>(tb[0], cp.psi_h[0][1]);
+        }
+        hx[idx] = hxo - cb * (dA - dB);
     }
     if (do_y) {   // grid.c:78, cpml.c:198-202
-        T dA = ex[idx + dk] - exc;
-        T dB = ez[idx + di] - ezc;
+        T dA = ex_k - exc;
+        T dB = ez_i - ezc;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, true, 2, k, static_cast<T *>(cp.psi_h[1][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[1][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[1], dA, dx);
+            dB = term_apply<T>(tb[1], dB, dx);
+            term_store<T>(ta[1], cp.psi_h[1][0]);
+            term_store<T>(tb[1], cp.psi_h[1][1]);
         }
-        hy[idx] -= cb * (dA - dB);
+        hy[idx] = hyo - cb * (dA - dB);
     }
     if (do_z) {   // grid.c:86, cpml.c:211-215
-        T dA = ey[idx + di] - eyc;
-        T dB = ex[idx + dj] - exc;
+        T dA = ey_i - eyc;
+        T dB = ex_j - exc;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, true, 0, i, static_cast<T *>(cp.psi_h[2][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, true, 1, j, static_cast<T *>(cp.psi_h[2][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[2], dA, dx);
+            dB = term_apply<T>(tb[2], dB, dx);
+            term_store<T>(ta[2], cp.psi_h[2][0]);
+            term_store<T>(tb[2], cp.psi_h[2][1]);
         }
-        hz[idx] -= cb * (dA - dB);
+        hz[idx] = hzo - cb * (dA - dB);
     }
 }
 
 // ----------------------------------------------------------------- E
 
-template <typename T, typename TagT, bool CPML>
-__device__ __forceinline__ void e_component(int c, T *__restrict__ E, long long idx, T dA, T dB,
-                                            const rf_mat_t &m, unsigned tag, const Geo &g,
-                                            int i, int j, int k) {
-    using Entry = typename EntryOf<T>::type;
+// E update of component c with old value e (materials.c:975-1058 for one
+// component): false for a tensor edge, which this kernel leaves alone.
+template <typename T, typename TagT>
+__device__ __forceinline__ bool e_value(int c, T &e, T dA, T dB, const rf_mat_t &m, uint32_t fl, T ca, T cb,
+                                        int aid, int i, int j, int k) {
     using Ade = typename EntryOf<T>::ade;
     if constexpr (std::is_same<TagT, NoTag>::value) {
-        E[idx] = T(1) * E[idx] + static_cast<T>(m.vac_ca) * (dA - dB);
+        e = T(1) * e + static_cast<T>(m.vac_ca) * (dA - dB);
     } else {
-        const Entry &en = static_cast<const Entry *>(m.entries)[tag];
-        const uint32_t fl = en.flags[c];
-        if (fl & RF_EDGE_TENSOR) return;
+        if (fl & RF_EDGE_TENSOR) return false;
         if (fl & RF_EDGE_ADE) {
-            const Ade &a = static_cast<const Ade *>(m.ade)[en.ade[c]];
-            E[idx] = ade_edge<T>(a, en.Ca[c], en.Cb[c], E[idx], dA, dB, static_cast<T *>(m.J),
-                                 static_cast<T *>(m.P), 3 * a.st, ade_off(a, c, i, j, k));
-            return;
+            const Ade &a = static_cast<const Ade *>(m.ade)[aid];
+            e = ade_edge<T>(a, ca, cb, e, dA, dB, static_cast<T *>(m.J), static_cast<T *>(m.P), 3 * a.st,
+                            ade_off(a, c, i, j, k));
+            return true;
         }
-        E[idx] = en.Ca[c] * E[idx] + en.Cb[c] * (dA - dB);
+        e = ca * e + cb * (dA - dB);
     }
+    return true;
 }
 
 template <typename T, typename TagT, bool CPML>
@@ -140,34 +179,100 @@ k_update_e(rf_geom_t gg, T *__restrict__ ex, T *__restrict__ ey, T *__restrict__
     unsigned tag = 0;
     if constexpr (!std::is_same<TagT, NoTag>::value) tag = load_tag<TagT>(m.tag, idx);
 
+    // every load of the node first (fields, psi and the material entry),
+    // then the updates, then the stores (term_load, yee_device.h)
     const T hxc = hx[idx], hyc = hy[idx], hzc = hz[idx];
-    if (do_x) {   // materials.c:975-1000
-        T dA = hzc - hz[idx + dj];
-        T dB = hyc - hy[idx + dk];
-        if (CPML) {
-            dA = stretch<T>(dA, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[0][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[0][1]), i, j, k, dx);
+    T hz_j = T(0), hy_k = T(0), hx_k = T(0), hz_i = T(0), hy_i = T(0), hx_j = T(0);
+    T e[3] = {T(0), T(0), T(0)};
+    if (do_x) {
+        hz_j = hz[idx + dj];
+        hy_k = hy[idx + dk];
+        e[0] = ex[idx];
+    }
+    if (do_y) {
+        hx_k = hx[idx + dk];
+        hz_i = hz[idx + di];
+        e[1] = ey[idx];
+    }
+    if (do_z) {
+        hy_i = hy[idx + di];
+        hx_j = hx[idx + dj];
+        e[2] = ez[idx];
+    }
+    Term<T> ta[3] = {}, tb[3] = {};
+    if (CPML) {
+        if (do_x) {
+            ta[0] = term_load<T>(cp, g, false, 1, j, cp.psi_e[0][0], i, j, k);
+            tb[0] = term_load<T>(cp, g, false, 2, k, cp.psi_e[0][1], i, j, k);
         }
-        e_component<T, TagT, CPML>(0, ex, idx, dA, dB, m, tag, g, i, j, k);
+        if (do_y) {
+            ta[1] = term_load<T>(cp, g, false, 2, k, cp.psi_e[1][0], i, j, k);
+            tb[1] = term_load<T>(cp, g, false, 0, i, cp.psi_e[1][1], i, j, k);
+        }
+        if (do_z) {
+            ta[2] = term_load<T>(cp, g, false, 0, i, cp.psi_e[2][0], i, j, k);
+            tb[2] = term_load<T>(cp, g, false, 1, j, cp.psi_e[2][1], i, j, k);
+        }
+    }
+    uint32_t fl[3] = {0u, 0u, 0u};
+    T ca[3] = {T(0), T(0), T(0)}, cbv[3] = {T(0), T(0), T(0)};
+    int aid[3] = {0, 0, 0};
+    if constexpr (!std::is_same<TagT, NoTag>::value) {
+        typedef const __attribute__((address_space(1))) typename EntryOf<T>::type g_entry;
+        const g_entry &en = ((g_entry *)m.entries)[tag];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            fl[c] = en.flags[c];
+            ca[c] = en.Ca[c];
+            cbv[c] = en.Cb[c];
+            aid[c] = en.ade[c];
+        }
+    }
+    bool put[3] = {false, false, false};
+    if (do_x) {   // materials.c:975-1000
+        T dA = hzc - hz_j;
+        T dB = hyc - hy_k;
+        if (CPML) {
+            dA = term_apply<T>(ta[0], dA, dx);
+            dB = term_apply<T>(tb[0], dB, dx);
+        }
+        put[0] = e_value<T, TagT>(0, e[0], dA, dB, m, fl[0], ca[0], cbv[0], aid[0], i, j, k);
     }
     if (do_y) {   // materials.c:1015-1040
-        T dA = hxc - hx[idx + dk];
-        T dB = hzc - hz[idx + di];
+        T dA = hxc - hx_k;
+        T dB = hzc - hz_i;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, false, 2, k, static_cast<T *>(cp.psi_e[1][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[1][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[1], dA, dx);
+            dB = term_apply<T>(tb[1], dB, dx);
         }
-        e_component<T, TagT, CPML>(1, ey, idx, dA, dB, m, tag, g, i, j, k);
+        put[1] = e_value<T, TagT>(1, e[1], dA, dB, m, fl[1], ca[1], cbv[1], aid[1], i, j, k);
     }
     if (do_z) {   // materials.c:1050-1058
-        T dA = hyc - hy[idx + di];
-        T dB = hxc - hx[idx + dj];
+        T dA = hyc - hy_i;
+        T dB = hxc - hx_j;
         if (CPML) {
-            dA = stretch<T>(dA, cp, g, false, 0, i, static_cast<T *>(cp.psi_e[2][0]), i, j, k, dx);
-            dB = stretch<T>(dB, cp, g, false, 1, j, static_cast<T *>(cp.psi_e[2][1]), i, j, k, dx);
+            dA = term_apply<T>(ta[2], dA, dx);
+            dB = term_apply<T>(tb[2], dB, dx);
         }
-        e_component<T, TagT, CPML>(2, ez, idx, dA, dB, m, tag, g, i, j, k);
+        put[2] = e_value<T, TagT>(2, e[2], dA, dB, m, fl[2], ca[2], cbv[2], aid[2], i, j, k);
     }
+    if (CPML) {
+        if (do_x) {
+            term_store<T>(ta[0], cp.psi_e[0][0]);
+            term_store<T>(tb[0], cp.psi_e[0][1]);
+        }
+        if (do_y) {
+            term_store<T>(ta[1], cp.psi_e[1][0]);
+            term_store<T>(tb[1], cp.psi_e[1][1]);
+        }
+        if (do_z) {
+            term_store<T>(ta[2], cp.psi_e[2][0]);
+            term_store<T>(tb[2], cp.psi_e[2][1]);
+        }
+    }
+    if (put[0]) ex[idx] = e[0];
+    if (put[1]) ey[idx] = e[1];
+    if (put[2]) ez[idx] = e[2];
 }
 
 // ----------------------------------------------------------- tensors
