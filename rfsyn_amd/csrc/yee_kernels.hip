@@ -327,47 +327,71 @@ k_tensor(rf_geom_t gg, T *ex, T *ey, T *ez, const T *hx, const T *hy, const T *h
     if (t >= n_edges) return;
     const Geo g = geo_of(gg);
     const typename TEdgeOf<T>::type e = edges[t];
-    const T *const H[3] = {hx, hy, hz};
-    T *const E[3] = {ex, ey, ez};
     const int c = e.comp;
-    Pos x;
-    x.x[0] = e.i; x.x[1] = e.j; x.x[2] = e.k;
-    const long long idx = lin(g, x);
-    T value;
-    if (e.kind == 2) {   // materials.c:858, 881
-        Pos y = x;
-        if (c == 0) {
-            y.x[1] = prv(g, 1, x.x[1]);
-            value = e.diag * (hz[idx] - hz[lin(g, y)]);
-        } else {
-            y.x[0] = prv(g, 0, x.x[0]);
-            value = -e.diag * (hz[idx] - hz[lin(g, y)]);
-        }
-    } else {
-        value = e.diag * curl_full<T>(g, H, c, x);
-    }
-    // The eight neighbour curls are gathered first, whatever the mask (every
-    // neighbour position is a wrapped in-grid node), so their loads issue
-    // together; the masked ones are then added in slot order, the same
-    // operations as adding each present curl in turn.
-    T cd[8];
-    for (int slot = 0; slot < 2; slot++) {
-        // the other two components in increasing order
-        const int d = slot == 0 ? (c == 0 ? 1 : 0) : (c == 2 ? 1 : 2);
-        const int lo = c < d ? c : d;
-        for (int q = 0; q < 4; q++) {
+    const bool hz_only = e.kind == 2;   // materials.c:858, 881
+    const int x0 = e.i, x1 = e.j, x2 = e.k;
+    // The nine curls of the edge (n = 0 its own, n = 1 + 4 slot + q the
+    // neighbour curls, every neighbour position a wrapped in-grid node) are
+    // gathered first, with the positions, arrays and indices chosen by
+    // selects so that all their loads issue together; the masked neighbour
+    // curls are then added in slot order, the same operations as adding each
+    // present curl in turn.  A curl of component cc at y reads
+    // a = H[(cc+2)%3] at y and at y less one along (cc+1)%3, and
+    // b = H[(cc+1)%3] at y and at y less one along (cc+2)%3; the Hz-only
+    // curls read a = Hz at y and at y less one along axis 1 (cc = 0) or
+    // axis 0, and negate for cc != 0 (their b loads are made and unused).
+    T a0[9], a1[9], b0[9], b1[9];
+    int cc_of[9];
+#pragma unroll
+    for (int n = 0; n < 9; n++) {
+        int y0 = x0, y1 = x1, y2 = x2, cc = c;
+        if (n > 0) {
+            const int slot = (n - 1) >> 2, q = (n - 1) & 3;
+            // the other two components in increasing order
+            const int d = slot == 0 ? (c == 0 ? 1 : 0) : (c == 2 ? 1 : 2);
+            const int lo = c < d ? c : d;
             const int s_lo = q & 1, s_hi = (q >> 1) & 1;
-            const int s_c = (c == lo) ? s_lo : s_hi;
-            const int s_d = (d == lo) ? s_lo : s_hi;
-            Pos y = x;
-            if (s_c) y.x[c] = nxt(g, c, x.x[c]);
-            if (s_d) y.x[d] = prv(g, d, x.x[d]);
-            cd[slot * 4 + q] = e.kind == 2 ? curl_hz<T>(g, hz, d, y) : curl_full<T>(g, H, d, y);
+            const bool s_c = ((c == lo) ? s_lo : s_hi) != 0;
+            const bool s_d = ((d == lo) ? s_lo : s_hi) != 0;
+            y0 = (s_c && c == 0) ? nxt(g, 0, x0) : (s_d && d == 0) ? prv(g, 0, x0) : x0;
+            y1 = (s_c && c == 1) ? nxt(g, 1, x1) : (s_d && d == 1) ? prv(g, 1, x1) : x1;
+            y2 = (s_c && c == 2) ? nxt(g, 2, x2) : (s_d && d == 2) ? prv(g, 2, x2) : x2;
+            cc = d;
         }
+        cc_of[n] = cc;
+        const int pa = (cc + 1) % 3, qa = (cc + 2) % 3;
+        // the axis of a's second node and of b's
+        const int ax_a = hz_only ? (cc == 0 ? 1 : 0) : pa;
+        const int ax_b = qa;
+        const long long idx_y = (long long)y0 * g.sx + (long long)y1 * g.sy + y2;
+        const long long ia1 = (long long)(ax_a == 0 ? prv(g, 0, y0) : y0) * g.sx +
+                              (long long)(ax_a == 1 ? prv(g, 1, y1) : y1) * g.sy + (ax_a == 2 ? prv(g, 2, y2) : y2);
+        const long long ib1 = (long long)(ax_b == 0 ? prv(g, 0, y0) : y0) * g.sx +
+                              (long long)(ax_b == 1 ? prv(g, 1, y1) : y1) * g.sy + (ax_b == 2 ? prv(g, 2, y2) : y2);
+        const T *ha = hz_only ? hz : (qa == 0 ? hx : qa == 1 ? hy : hz);
+        const T *hb = pa == 0 ? hx : pa == 1 ? hy : hz;
+        a0[n] = ha[idx_y];
+        a1[n] = ha[ia1];
+        b0[n] = hb[idx_y];
+        b1[n] = hb[ib1];
+    }
+    T value;
+    if (hz_only)
+        value = c == 0 ? e.diag * (a0[0] - a1[0]) : -e.diag * (a0[0] - a1[0]);
+    else
+        value = e.diag * ((a0[0] - a1[0]) - (b0[0] - b1[0]));
+    T cd[8];
+#pragma unroll
+    for (int t8 = 0; t8 < 8; t8++) {
+        const int n = t8 + 1;
+        const T da = a0[n] - a1[n];
+        cd[t8] = hz_only ? (cc_of[n] == 0 ? da : -da) : da - (b0[n] - b1[n]);
     }
     for (int t8 = 0; t8 < 8; t8++)
         if ((e.mask >> t8) & 1u) value += e.w[t8] * cd[t8];
-    E[c][idx] += e.scale * value;
+    T *const ec = c == 0 ? ex : c == 1 ? ey : ez;
+    const long long idx = (long long)x0 * g.sx + (long long)x1 * g.sy + x2;
+    ec[idx] += e.scale * value;
 }
 
 // ----------------------------------------------------------- sources
