@@ -469,6 +469,13 @@ int rf_launch_update_h(int prec, const rf_geom_t *g, void *const f[6], double cb
                        const rf_cpml_t *cpml, rf_stream_t *s);
 int rf_launch_update_e(int prec, const rf_geom_t *g, void *const f[6], double dx,
                        const rf_cpml_t *cpml, const rf_mat_t *mat, rf_stream_t *s);
+/* rf_launch_update_e with the n_pts point sources pts (device) applied to
+ * E^n first, as rf_launch_point_sources at the step *d_step would; every
+ * source must sit on a node and component the E update writes (not a
+ * boundary node it skips, not a tensor edge). */
+int rf_launch_update_e_src(int prec, const rf_geom_t *g, void *const f[6], double dx, const rf_cpml_t *cpml,
+                           const rf_mat_t *mat, const rf_point_src_t *pts, int n_pts, const long long *d_step,
+                           rf_stream_t *s);
 int rf_launch_tensor(int prec, const rf_geom_t *g, void *const f[6], const void *edges,
                      long long n_edges, rf_stream_t *s);
 

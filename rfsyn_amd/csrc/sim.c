@@ -918,6 +918,29 @@ static int enqueue_fused(fdtd_sim_t *s, int *caps_done) {
 }
 
 
+/* Whether the two-kernel step applies the point sources inside the E
+ * update (rf_launch_update_e_src) instead of a kernel of their own: no
+ * plane source between them (the sources keep their order), no tensor
+ * edges (the E update leaves those to the tensor kernel), and every source
+ * on a node and component the E update writes (grid.c:95-117 ranges). */
+static int pts_in_update_e(const fdtd_sim_t *s) {
+    if (!s->n_pts || !s->h_pts) return 0;
+    if (s->has_mat && s->mat.n_tensor) return 0;
+    for (size_t p = 0; p < s->n_planes; p++)
+        if (!s->planes[p].after_e) return 0;
+    const int px = s->per & 1, py = (s->per >> 1) & 1, pz = (s->per >> 2) & 1;
+    for (size_t p = 0; p < s->n_pts; p++) {
+        const rf_point_src_t *q = &s->h_pts[p];
+        if (q->i < 0 || q->j < 0 || q->k < 0 || (size_t)q->i >= s->nx || (size_t)q->j >= s->ny ||
+            (size_t)q->k >= s->nz)
+            return 0;
+        const int i_ok = px || q->i >= 1, j_ok = py || q->j >= 1, k_ok = pz || q->k >= 1;
+        const int on = q->comp == 0 ? (j_ok && k_ok) : q->comp == 1 ? (i_ok && k_ok) : (i_ok && j_ok);
+        if (!on) return 0;
+    }
+    return 1;
+}
+
 static int enqueue_step(fdtd_sim_t *s) {
     const rf_cpml_t *cp = s->has_cpml ? &s->cp.d : NULL;
     const rf_mat_t *mt = s->has_mat ? &s->mat.d : &s->vac;
@@ -949,14 +972,18 @@ static int enqueue_step(fdtd_sim_t *s) {
         if (mark(s, 0)) return -1;
         if (rf_launch_update_h(s->prec, &s->geom, s->f, cb, s->dx, cp, s->st)) return -1;
         if (mark(s, 1)) return -1;
-        if (s->n_pts && rf_launch_point_sources(s->prec, s->f, s->d_pts, (int)s->n_pts, s->d_step, 0, s->st))
+        const int pts_in_e = pts_in_update_e(s);
+        if (s->n_pts && !pts_in_e &&
+            rf_launch_point_sources(s->prec, s->f, s->d_pts, (int)s->n_pts, s->d_step, 0, s->st))
             return -1;
         for (size_t p = 0; p < s->n_planes; p++)
             if (!s->planes[p].after_e &&
                 rf_launch_plane_source(s->prec, &s->geom, s->f, &s->planes[p].d, s->d_step, 0, s->st))
                 return -1;
         if (mark(s, 2)) return -1;
-        if (rf_launch_update_e(s->prec, &s->geom, s->f, s->dx, cp, mt, s->st)) return -1;
+        if (rf_launch_update_e_src(s->prec, &s->geom, s->f, s->dx, cp, mt, pts_in_e ? s->d_pts : NULL,
+                                   pts_in_e ? (int)s->n_pts : 0, s->d_step, s->st))
+            return -1;
         if (mark(s, 3) || mark(s, 4) || mark(s, 5)) return -1;
         if (s->profiling) s->ev_used += 6;
         if (s->has_mat && s->mat.n_tensor &&

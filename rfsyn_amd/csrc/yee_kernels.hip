@@ -156,7 +156,8 @@ template <typename T, typename TagT, bool CPML>
 __global__ void __launch_bounds__(256)
 k_update_e(rf_geom_t gg, T *__restrict__ ex, T *__restrict__ ey, T *__restrict__ ez,
            const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
-           T dx, rf_cpml_t cp, rf_mat_t m) {
+           T dx, rf_cpml_t cp, rf_mat_t m, const rf_point_src_t *__restrict__ pts, int n_pts,
+           const long long *__restrict__ d_step) {
     const Geo g = geo_of(gg);
     int k, j, i;
     grid_node(k, j, i);
@@ -198,6 +199,21 @@ k_update_e(rf_geom_t gg, T *__restrict__ ex, T *__restrict__ ey, T *__restrict__
         hy_i = hy[idx + di];
         hx_j = hx[idx + dj];
         e[2] = ez[idx];
+    }
+    // the point sources on E^n first, in insertion order (k_point_sources;
+    // the host passes them only when each sits on a component this thread
+    // updates)
+    if (n_pts > 0) {
+        const long long step = *d_step;
+        for (int q = 0; q < n_pts; q++) {
+            const rf_point_src_t &p = pts[q];
+            if (p.idx == idx) {
+                const double v = ((gmem_c<double> *)p.wave)[step];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    if (p.comp == c) e[c] = p.hard ? static_cast<T>(v) : static_cast<T>(static_cast<double>(e[c]) + v);
+            }
+        }
     }
     Term<T> ta[3] = {}, tb[3] = {};
     if (CPML) {
@@ -837,29 +853,30 @@ int launch_h(const rf_geom_t *g, void *const f[6], double cb, double dx, const r
 
 template <typename T, typename TagT>
 int launch_e_tag(const rf_geom_t *g, T *const F[6], double dx, const rf_cpml_t *cp,
-                 const rf_mat_t *m, hipStream_t st) {
+                 const rf_mat_t *m, const rf_point_src_t *pts, int n_pts, const long long *d_step,
+                 hipStream_t st) {
     dim3 blk(64, 4, 1);
     dim3 grd((g->n[2] + 63) / 64, (g->n[1] + 3) / 4, g->n[0]);
     rf_cpml_t none{};
     if (cp)
         hipLaunchKernelGGL((k_update_e<T, TagT, true>), grd, blk, 0, st, g[0], F[0], F[1], F[2], F[3],
-                           F[4], F[5], static_cast<T>(dx), *cp, *m);
+                           F[4], F[5], static_cast<T>(dx), *cp, *m, pts, n_pts, d_step);
     else
         hipLaunchKernelGGL((k_update_e<T, TagT, false>), grd, blk, 0, st, g[0], F[0], F[1], F[2],
-                           F[3], F[4], F[5], static_cast<T>(dx), none, *m);
+                           F[3], F[4], F[5], static_cast<T>(dx), none, *m, pts, n_pts, d_step);
     return launched();
 }
 
 template <typename T>
 int launch_e(const rf_geom_t *g, void *const f[6], double dx, const rf_cpml_t *cp,
-             const rf_mat_t *m, hipStream_t st) {
+             const rf_mat_t *m, const rf_point_src_t *pts, int n_pts, const long long *d_step, hipStream_t st) {
     T *F[6];
     for (int q = 0; q < 6; q++) F[q] = static_cast<T *>(f[q]);
-    if (!m->tag) return launch_e_tag<T, NoTag>(g, F, dx, cp, m, st);
+    if (!m->tag) return launch_e_tag<T, NoTag>(g, F, dx, cp, m, pts, n_pts, d_step, st);
     switch (m->tag_bytes) {
-    case 1: return launch_e_tag<T, uint8_t>(g, F, dx, cp, m, st);
-    case 2: return launch_e_tag<T, uint16_t>(g, F, dx, cp, m, st);
-    case 4: return launch_e_tag<T, uint32_t>(g, F, dx, cp, m, st);
+    case 1: return launch_e_tag<T, uint8_t>(g, F, dx, cp, m, pts, n_pts, d_step, st);
+    case 2: return launch_e_tag<T, uint16_t>(g, F, dx, cp, m, pts, n_pts, d_step, st);
+    case 4: return launch_e_tag<T, uint32_t>(g, F, dx, cp, m, pts, n_pts, d_step, st);
     }
     return -1;
 }
@@ -933,8 +950,15 @@ int rf_launch_update_h(int prec, const rf_geom_t *g, void *const f[6], double cb
 
 int rf_launch_update_e(int prec, const rf_geom_t *g, void *const f[6], double dx,
                        const rf_cpml_t *cpml, const rf_mat_t *mat, rf_stream_t *s) {
-    return prec == 0 ? launch_e<double>(g, f, dx, cpml, mat, hs(s))
-                     : launch_e<float>(g, f, dx, cpml, mat, hs(s));
+    return rf_launch_update_e_src(prec, g, f, dx, cpml, mat, NULL, 0, NULL, s);
+}
+
+int rf_launch_update_e_src(int prec, const rf_geom_t *g, void *const f[6], double dx, const rf_cpml_t *cpml,
+                           const rf_mat_t *mat, const rf_point_src_t *pts, int n_pts, const long long *d_step,
+                           rf_stream_t *s) {
+    if (n_pts > 0 && (!pts || !d_step)) return -1;
+    return prec == 0 ? launch_e<double>(g, f, dx, cpml, mat, pts, n_pts, d_step, hs(s))
+                     : launch_e<float>(g, f, dx, cpml, mat, pts, n_pts, d_step, hs(s));
 }
 
 int rf_launch_tensor(int prec, const rf_geom_t *g, void *const f[6], const void *edges,
