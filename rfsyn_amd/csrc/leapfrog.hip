@@ -1983,16 +1983,17 @@ k_ade_pass(rf_geom_t gg, int lo0, int lo1, int lo2, unsigned nj, unsigned nk, un
            const T *__restrict__ exa, const T *__restrict__ eya, const T *__restrict__ eza, T *__restrict__ exb,
            T *__restrict__ eyb, T *__restrict__ ezb, rf_mat_t m, int ncache, unsigned ade_blocks, rf_caps_t cs,
            int cap_ring, const long long *d_step) {
-    // the blocks past the ADE range capture the plane monitors of the step
-    // (their planes lie outside the ADE box, so their values are final);
-    // the ADE blocks are renumbered per XCD among themselves (xcd_block),
-    // the capture blocks stay dealt over all XCDs
-    if (blockIdx.x >= ade_blocks) {
-        plane_capture_elem<T>(gg, cs, (long long)(blockIdx.x - ade_blocks) * blockDim.x + threadIdx.x, cap_ring,
-                              *d_step);
+    // the first gridDim.x - ade_blocks blocks capture the plane monitors of
+    // the step (their planes lie outside the ADE box, so their values are
+    // final): one-value gathers, dispatched first so that they overlap the
+    // streaming ADE blocks instead of trailing them; the ADE blocks are
+    // renumbered per XCD among themselves (xcd_block)
+    const unsigned cap_blocks = gridDim.x - ade_blocks;
+    if (blockIdx.x < cap_blocks) {
+        plane_capture_elem<T>(gg, cs, (long long)blockIdx.x * blockDim.x + threadIdx.x, cap_ring, *d_step);
         return;
     }
-    const unsigned b = xcd_block(blockIdx.x, ade_blocks);
+    const unsigned b = xcd_block(blockIdx.x - cap_blocks, ade_blocks);
     ade_pass_block<T, TagT, NS>(gg, lo0, lo1, lo2, nj, nk, count, exa, eya, eza, exb, eyb, ezb, m, ncache, b,
                                 ade_blocks);
 }
