@@ -1541,50 +1541,33 @@ k_shell_h_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *
                           jb.cp, jb.zs);
 }
 
-// The first SHELL_LDS_ENTRIES entries of the material table, staged in LDS
-// by the block (every thread takes part, before any returns); returns how
-// many the block holds.
-constexpr int SHELL_LDS_ENTRIES = 64;
-template <typename T, typename TagT>
-__device__ __forceinline__ int shell_stage_entries(const rf_mat_t &m, typename EntryOf<T>::type *s_ent) {
-    if constexpr (std::is_same<TagT, NoTag>::value) {
-        return 0;
-    } else {
-        using Entry = typename EntryOf<T>::type;
-        const int n = m.n_entries < SHELL_LDS_ENTRIES ? m.n_entries : SHELL_LDS_ENTRIES;
-        const Entry *ents = static_cast<const Entry *>(m.entries);
-        for (int e = threadIdx.x; e < n; e += blockDim.x) s_ent[e] = ents[e];
-        __syncthreads();
-        return n;
-    }
-}
-
 // E update of one component on a shell node (e_component of the
 // two-kernel path, returning the value).
 template <typename T, typename TagT>
 __device__ __forceinline__ T shell_e_value(int c, T e_old, T dA, T dB, const rf_mat_t &m, unsigned tag, int i,
-                                           int j, int k, const typename EntryOf<T>::type *s_ent, int n_lds) {
+                                           int j, int k) {
     using Entry = typename EntryOf<T>::type;
     using Ade = typename EntryOf<T>::ade;
     if constexpr (std::is_same<TagT, NoTag>::value) {
         return T(1) * e_old + static_cast<T>(m.vac_ca) * (dA - dB);
     } else {
-        // LDS reads only when every active lane's entry is staged (one
-        // merged path for LDS and HBM entries compiles to FLAT loads, which
-        // also hold the vector memory counter)
+        // the entry read straight from the table: with one tag for every
+        // active lane (nearly every wavefront of the shell) through the
+        // scalar cache, no block-wide staging and barrier ahead of the nodes
+        typedef const __attribute__((address_space(1))) Entry g_entry;
+        const g_entry *ents = (const g_entry *)m.entries;
+        const unsigned t0 = __builtin_amdgcn_readfirstlane(tag);
         uint32_t fl;
         T ca, cb;
         int aid;
-        if (__ballot(static_cast<int>(tag) >= n_lds) == 0) {
-            typedef __attribute__((address_space(3))) const Entry lds_entry;
-            const lds_entry &en = ((lds_entry *)s_ent)[tag];
+        if (__ballot(tag != t0) == 0) {
+            const g_entry &en = ents[t0];
             fl = en.flags[c];
             ca = en.Ca[c];
             cb = en.Cb[c];
             aid = en.ade[c];
         } else {
-            const Entry &en =
-                static_cast<int>(tag) < n_lds ? s_ent[tag] : static_cast<const Entry *>(m.entries)[tag];
+            const g_entry &en = ents[tag];
             fl = en.flags[c];
             ca = en.Ca[c];
             cb = en.Cb[c];
@@ -1608,8 +1591,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
                                              const T *__restrict__ hy_p, const T *__restrict__ hz_p,
                                              T *__restrict__ exb_p, T *__restrict__ eyb_p, T *__restrict__ ezb_p,
                                              T dx, const rf_cpml_t &cp, const rf_zpsi_t &zs, const rf_mat_t &m,
-                                             const rf_leap_src_t *srcs, int n_srcs, const long long *d_step,
-                                             const typename EntryOf<T>::type *s_ent, int n_lds) {
+                                             const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
     gmem_c<T> *const exa = (gmem_c<T> *)exa_p, *const eya = (gmem_c<T> *)eya_p, *const eza = (gmem_c<T> *)eza_p;
     gmem_c<T> *const hx = (gmem_c<T> *)hx_p, *const hy = (gmem_c<T> *)hy_p, *const hz = (gmem_c<T> *)hz_p;
     gmem<T> *const exb = (gmem<T> *)exb_p, *const eyb = (gmem<T> *)eyb_p, *const ezb = (gmem<T> *)ezb_p;
@@ -1679,7 +1661,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
             dA = term_apply<T>(ta[0], dA, dx);
             dB = term_apply<T>(tb[0], dB, dx);
         }
-        e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k, s_ent, n_lds);
+        e0 = shell_e_value<T, TagT>(0, e0, dA, dB, m, tag, i, j, k);
     }
     if (do_y) {   // materials.c:1015-1040
         T dA = hxc - hx_k;
@@ -1688,7 +1670,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
             dA = term_apply<T>(ta[1], dA, dx);
             dB = term_apply<T>(tb[1], dB, dx);
         }
-        e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k, s_ent, n_lds);
+        e1 = shell_e_value<T, TagT>(1, e1, dA, dB, m, tag, i, j, k);
     }
     if (do_z) {   // materials.c:1050-1058
         T dA = hyc - hy_i;
@@ -1697,7 +1679,7 @@ __device__ __forceinline__ void shell_e_node(const rf_geom_t &gg, const rf_shell
             dA = term_apply<T>(ta[2], dA, dx);
             dB = term_apply<T>(tb[2], dB, dx);
         }
-        e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k, s_ent, n_lds);
+        e2 = shell_e_value<T, TagT>(2, e2, dA, dB, m, tag, i, j, k);
     }
     if (CPML) {
         if (do_x) {
@@ -1729,11 +1711,8 @@ k_shell_e(rf_geom_t gg, rf_shell_t sh, const T *__restrict__ exa, const T *__res
           const T *__restrict__ eza, const T *__restrict__ hx, const T *__restrict__ hy, const T *__restrict__ hz,
           T *__restrict__ exb, T *__restrict__ eyb, T *__restrict__ ezb, T dx, rf_cpml_t cp, rf_zpsi_t zs,
           rf_mat_t m, const rf_leap_src_t *srcs, int n_srcs, const long long *d_step) {
-    __shared__ typename EntryOf<T>::type s_ent[SHELL_LDS_ENTRIES];
-    const int n_lds = shell_stage_entries<T, TagT>(m, s_ent);
     shell_e_node<T, TagT, CPML>(gg, sh, (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, exa,
-                                eya, eza, hx, hy, hz, exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step, s_ent,
-                                n_lds);
+                                eya, eza, hx, hy, hz, exb, eyb, ezb, dx, cp, zs, m, srcs, n_srcs, d_step);
 }
 
 // The shell E updates of many scenes (block b, renumbered per XCD: job
@@ -1746,11 +1725,9 @@ k_shell_e_packed(const rf_shell_job_t *__restrict__ jobs, const unsigned short *
     const T *const *fa = reinterpret_cast<const T *const *>(jb.fa);
     const T *const *hb = reinterpret_cast<const T *const *>(jb.hb);
     T *const *eb = reinterpret_cast<T *const *>(jb.eb);
-    __shared__ typename EntryOf<T>::type s_ent[SHELL_LDS_ENTRIES];
-    const int n_lds = shell_stage_entries<T, TagT>(jb.m, s_ent);
     shell_e_node<T, TagT, CPML>(jb.g, jb.sh, (long long)(b - jb.block0) * blockDim.x + threadIdx.x, fa[0], fa[1],
                                 fa[2], hb[0], hb[1], hb[2], eb[0], eb[1], eb[2], static_cast<T>(jb.dx), jb.cp, jb.zs,
-                                jb.m, jb.srcs, jb.n_srcs, jb.d_step, s_ent, n_lds);
+                                jb.m, jb.srcs, jb.n_srcs, jb.d_step);
 }
 
 template <typename T>
