@@ -4,7 +4,8 @@ update writes and no plane source runs between them; otherwise (a source on
 a boundary node the update skips, a plane source before the E update) they
 keep their own kernel.  Either way the step is source_apply(n dt) followed
 by the E update (the reference's order): FP64 fields and probe series
-bitwise the oracle's."""
+bitwise the oracle's.  On the fused step, sources on the CPML slab nodes
+are applied by the shell sweeps, in the wavefronts that hold one."""
 import numpy as np
 import pytest
 
@@ -12,6 +13,8 @@ from rfsyn_amd import capi
 from rfsyn_amd.driver import NativeRun, c1_cpml_params, c1_freqs, modgauss_params, same_values
 
 pytestmark = pytest.mark.gpu
+
+ORACLE_LIB = [None]   # the oracle library (it has no fused step to report)
 
 WAVE = ("source_waveform_modulated_gaussian", modgauss_params(30e9, 20e9))
 
@@ -37,16 +40,46 @@ def _run(lib, sources, plane=False, steps=40):
     p = run.add_probe(capi.FIELD_EZ, (12, 11, 10), steps, c1_freqs(8))
     run.run(steps)
     out = (run.td(p, steps), run.fields())
+    fused = run.lib.fdtd_sim_fused(run.sim) if lib is not ORACLE_LIB[0] else None
     run.close()
-    return out
+    return out + (fused,)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("plane", [False, True])
 def test_two_kernel_point_sources_fp64_vs_oracle(gpu, oracle, case, plane, monkeypatch):
     monkeypatch.setenv("RFSYN_FUSED", "0")
-    td, fields = _run(gpu, CASES[case], plane)
-    td_ref, fields_ref = _run(oracle, CASES[case], plane)
+    ORACLE_LIB[0] = oracle
+    td, fields, fused = _run(gpu, CASES[case], plane)
+    td_ref, fields_ref, _ = _run(oracle, CASES[case], plane)
+    assert fused == 0
+    assert np.max(np.abs(td_ref)) > 0
+    assert same_values(td, td_ref)
+    for k in fields_ref:
+        assert same_values(fields[k], fields_ref[k]), k
+
+
+# ------------------------------------------------------- fused step, shell
+
+SHELL_CASES = {
+    # soft Ez in the low-x slab, hard Ex in the high-y slab (both updated
+    # by the shell sweeps, which apply sources only in wavefronts holding one)
+    "slabs": ([(capi.FIELD_EZ, (2, 10, 10), False), (capi.FIELD_EX, (10, 17, 9), True)], False),
+    # and a plane source across the box and the slabs
+    "slabs_and_plane": ([(capi.FIELD_EZ, (2, 10, 10), False), (capi.FIELD_EX, (10, 17, 9), True)], True),
+}
+
+
+@pytest.mark.parametrize("case", sorted(SHELL_CASES))
+def test_fused_step_shell_sources_fp64_vs_oracle(gpu, oracle, case, monkeypatch):
+    """Sources on CPML slab nodes, which the shell kernels update around the
+    fused box: FP64 bitwise the oracle."""
+    monkeypatch.setenv("RFSYN_FUSED", "1")
+    sources, plane = SHELL_CASES[case]
+    ORACLE_LIB[0] = oracle
+    td, fields, fused = _run(gpu, sources, plane)
+    td_ref, fields_ref, _ = _run(oracle, sources, plane)
+    assert fused == 1
     assert np.max(np.abs(td_ref)) > 0
     assert same_values(td, td_ref)
     for k in fields_ref:
